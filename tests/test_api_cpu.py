@@ -1,0 +1,194 @@
+"""CPU tests of the C-ABI library: it loads, exports every declared symbol,
+keeps the reference's versioned-struct and error semantics, and lowers graphs
+exactly as the reference's GraphBuilder (traces of graphbuilder_test.cpp).
+No GPU work is issued here."""
+import ctypes
+import json
+import os
+import re
+import subprocess
+
+import pytest
+
+import zimg_amd as z
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "zimg.h")).read()
+    declared = set(re.findall(r"\b(zimg_[a-z0-9_]+)\s*\(", header)) - {"zimg_filter_graph_callback"}
+    assert declared == set(z.EXPORTS)
+    out = subprocess.run(["nm", "-D", "--defined-only", z.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = declared - exported
+    assert not missing, missing
+    # nothing else of the C++ implementation leaks into the ABI
+    assert all(s.startswith("zimg_") for s in exported if not s.startswith("_")), sorted(exported)[:20]
+
+
+def test_version():
+    L = z.lib()
+    a, b, c = ctypes.c_uint(), ctypes.c_uint(), ctypes.c_uint()
+    L.zimg_get_version_info(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    assert (a.value, b.value, c.value) == (3, 0, 6)
+    assert L.zimg_get_api_version(ctypes.byref(a), ctypes.byref(b)) == z.API_VERSION
+    assert (a.value, b.value) == (2, 4)
+
+
+def test_select_buffer_mask():
+    L = z.lib()
+    assert L.zimg_select_buffer_mask(0) == 0
+    assert L.zimg_select_buffer_mask(1) == 0
+    assert L.zimg_select_buffer_mask(2) == 1
+    assert L.zimg_select_buffer_mask(3) == 3
+    assert L.zimg_select_buffer_mask(4) == 3
+    assert L.zimg_select_buffer_mask(5) == 7
+    assert L.zimg_select_buffer_mask(0xFFFFFFFF) == 0xFFFFFFFF
+
+
+def _poison(struct, start_field):
+    off = getattr(type(struct), start_field).offset
+    raw = (ctypes.c_ubyte * ctypes.sizeof(struct)).from_buffer(struct)
+    for i in range(off, ctypes.sizeof(struct)):
+        raw[i] = 0xCC
+    return off, raw
+
+
+@pytest.mark.parametrize("version,cls,field", [
+    ((2 << 8) | 0, z.ImageFormatStruct, "active_region"),
+    ((2 << 8) | 3, z.ImageFormatStruct, "alpha"),
+    ((2 << 8) | 1, z.GraphBuilderParamsStruct, "nominal_peak_luminance"),
+    ((2 << 8) | 4, z.GraphBuilderParamsStruct, "scene_referred"),
+])
+def test_default_never_writes_past_version(version, cls, field):
+    """api_test.cpp:8-88: defaulting a struct of an older version leaves newer fields untouched."""
+    s = cls()
+    off, raw = _poison(s, field)
+    if cls is z.ImageFormatStruct:
+        z.lib().zimg_image_format_default(ctypes.byref(s), version)
+    else:
+        z.lib().zimg_graph_builder_params_default(ctypes.byref(s), version)
+    assert s.version == version
+    assert all(raw[i] == 0xCC for i in range(off, ctypes.sizeof(s)))
+
+
+def test_api_2_0_ignores_active_region():
+    """An API 2.0 caller's garbage active region must not be read (api_test.cpp:8-40)."""
+    f = z.ImageFormatStruct()
+    z.lib().zimg_image_format_default(ctypes.byref(f), (2 << 8) | 0)
+    f.width, f.height, f.pixel_type = 640, 480, z.PIXEL_BYTE
+    f.active_region.width = float("-inf")
+    f.active_region.height = float("-inf")
+    p = z.GraphBuilderParamsStruct()
+    z.lib().zimg_graph_builder_params_default(ctypes.byref(p), (2 << 8) | 0)
+    g = z.FilterGraph(f, f, p)
+    assert g.handle
+
+
+def test_defaults_match_reference():
+    f = z.image_format()
+    assert (f.color_family, f.matrix_coefficients, f.transfer_characteristics, f.color_primaries) == (
+        z.COLOR_GREY, z.MATRIX_UNSPECIFIED, z.TRANSFER_UNSPECIFIED, z.PRIMARIES_UNSPECIFIED)
+    assert f.pixel_range == z.RANGE_LIMITED and f.chroma_location == z.CHROMA_LEFT and f.alpha == z.ALPHA_NONE
+    p = z.builder_params()
+    assert p.resample_filter == z.RESIZE_BICUBIC and p.resample_filter_uv == z.RESIZE_BILINEAR
+    assert p.dither_type == z.DITHER_NONE and p.cpu_type == z.CPU_AUTO
+
+
+def _err_of(src, dst, params=None):
+    with pytest.raises(z.ZimgError) as ei:
+        z.FilterGraph(src, dst, params)
+    return ei.value.code
+
+
+def test_error_codes():
+    base = dict(width=64, height=48, pixel_type=z.PIXEL_BYTE)
+    ok = z.image_format(**base)
+    assert _err_of(z.image_format(**base, subsample_w=1), ok) == z.ERROR_GREYSCALE_SUBSAMPLING
+    assert _err_of(z.image_format(**base, matrix_coefficients=z.MATRIX_RGB), ok) == z.ERROR_COLOR_FAMILY_MISMATCH
+    assert _err_of(z.image_format(**dict(base, width=63), color_family=z.COLOR_YUV, subsample_w=1), ok) == z.ERROR_IMAGE_NOT_DIVISIBLE
+    assert _err_of(z.image_format(**base, depth=9), ok) == z.ERROR_BIT_DEPTH_OVERFLOW
+    assert _err_of(z.image_format(**base, pixel_type=7) if False else z.image_format(width=64, height=48, pixel_type=7), ok) == z.ERROR_ENUM_OUT_OF_RANGE
+    assert _err_of(z.image_format(**dict(base, width=0)), ok) == z.ERROR_INVALID_IMAGE_SIZE
+    assert _err_of(z.image_format(**base, color_family=z.COLOR_RGB, subsample_w=1), ok) == z.ERROR_UNSUPPORTED_SUBSAMPLING
+    assert _err_of(z.image_format(**base, color_family=z.COLOR_YUV, subsample_w=3), ok) == z.ERROR_UNSUPPORTED_SUBSAMPLING
+    assert _err_of(ok, z.image_format(**base, active_left=1.0)) == z.ERROR_RESAMPLING_NOT_AVAILABLE
+    bad_matrix = z.image_format(**base, color_family=z.COLOR_YUV, matrix_coefficients=3)
+    assert _err_of(bad_matrix, z.image_format(**base, color_family=z.COLOR_YUV, matrix_coefficients=z.MATRIX_BT709)) == \
+        z.ERROR_NO_COLORSPACE_CONVERSION
+    # last error is thread-local and sticky until cleared
+    code, msg = z.last_error()
+    assert code == z.ERROR_NO_COLORSPACE_CONVERSION and msg
+    z.lib().zimg_clear_last_error()
+    assert z.last_error() == (0, "")
+
+
+def _fmt_from_spec(spec):
+    base = spec["base"]
+    f = z.image_format(width=64, height=48, pixel_type=z.PIXEL_FLOAT, pixel_range=z.RANGE_LIMITED, chroma_location=z.CHROMA_CENTER,
+                       color_family=z.COLOR_RGB if base == "rgb" else z.COLOR_YUV,
+                       matrix_coefficients=z.MATRIX_RGB if base == "rgb" else z.MATRIX_BT709,
+                       transfer_characteristics=z.TRANSFER_BT709, color_primaries=z.PRIMARIES_BT709)
+    for k, v in spec.items():
+        if k == "base":
+            continue
+        if k.startswith("active_"):
+            setattr(f.active_region, k[len("active_"):], v)
+        else:
+            setattr(f, k, v)
+    return f
+
+
+CASES = json.load(open(os.path.join(HERE, "golden", "graphbuilder_traces.json")))
+
+
+@pytest.mark.parametrize("case", CASES["cases"], ids=[c["name"] for c in CASES["cases"]])
+def test_graph_lowering_matches_reference_trace(case):
+    src, dst = _fmt_from_spec(case["src"]), _fmt_from_spec(case["dst"])
+    g = z.FilterGraph(src, dst)
+    trace = g.plan_trace()
+    expected = case["trace"]
+    same_csp = all(getattr(src, k) == getattr(dst, k) for k in
+                   ("color_family", "matrix_coefficients", "transfer_characteristics", "color_primaries"))
+    if same_csp:
+        # The reference test drives GraphBuilder directly; through the C API an
+        # identical colourspace pair is canonicalised to "unspecified"
+        # (zimg.cpp:414-431), so no YCgCo range flag can be set.
+        expected = [t.replace(" ycgco", "") for t in expected]
+    assert len(trace) == len(expected), trace
+    for got, want in zip(trace, expected):
+        assert got.startswith(want), (got, want)
+
+
+def test_unsupported_alpha_premultiplication_is_reported():
+    src = _fmt_from_spec({"base": "rgb", "alpha": 1})
+    dst = _fmt_from_spec({"base": "rgb", "alpha": 0})
+    assert _err_of(src, dst) == z.ERROR_UNSUPPORTED_OPERATION
+
+
+def test_device_tmp_size_scales_with_frames():
+    src = z.image_format(width=1920, height=1080, pixel_type=z.PIXEL_WORD, depth=10, subsample_w=1, subsample_h=1,
+                         color_family=z.COLOR_YUV, matrix_coefficients=z.MATRIX_BT709,
+                         transfer_characteristics=z.TRANSFER_BT709, color_primaries=z.PRIMARIES_BT709)
+    dst = z.image_format(width=1920, height=1080, pixel_type=z.PIXEL_WORD, depth=10, subsample_w=1, subsample_h=1,
+                         color_family=z.COLOR_YUV, matrix_coefficients=z.MATRIX_BT2020_NCL,
+                         transfer_characteristics=z.TRANSFER_BT709, color_primaries=z.PRIMARIES_BT2020)
+    g = z.FilterGraph(src, dst, z.builder_params(resample_filter_uv=z.RESIZE_BICUBIC))
+    one, four = g.device_tmp_size(1), g.device_tmp_size(4)
+    assert one > 0 and four == 4 * one
+    # fused lowering: one launch per chroma resize pair, one colourspace launch
+    dev = g.describe().split("device:\n")[1]
+    lines = [l for l in dev.splitlines() if l]
+    assert sum(l.startswith("colorspace") for l in lines) == 1 and "quantize[0]" in dev
+    assert len(lines) == 5
+
+
+def test_single_plane_lanczos_is_one_fused_launch():
+    src = z.image_format(width=1920, height=1080, pixel_type=z.PIXEL_WORD, depth=16)
+    dst = z.image_format(width=3840, height=2160, pixel_type=z.PIXEL_WORD, depth=16)
+    g = z.FilterGraph(src, dst, z.builder_params(resample_filter=z.RESIZE_LANCZOS, filter_param_a=3))
+    dev = [l for l in g.describe().split("device:\n")[1].splitlines() if l]
+    assert len(dev) == 1 and "passes=2 h_first" in dev[0]
+    assert g.device_tmp_size(8) == 0

@@ -1,0 +1,226 @@
+"""GPU parity: the HIP path through the C ABI against the oracle, bit for bit.
+
+Integer outputs must be identical; float outputs are compared bit for bit too
+(the kernels evaluate the reference's C expressions in the same order without
+contraction), which is stricter than the 1e-5 relative tolerance."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import pipelines as P
+import zimg_amd as z
+from oracle_lib import (BICUBIC, BILINEAR, BYTE, FLOAT, LANCZOS, POINT, SPLINE16, SPLINE36, SPLINE64, WORD, dither,
+                        left_shift, resize_plane, scale_offset)
+from zimg_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+gpu_util = pytest.importorskip("gpu_util")
+
+
+def _word_graph(sw, sh, dw, dh, ftype=z.RESIZE_LANCZOS, a=3.0, b=float("nan"), depth=16):
+    src = z.image_format(width=sw, height=sh, pixel_type=z.PIXEL_WORD, depth=depth)
+    dst = z.image_format(width=dw, height=dh, pixel_type=z.PIXEL_WORD, depth=depth)
+    return z.FilterGraph(src, dst, z.builder_params(resample_filter=ftype, filter_param_a=a, filter_param_b=b))
+
+
+def _assert_equal(got, want, what):
+    assert got.shape == want.shape, what
+    if got.dtype == np.float32:
+        bad = np.count_nonzero(got.view(np.uint32) != want.view(np.uint32))
+    else:
+        bad = np.count_nonzero(got != want)
+    if bad:
+        idx = np.argwhere(got != want)[:5]
+        pytest.fail(f"{what}: {bad} mismatches, first at {idx.tolist()}: got {got[tuple(idx[0])]} want {want[tuple(idx[0])]}")
+
+
+def test_uhd_lanczos_word_bit_exact():
+    """1920x1080 WORD -> 3840x2160 Lanczos3 (fused H+V launch) == reference C path."""
+    print("case: WORD 1920x1080->3840x2160 lanczos3", flush=True)
+    src = W.word_plane(1920, 1080, seed=1)
+    g = _word_graph(1920, 1080, 3840, 2160)
+    (out, *_) = gpu_util.run_graph(g, [src, None, None, None], [(2160, 3840, np.uint16), None, None, None])
+    _assert_equal(out, P.word_resize(src, 3840, 2160), "uhd lanczos")
+
+
+WORD_CASES = [
+    (640, 480, 1280, 960, LANCZOS, 3.0, 0.0, 16),
+    (640, 480, 320, 240, LANCZOS, 3.0, 0.0, 16),
+    (641, 479, 1000, 333, BICUBIC, 0.0, 0.5, 16),
+    (333, 1000, 641, 479, SPLINE36, 0.0, 0.0, 10),
+    (500, 300, 500, 719, BILINEAR, 0.0, 0.0, 12),   # vertical only
+    (500, 300, 1217, 300, SPLINE16, 0.0, 0.0, 16),  # horizontal only
+    (96, 64, 1920, 1080, SPLINE64, 0.0, 0.0, 16),   # large upscale
+    (1920, 1080, 200, 120, LANCZOS, 4.0, 0.0, 16),  # ~10x downscale, wide filters
+    (64, 48, 97, 31, POINT, 0.0, 0.0, 16),
+]
+
+
+@pytest.mark.parametrize("sw,sh,dw,dh,ft,a,b,depth", WORD_CASES)
+def test_word_resize_grid(sw, sh, dw, dh, ft, a, b, depth):
+    print(f"case: WORD {sw}x{sh}->{dw}x{dh} filter {ft} depth {depth}", flush=True)
+    src = (W.word_plane(sw, sh, seed=sw * 7 + dh) >> (16 - depth)).astype(np.uint16)
+    g = _word_graph(sw, sh, dw, dh, ft, a, b if ft == BICUBIC else float("nan"), depth)
+    for pad in (0, 100):
+        (out, *_) = gpu_util.run_graph(g, [src, None, None, None], [(dh, dw, np.uint16), None, None, None], pad_bytes=pad)
+        _assert_equal(out, P.word_resize(src, dw, dh, ft, a, b, depth), f"word resize pad={pad}")
+
+
+FLOAT_CASES = [
+    (3840, 2160, 1920, 1080, LANCZOS, 3.0, 0.0),
+    (1280, 720, 3840, 2160, BICUBIC, 0.0, 0.5),
+    (777, 555, 1111, 333, SPLINE36, 0.0, 0.0),
+]
+
+
+@pytest.mark.parametrize("sw,sh,dw,dh,ft,a,b", FLOAT_CASES)
+def test_float_resize_bit_exact(sw, sh, dw, dh, ft, a, b):
+    print(f"case: FLOAT {sw}x{sh}->{dw}x{dh} filter {ft}", flush=True)
+    src = W.fp32_plane(sw, sh, seed=3)
+    s, d, p = W.fp32_formats(sw, sh, dw, dh, ft, a, b if ft == BICUBIC else float("nan"))
+    g = z.FilterGraph(s, d, p)
+    (out, *_) = gpu_util.run_graph(g, [src, None, None, None], [(dh, dw, np.float32), None, None, None])
+    _assert_equal(out, P.float_resize(src, dw, dh, ft, a, b), "float resize")
+
+
+def test_fhd_709_to_2020_bit_exact():
+    """FHD 4:2:0 10-bit BT.709 -> BT.2020 NCL: 5 fused launches == reference chain."""
+    print("case: FHD 4:2:0 10-bit 709->2020", flush=True)
+    Y, U, V = W.yuv420_10bit(1920, 1080, seed=2)
+    s, d, p = W.fhd_709_to_2020_formats()
+    g = z.FilterGraph(s, d, p)
+    outs = gpu_util.run_graph(g, [Y, U, V, None], [(1080, 1920, np.uint16), (540, 960, np.uint16), (540, 960, np.uint16), None])
+    want = P.yuv420_10_709_to_2020(Y, U, V)
+    for got, ref, name in zip(outs[:3], want, "YUV"):
+        _assert_equal(got, ref, f"709->2020 plane {name}")
+
+
+@pytest.mark.parametrize("w,h", [(640, 360), (3840, 2160)])
+def test_pq_to_hlg_error_diffusion_bit_exact(w, h):
+    """RGB FP32 PQ -> HLG -> 12-bit Floyd-Steinberg (row-skewed wavefront) == sequential reference."""
+    print(f"case: PQ->HLG ED {w}x{h}", flush=True)
+    R, G, B = W.pq_rgb(w, h, seed=4)
+    s, d, p = W.uhd_pq_to_hlg_formats(w, h)
+    g = z.FilterGraph(s, d, p)
+    outs = gpu_util.run_graph(g, [R, G, B, None], [(h, w, np.uint16)] * 3 + [None])
+    want = P.rgb_pq_to_hlg_12bit_ed(R, G, B)
+    for got, ref, name in zip(outs[:3], want, "RGB"):
+        _assert_equal(got, ref, f"pq->hlg plane {name}")
+
+
+def test_batch_equals_single_frames():
+    print("case: batch of 5 frames vs single", flush=True)
+    frames = np.stack([W.word_plane(480, 270, seed=k) for k in range(5)])
+    g = _word_graph(480, 270, 960, 540)
+    batch, *_ = gpu_util.run_graph(g, [frames, None, None, None], [(540, 960, np.uint16), None, None, None], pad_bytes=64)
+    for k in range(5):
+        _assert_equal(batch[k], P.word_resize(frames[k], 960, 540), f"frame {k}")
+
+
+def _yuv420_byte(w, h, seed):
+    r = np.random.default_rng(seed)
+    Y = r.integers(16, 236, (h, w)).astype(np.uint8)
+    U = r.integers(16, 241, (h // 2, w // 2)).astype(np.uint8)
+    V = r.integers(16, 241, (h // 2, w // 2)).astype(np.uint8)
+    return Y, U, V
+
+
+@pytest.mark.parametrize("src_type", ["byte", "word10"])
+def test_yuv420_resize_ordered_dither_to_byte(src_type):
+    """4:2:0 u8 or 10-bit -> resized -> 8-bit with Bayer ordered dither (batched workload chain)."""
+    print(f"case: 4:2:0 {src_type} 1280x720->1920x1080 spline36 ordered dither", flush=True)
+    sw, sh, dw, dh = 1280, 720, 1920, 1080
+    Y8, U8, V8 = _yuv420_byte(sw, sh, 5)
+    if src_type == "byte":
+        planes, fmt_t, depth = (Y8, U8, V8), z.PIXEL_BYTE, 8
+    else:
+        planes, fmt_t, depth = tuple((p.astype(np.uint16) << 2) | 1 for p in (Y8, U8, V8)), z.PIXEL_WORD, 10
+    common = dict(subsample_w=1, subsample_h=1, color_family=z.COLOR_YUV, matrix_coefficients=z.MATRIX_BT709,
+                  transfer_characteristics=z.TRANSFER_BT709, color_primaries=z.PRIMARIES_BT709)
+    s = z.image_format(width=sw, height=sh, pixel_type=fmt_t, depth=depth, **common)
+    d = z.image_format(width=dw, height=dh, pixel_type=z.PIXEL_BYTE, depth=8, **common)
+    p = z.builder_params(resample_filter=z.RESIZE_SPLINE36, resample_filter_uv=z.RESIZE_SPLINE36, dither_type=z.DITHER_ORDERED)
+    g = z.FilterGraph(s, d, p)
+    outs = gpu_util.run_graph(g, list(planes) + [None], [(dh, dw, np.uint8), (dh // 2, dw // 2, np.uint8), (dh // 2, dw // 2, np.uint8), None])
+
+    # Oracle chain (graphbuilder.cpp:721-751 promotes limited BYTE to WORD16 by left shift).
+    for q, (plane, got) in enumerate(zip(planes, outs[:3])):
+        chroma = q > 0
+        if src_type == "byte":
+            x = left_shift(plane, BYTE, WORD, 8)
+            wdepth = 16
+        else:
+            x = plane
+            wdepth = 10
+        cw_s, ch_s = plane.shape[1], plane.shape[0]
+        cw_d, ch_d = (dw // 2, dh // 2) if chroma else (dw, dh)
+        if chroma:
+            scale_w = cw_d / cw_s
+            shift_w = 0.25 - 0.25 / scale_w
+        else:
+            shift_w = 0.0
+        r = resize_plane(x, SPLINE36, 0.0, 0.0, cw_d, ch_d, shift_w=shift_w, shift_h=0.0, subwidth=float(cw_s),
+                         subheight=float(ch_s), depth=wdepth)
+        sc, of = scale_offset((WORD, wdepth, 0, int(chroma), 0), (BYTE, 8, 0, int(chroma), 0))
+        want = dither(r, WORD, BYTE, sc, of, 8, dither_type=1, plane=q)
+        _assert_equal(got, want, f"plane {q}")
+
+
+def test_host_process_with_ring_buffers_and_callbacks():
+    """zimg_filter_graph_process: host buffers, ring-buffer masks and callbacks (zimg.h:390-484)."""
+    print("case: host process with callbacks", flush=True)
+    sw, sh, dw, dh = 320, 240, 480, 360
+    Y, U, V = W.yuv420_10bit(sw, sh, seed=6)
+    s, d, p = W.fhd_709_to_2020_formats(sw, sh)
+    d.width, d.height = dw, dh
+    g = z.FilterGraph(s, d, p)
+    in_lines, out_lines = g.input_buffering(), g.output_buffering()
+    in_mask = z.lib().zimg_select_buffer_mask(in_lines)
+    out_mask = z.lib().zimg_select_buffer_mask(out_lines)
+    ring_in = [np.zeros((in_mask + 1, pl.shape[1]), np.uint16) for pl in (Y, U, V)]
+    ring_out = [np.zeros((out_mask + 1, w_), np.uint16) for w_ in (dw, dw // 2, dw // 2)]
+    result = [np.zeros((dh, dw), np.uint16), np.zeros((dh // 2, dw // 2), np.uint16), np.zeros((dh // 2, dw // 2), np.uint16)]
+    calls = {"unpack": 0, "pack": 0}
+
+    def unpack(user, i, left, right):
+        calls["unpack"] += 1
+        for q, pl in enumerate((Y, U, V)):
+            sub = 1 if q else 0
+            for r in range(i >> sub, (i + 2) >> sub):
+                ring_in[q][r & in_mask] = pl[r]
+        return 0
+
+    def pack(user, i, left, right):
+        calls["pack"] += 1
+        for q in range(3):
+            sub = 1 if q else 0
+            for r in range(i >> sub, (i + 2) >> sub):
+                result[q][r] = ring_out[q][r & out_mask]
+        return 0
+
+    ucb, pcb = z.CALLBACK(unpack), z.CALLBACK(pack)
+    src = z.image_buffer([(r.ctypes.data, r.strides[0], in_mask) for r in ring_in])
+    dst = z.image_buffer([(r.ctypes.data, r.strides[0], out_mask) for r in ring_out])
+    tmp = np.zeros(g.tmp_size(), np.uint8)
+    g.process(src, dst, tmp.ctypes.data, ucb, None, pcb, None)
+    assert calls["unpack"] == sh // 2 and calls["pack"] == dh // 2
+
+    # Same graph through the device entry point must agree.
+    outs = gpu_util.run_graph(g, [Y, U, V, None], [(dh, dw, np.uint16), (dh // 2, dw // 2, np.uint16), (dh // 2, dw // 2, np.uint16), None])
+    for q in range(3):
+        _assert_equal(result[q], outs[q], f"host vs device plane {q}")
+
+
+def test_failing_callback_reports_error():
+    print("case: failing callback", flush=True)
+    g = _word_graph(64, 48, 128, 96)
+    src_img = W.word_plane(64, 48)
+    out = np.zeros((96, 128), np.uint16)
+    src = z.image_buffer([(src_img.ctypes.data, src_img.strides[0]), None, None, None])
+    dst = z.image_buffer([(out.ctypes.data, out.strides[0]), None, None, None])
+    bad = z.CALLBACK(lambda user, i, l, r: 1)
+    with pytest.raises(z.ZimgError) as ei:
+        g.process(src, dst, None, bad, None)
+    assert ei.value.code == z.ERROR_USER_CALLBACK_FAILED

@@ -1,0 +1,272 @@
+"""Python binding of the MI355X zimg C ABI (include/zimg.h) over ctypes.
+
+It mirrors the reference's C++ wrapper (src/zimg/api/zimg++.hpp:143-252):
+``ImageFormat`` / ``GraphBuilderParams`` default themselves through the C
+library, ``FilterGraph`` owns a graph handle. The device entry points take raw
+device pointers, so any allocator works (the tests and bench use torch tensors).
+
+The library is the built ``zimg_amd/libzimg_amd.so``; importing this module
+raises if it is missing — there is no fallback implementation.
+"""
+import ctypes
+import math
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libzimg_amd.so")
+
+API_VERSION = (2 << 8) | 4
+BUFFER_MAX = 0xFFFFFFFF
+
+# enumerations (zimg.h)
+PIXEL_BYTE, PIXEL_WORD, PIXEL_HALF, PIXEL_FLOAT = 0, 1, 2, 3
+RANGE_LIMITED, RANGE_FULL = 0, 1
+COLOR_GREY, COLOR_RGB, COLOR_YUV = 0, 1, 2
+ALPHA_NONE, ALPHA_STRAIGHT, ALPHA_PREMULTIPLIED = 0, 1, 2
+FIELD_PROGRESSIVE, FIELD_TOP, FIELD_BOTTOM = 0, 1, 2
+CHROMA_LEFT, CHROMA_CENTER, CHROMA_TOP_LEFT, CHROMA_TOP, CHROMA_BOTTOM_LEFT, CHROMA_BOTTOM = range(6)
+MATRIX_RGB, MATRIX_BT709, MATRIX_UNSPECIFIED = 0, 1, 2
+MATRIX_FCC, MATRIX_BT470_BG, MATRIX_ST170_M, MATRIX_ST240_M, MATRIX_YCGCO = 4, 5, 6, 7, 8
+MATRIX_BT2020_NCL, MATRIX_BT2020_CL = 9, 10
+MATRIX_CHROMATICITY_DERIVED_NCL, MATRIX_CHROMATICITY_DERIVED_CL, MATRIX_BT2100_ICTCP = 12, 13, 14
+TRANSFER_BT709, TRANSFER_UNSPECIFIED, TRANSFER_BT470_M, TRANSFER_BT470_BG = 1, 2, 4, 5
+TRANSFER_BT601, TRANSFER_ST240_M, TRANSFER_LINEAR, TRANSFER_LOG_100, TRANSFER_LOG_316 = 6, 7, 8, 9, 10
+TRANSFER_IEC_61966_2_4, TRANSFER_BT1361, TRANSFER_IEC_61966_2_1 = 11, 12, 13
+TRANSFER_BT2020_10, TRANSFER_BT2020_12, TRANSFER_ST2084, TRANSFER_ST428, TRANSFER_ARIB_B67 = 14, 15, 16, 17, 18
+PRIMARIES_BT709, PRIMARIES_UNSPECIFIED, PRIMARIES_BT470_M, PRIMARIES_BT470_BG = 1, 2, 4, 5
+PRIMARIES_ST170_M, PRIMARIES_ST240_M, PRIMARIES_FILM, PRIMARIES_BT2020 = 6, 7, 8, 9
+PRIMARIES_ST428, PRIMARIES_ST431_2, PRIMARIES_ST432_1, PRIMARIES_EBU3213_E = 10, 11, 12, 22
+DITHER_NONE, DITHER_ORDERED, DITHER_RANDOM, DITHER_ERROR_DIFFUSION = 0, 1, 2, 3
+RESIZE_POINT, RESIZE_BILINEAR, RESIZE_BICUBIC, RESIZE_SPLINE16, RESIZE_SPLINE36, RESIZE_LANCZOS, RESIZE_SPLINE64 = 0, 1, 2, 3, 4, 5, 6
+CPU_NONE, CPU_AUTO, CPU_AUTO_64B = 0, 1, 2
+
+ERROR_SUCCESS = 0
+ERROR_UNKNOWN = -1
+ERROR_OUT_OF_MEMORY = 1
+ERROR_USER_CALLBACK_FAILED = 2
+ERROR_LOGIC = 1 << 10
+ERROR_GREYSCALE_SUBSAMPLING = ERROR_LOGIC + 1
+ERROR_COLOR_FAMILY_MISMATCH = ERROR_LOGIC + 2
+ERROR_IMAGE_NOT_DIVISIBLE = ERROR_LOGIC + 3
+ERROR_BIT_DEPTH_OVERFLOW = ERROR_LOGIC + 4
+ERROR_ILLEGAL_ARGUMENT = 2 << 10
+ERROR_ENUM_OUT_OF_RANGE = ERROR_ILLEGAL_ARGUMENT + 1
+ERROR_INVALID_IMAGE_SIZE = ERROR_ILLEGAL_ARGUMENT + 2
+ERROR_UNSUPPORTED_OPERATION = 3 << 10
+ERROR_UNSUPPORTED_SUBSAMPLING = ERROR_UNSUPPORTED_OPERATION + 1
+ERROR_NO_COLORSPACE_CONVERSION = ERROR_UNSUPPORTED_OPERATION + 2
+ERROR_NO_FIELD_PARITY_CONVERSION = ERROR_UNSUPPORTED_OPERATION + 3
+ERROR_RESAMPLING_NOT_AVAILABLE = ERROR_UNSUPPORTED_OPERATION + 4
+
+# every symbol include/zimg.h declares
+EXPORTS = [
+    "zimg_get_version_info", "zimg_get_api_version", "zimg_get_last_error", "zimg_clear_last_error",
+    "zimg_select_buffer_mask", "zimg_filter_graph_free", "zimg_filter_graph_get_tmp_size",
+    "zimg_filter_graph_get_input_buffering", "zimg_filter_graph_get_output_buffering", "zimg_filter_graph_process",
+    "zimg_image_format_default", "zimg_graph_builder_params_default", "zimg_filter_graph_build",
+    "zimg_filter_graph_get_device_tmp_size", "zimg_filter_graph_process_device",
+    "zimg_filter_graph_process_device_batch", "zimg_filter_graph_describe",
+]
+
+
+class _Region(ctypes.Structure):
+    _fields_ = [("left", ctypes.c_double), ("top", ctypes.c_double), ("width", ctypes.c_double), ("height", ctypes.c_double)]
+
+
+class ImageFormatStruct(ctypes.Structure):
+    """zimg_image_format (zimg.h:558-588)."""
+    _fields_ = [
+        ("version", ctypes.c_uint), ("width", ctypes.c_uint), ("height", ctypes.c_uint), ("pixel_type", ctypes.c_int),
+        ("subsample_w", ctypes.c_uint), ("subsample_h", ctypes.c_uint), ("color_family", ctypes.c_int),
+        ("matrix_coefficients", ctypes.c_int), ("transfer_characteristics", ctypes.c_int), ("color_primaries", ctypes.c_int),
+        ("depth", ctypes.c_uint), ("pixel_range", ctypes.c_int), ("field_parity", ctypes.c_int),
+        ("chroma_location", ctypes.c_int), ("active_region", _Region), ("alpha", ctypes.c_int),
+    ]
+
+
+class GraphBuilderParamsStruct(ctypes.Structure):
+    """zimg_graph_builder_params (zimg.h:593-647)."""
+    _fields_ = [
+        ("version", ctypes.c_uint), ("resample_filter", ctypes.c_int), ("filter_param_a", ctypes.c_double),
+        ("filter_param_b", ctypes.c_double), ("resample_filter_uv", ctypes.c_int), ("filter_param_a_uv", ctypes.c_double),
+        ("filter_param_b_uv", ctypes.c_double), ("dither_type", ctypes.c_int), ("cpu_type", ctypes.c_int),
+        ("nominal_peak_luminance", ctypes.c_double), ("allow_approximate_gamma", ctypes.c_char),
+        ("scene_referred", ctypes.c_char), ("chromatic_adaptation", ctypes.c_char),
+    ]
+
+
+class _Plane(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("stride", ctypes.c_ssize_t), ("mask", ctypes.c_uint)]
+
+
+class ImageBufferStruct(ctypes.Structure):
+    """zimg_image_buffer / zimg_image_buffer_const (zimg.h:409-430)."""
+    _fields_ = [("version", ctypes.c_uint), ("plane", _Plane * 4)]
+
+
+CALLBACK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint)
+
+_lib = None
+
+
+def lib():
+    """Load the built library; raise if it is absent (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built: run `make -C zimg_amd` (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, sz = ctypes.c_void_p, ctypes.c_size_t
+        L.zimg_get_version_info.argtypes = [ctypes.POINTER(ctypes.c_uint)] * 3
+        L.zimg_get_api_version.restype = ctypes.c_uint
+        L.zimg_get_api_version.argtypes = [ctypes.POINTER(ctypes.c_uint)] * 2
+        L.zimg_get_last_error.restype = ctypes.c_int
+        L.zimg_get_last_error.argtypes = [ctypes.c_char_p, sz]
+        L.zimg_select_buffer_mask.restype = ctypes.c_uint
+        L.zimg_select_buffer_mask.argtypes = [ctypes.c_uint]
+        L.zimg_filter_graph_free.argtypes = [vp]
+        L.zimg_filter_graph_get_tmp_size.restype = ctypes.c_int
+        L.zimg_filter_graph_get_tmp_size.argtypes = [vp, ctypes.POINTER(sz)]
+        L.zimg_filter_graph_get_input_buffering.restype = ctypes.c_int
+        L.zimg_filter_graph_get_input_buffering.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
+        L.zimg_filter_graph_get_output_buffering.restype = ctypes.c_int
+        L.zimg_filter_graph_get_output_buffering.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
+        L.zimg_filter_graph_process.restype = ctypes.c_int
+        L.zimg_filter_graph_process.argtypes = [vp, ctypes.POINTER(ImageBufferStruct), ctypes.POINTER(ImageBufferStruct), vp,
+                                                CALLBACK, vp, CALLBACK, vp]
+        L.zimg_image_format_default.argtypes = [ctypes.POINTER(ImageFormatStruct), ctypes.c_uint]
+        L.zimg_graph_builder_params_default.argtypes = [ctypes.POINTER(GraphBuilderParamsStruct), ctypes.c_uint]
+        L.zimg_filter_graph_build.restype = vp
+        L.zimg_filter_graph_build.argtypes = [ctypes.POINTER(ImageFormatStruct), ctypes.POINTER(ImageFormatStruct),
+                                              ctypes.POINTER(GraphBuilderParamsStruct)]
+        L.zimg_filter_graph_get_device_tmp_size.restype = ctypes.c_int
+        L.zimg_filter_graph_get_device_tmp_size.argtypes = [vp, ctypes.c_uint, ctypes.POINTER(sz)]
+        L.zimg_filter_graph_process_device.restype = ctypes.c_int
+        L.zimg_filter_graph_process_device.argtypes = [vp, ctypes.POINTER(ImageBufferStruct), ctypes.POINTER(ImageBufferStruct), vp, vp]
+        L.zimg_filter_graph_process_device_batch.restype = ctypes.c_int
+        L.zimg_filter_graph_process_device_batch.argtypes = [vp, ctypes.c_uint, ctypes.POINTER(ImageBufferStruct),
+                                                             ctypes.POINTER(ctypes.c_ssize_t), ctypes.POINTER(ImageBufferStruct),
+                                                             ctypes.POINTER(ctypes.c_ssize_t), vp, vp]
+        L.zimg_filter_graph_describe.restype = sz
+        L.zimg_filter_graph_describe.argtypes = [vp, ctypes.c_char_p, sz]
+        _lib = L
+    return _lib
+
+
+class ZimgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"zimg error {code}: {msg}")
+        self.code = code
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(1024)
+    code = lib().zimg_get_last_error(buf, 1024)
+    return code, buf.value.decode(errors="replace")
+
+
+def check(code):
+    if code != ERROR_SUCCESS:
+        c, msg = last_error()
+        raise ZimgError(code, msg)
+
+
+def image_format(version=API_VERSION, **kw):
+    f = ImageFormatStruct()
+    lib().zimg_image_format_default(ctypes.byref(f), version)
+    for k, v in kw.items():
+        if k in ("active_left", "active_top", "active_width", "active_height"):
+            setattr(f.active_region, k[len("active_"):], v)
+        else:
+            setattr(f, k, v)
+    return f
+
+
+def builder_params(version=API_VERSION, **kw):
+    p = GraphBuilderParamsStruct()
+    lib().zimg_graph_builder_params_default(ctypes.byref(p), version)
+    for k, v in kw.items():
+        if k in ("allow_approximate_gamma", "scene_referred", "chromatic_adaptation"):
+            v = bytes([int(v)])
+        setattr(p, k, v)
+    return p
+
+
+def image_buffer(planes, version=API_VERSION):
+    """planes: list of up to 4 (ptr, stride_bytes[, mask]) or None."""
+    b = ImageBufferStruct()
+    b.version = version
+    for i, pl in enumerate(planes):
+        if pl is None:
+            continue
+        b.plane[i].data = pl[0]
+        b.plane[i].stride = pl[1]
+        b.plane[i].mask = pl[2] if len(pl) > 2 else BUFFER_MAX
+    return b
+
+
+class FilterGraph:
+    """Owns a zimg_filter_graph handle (zimg++.hpp:143-252)."""
+
+    def __init__(self, src_format, dst_format, params=None):
+        L = lib()
+        self._h = L.zimg_filter_graph_build(ctypes.byref(src_format), ctypes.byref(dst_format),
+                                            ctypes.byref(params) if params is not None else None)
+        if not self._h:
+            code, msg = last_error()
+            raise ZimgError(code, msg)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.zimg_filter_graph_free(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def tmp_size(self):
+        out = ctypes.c_size_t()
+        check(lib().zimg_filter_graph_get_tmp_size(self._h, ctypes.byref(out)))
+        return out.value
+
+    def device_tmp_size(self, count=1):
+        out = ctypes.c_size_t()
+        check(lib().zimg_filter_graph_get_device_tmp_size(self._h, count, ctypes.byref(out)))
+        return out.value
+
+    def input_buffering(self):
+        out = ctypes.c_uint()
+        check(lib().zimg_filter_graph_get_input_buffering(self._h, ctypes.byref(out)))
+        return out.value
+
+    def output_buffering(self):
+        out = ctypes.c_uint()
+        check(lib().zimg_filter_graph_get_output_buffering(self._h, ctypes.byref(out)))
+        return out.value
+
+    def describe(self):
+        n = lib().zimg_filter_graph_describe(self._h, None, 0)
+        buf = ctypes.create_string_buffer(n + 1)
+        lib().zimg_filter_graph_describe(self._h, buf, n + 1)
+        return buf.value.decode()
+
+    def plan_trace(self):
+        text = self.describe()
+        plan = text.split("device:\n")[0].split("plan:\n", 1)[1]
+        return [line for line in plan.splitlines() if line]
+
+    def process(self, src, dst, tmp=None, unpack_cb=None, unpack_user=None, pack_cb=None, pack_user=None):
+        """Host buffers (zimg_filter_graph_process)."""
+        ucb = unpack_cb if unpack_cb is not None else CALLBACK()
+        pcb = pack_cb if pack_cb is not None else CALLBACK()
+        check(lib().zimg_filter_graph_process(self._h, ctypes.byref(src), ctypes.byref(dst), tmp, ucb, unpack_user, pcb, pack_user))
+
+    def process_device(self, src, dst, tmp=0, stream=0):
+        check(lib().zimg_filter_graph_process_device(self._h, ctypes.byref(src), ctypes.byref(dst), tmp, stream))
+
+    def process_device_batch(self, count, src, src_frame_stride, dst, dst_frame_stride, tmp=0, stream=0):
+        ss = (ctypes.c_ssize_t * 4)(*src_frame_stride)
+        ds = (ctypes.c_ssize_t * 4)(*dst_frame_stride)
+        check(lib().zimg_filter_graph_process_device_batch(self._h, count, ctypes.byref(src), ss, ctypes.byref(dst), ds, tmp, stream))

@@ -1,0 +1,689 @@
+// The C API (include/zimg.h). Enumeration translation, versioned structure
+// import and error reporting follow the reference's api/zimg.cpp (:33-687) so
+// that callers observe the same codes, defaults and version semantics; graph
+// construction and execution are this library's (planner + device program).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "device.h"
+#include "graph.h"
+#include "zimg.h"
+
+struct zimg_filter_graph {
+	std::unique_ptr<zamd::DeviceProgram> prog;
+	unsigned src_subsample_h = 0, dst_subsample_h = 0;
+	unsigned src_bpp = 1, dst_bpp = 1;
+
+	// Host-path staging in HBM (zimg_filter_graph_process), allocated on first use.
+	mutable std::mutex host_mutex;
+	mutable void *dev_mem = nullptr;
+	mutable size_t dev_mem_size = 0;
+	mutable int dev_mem_device = -1;
+
+	~zimg_filter_graph()
+	{
+		if (dev_mem) {
+			int cur = 0;
+			if (hipGetDevice(&cur) == hipSuccess) {
+				(void)hipSetDevice(dev_mem_device);
+				(void)hipFree(dev_mem);
+				(void)hipSetDevice(cur);
+			}
+		}
+	}
+};
+
+namespace {
+
+using namespace zamd;
+
+constexpr unsigned API_2_0 = ZIMG_MAKE_API_VERSION(2, 0);
+constexpr unsigned API_2_1 = ZIMG_MAKE_API_VERSION(2, 1);
+constexpr unsigned API_2_2 = ZIMG_MAKE_API_VERSION(2, 2);
+constexpr unsigned API_2_4 = ZIMG_MAKE_API_VERSION(2, 4);
+constexpr unsigned API_2_5 = ZIMG_MAKE_API_VERSION(2, 5);
+constexpr unsigned VERSION[3] = { 3, 0, 6 };
+
+thread_local zimg_error_code_e g_last_error = ZIMG_ERROR_SUCCESS;
+thread_local std::string g_last_message;
+
+zimg_error_code_e record(zimg_error_code_e code, const char *msg)
+{
+	g_last_error = code;
+	try {
+		g_last_message = msg ? msg : "";
+	} catch (...) {
+		g_last_message.clear();
+	}
+	return code;
+}
+
+zimg_error_code_e handle_current_exception()
+{
+	try {
+		throw;
+	} catch (const Error &e) {
+		return record(e.code, e.what());
+	} catch (const std::bad_alloc &) {
+		return record(ZIMG_ERROR_OUT_OF_MEMORY, "out of memory");
+	} catch (const std::exception &e) {
+		return record(ZIMG_ERROR_UNKNOWN, e.what());
+	} catch (...) {
+		return record(ZIMG_ERROR_UNKNOWN, "unknown error");
+	}
+}
+
+[[noreturn]] void enum_error(const char *msg) { fail(ZIMG_ERROR_ENUM_OUT_OF_RANGE, msg); }
+
+PixelType tr_pixel(zimg_pixel_type_e t)
+{
+	switch (t) {
+	case ZIMG_PIXEL_BYTE: return PixelType::BYTE;
+	case ZIMG_PIXEL_WORD: return PixelType::WORD;
+	case ZIMG_PIXEL_HALF: return PixelType::HALF;
+	case ZIMG_PIXEL_FLOAT: return PixelType::FLOAT;
+	default: enum_error("unrecognized pixel type");
+	}
+}
+
+bool tr_range(zimg_pixel_range_e r)
+{
+	switch (r) {
+	case ZIMG_RANGE_LIMITED: return false;
+	case ZIMG_RANGE_FULL: return true;
+	default: enum_error("unrecognized pixel range");
+	}
+}
+
+ColorFamily tr_family(zimg_color_family_e f)
+{
+	switch (f) {
+	case ZIMG_COLOR_GREY: return ColorFamily::GREY;
+	case ZIMG_COLOR_RGB: return ColorFamily::RGB;
+	case ZIMG_COLOR_YUV: return ColorFamily::YUV;
+	default: enum_error("unrecognized color family");
+	}
+}
+
+AlphaType tr_alpha(zimg_alpha_type_e a)
+{
+	switch (a) {
+	case ZIMG_ALPHA_NONE: return AlphaType::NONE;
+	case ZIMG_ALPHA_STRAIGHT: return AlphaType::STRAIGHT;
+	case ZIMG_ALPHA_PREMULTIPLIED: return AlphaType::PREMULTIPLIED;
+	default: enum_error("unrecognized alpha type");
+	}
+}
+
+FieldParity tr_parity(zimg_field_parity_e p)
+{
+	switch (p) {
+	case ZIMG_FIELD_PROGRESSIVE: return FieldParity::PROGRESSIVE;
+	case ZIMG_FIELD_TOP: return FieldParity::TOP;
+	case ZIMG_FIELD_BOTTOM: return FieldParity::BOTTOM;
+	default: enum_error("unrecognized field parity");
+	}
+}
+
+void tr_chroma(zimg_chroma_location_e c, ChromaLocW &w, ChromaLocH &h)
+{
+	switch (c) {
+	case ZIMG_CHROMA_LEFT: w = ChromaLocW::LEFT; h = ChromaLocH::CENTER; return;
+	case ZIMG_CHROMA_CENTER: w = ChromaLocW::CENTER; h = ChromaLocH::CENTER; return;
+	case ZIMG_CHROMA_TOP_LEFT: w = ChromaLocW::LEFT; h = ChromaLocH::TOP; return;
+	case ZIMG_CHROMA_TOP: w = ChromaLocW::CENTER; h = ChromaLocH::TOP; return;
+	case ZIMG_CHROMA_BOTTOM_LEFT: w = ChromaLocW::LEFT; h = ChromaLocH::BOTTOM; return;
+	case ZIMG_CHROMA_BOTTOM: w = ChromaLocW::CENTER; h = ChromaLocH::BOTTOM; return;
+	default: enum_error("unrecognized chroma location");
+	}
+}
+
+// ITU enumerations: out of [0, 255] is a range error, an unknown code inside it
+// means "no conversion" (zimg.cpp:114-124).
+template <class E>
+void itu_check(E v, const char *msg)
+{
+	if (static_cast<int>(v) < 0 || static_cast<int>(v) > 255)
+		enum_error(msg);
+}
+
+Matrix tr_matrix(zimg_matrix_coefficients_e m)
+{
+	itu_check(m, "unrecognized matrix coefficients");
+	switch (m) {
+	case ZIMG_MATRIX_RGB: return Matrix::RGB;
+	case ZIMG_MATRIX_BT709: return Matrix::REC_709;
+	case ZIMG_MATRIX_UNSPECIFIED: return Matrix::UNSPECIFIED;
+	case ZIMG_MATRIX_FCC: return Matrix::FCC;
+	case ZIMG_MATRIX_BT470_BG:
+	case ZIMG_MATRIX_ST170_M: return Matrix::REC_601;
+	case ZIMG_MATRIX_ST240_M: return Matrix::SMPTE_240M;
+	case ZIMG_MATRIX_YCGCO: return Matrix::YCGCO;
+	case ZIMG_MATRIX_BT2020_NCL: return Matrix::REC_2020_NCL;
+	case ZIMG_MATRIX_BT2020_CL: return Matrix::REC_2020_CL;
+	case ZIMG_MATRIX_CHROMATICITY_DERIVED_NCL: return Matrix::CHROMATICITY_DERIVED_NCL;
+	case ZIMG_MATRIX_CHROMATICITY_DERIVED_CL: return Matrix::CHROMATICITY_DERIVED_CL;
+	case ZIMG_MATRIX_BT2100_ICTCP: return Matrix::REC_2100_ICTCP;
+	default: fail(ZIMG_ERROR_NO_COLORSPACE_CONVERSION, "unrecognized matrix coefficients");
+	}
+}
+
+Transfer tr_transfer(zimg_transfer_characteristics_e t)
+{
+	itu_check(t, "unrecognized transfer characteristics");
+	switch (t) {
+	case ZIMG_TRANSFER_BT709:
+	case ZIMG_TRANSFER_BT601:
+	case ZIMG_TRANSFER_BT2020_10:
+	case ZIMG_TRANSFER_BT2020_12: return Transfer::REC_709;
+	case ZIMG_TRANSFER_UNSPECIFIED: return Transfer::UNSPECIFIED;
+	case ZIMG_TRANSFER_ST240_M: return Transfer::SMPTE_240M;
+	case ZIMG_TRANSFER_BT470_M: return Transfer::REC_470_M;
+	case ZIMG_TRANSFER_BT470_BG: return Transfer::REC_470_BG;
+	case ZIMG_TRANSFER_IEC_61966_2_4: return Transfer::XVYCC;
+	case ZIMG_TRANSFER_BT1361: return Transfer::REC_1361;
+	case ZIMG_TRANSFER_IEC_61966_2_1: return Transfer::SRGB;
+	case ZIMG_TRANSFER_LINEAR: return Transfer::LINEAR;
+	case ZIMG_TRANSFER_LOG_100: return Transfer::LOG_100;
+	case ZIMG_TRANSFER_LOG_316: return Transfer::LOG_316;
+	case ZIMG_TRANSFER_ST2084: return Transfer::ST_2084;
+	case ZIMG_TRANSFER_ST428: return Transfer::ST_428;
+	case ZIMG_TRANSFER_ARIB_B67: return Transfer::ARIB_B67;
+	default: fail(ZIMG_ERROR_NO_COLORSPACE_CONVERSION, "unrecognized transfer characteristics");
+	}
+}
+
+Primaries tr_primaries(zimg_color_primaries_e p)
+{
+	itu_check(p, "unrecognized color primaries");
+	switch (p) {
+	case ZIMG_PRIMARIES_BT470_M: return Primaries::REC_470_M;
+	case ZIMG_PRIMARIES_BT470_BG: return Primaries::REC_470_BG;
+	case ZIMG_PRIMARIES_BT709: return Primaries::REC_709;
+	case ZIMG_PRIMARIES_UNSPECIFIED: return Primaries::UNSPECIFIED;
+	case ZIMG_PRIMARIES_ST170_M:
+	case ZIMG_PRIMARIES_ST240_M: return Primaries::SMPTE_C;
+	case ZIMG_PRIMARIES_FILM: return Primaries::FILM;
+	case ZIMG_PRIMARIES_BT2020: return Primaries::REC_2020;
+	case ZIMG_PRIMARIES_ST428: return Primaries::XYZ;
+	case ZIMG_PRIMARIES_ST431_2: return Primaries::DCI_P3;
+	case ZIMG_PRIMARIES_ST432_1: return Primaries::DCI_P3_D65;
+	case ZIMG_PRIMARIES_EBU3213_E: return Primaries::EBU_3213_E;
+	default: fail(ZIMG_ERROR_NO_COLORSPACE_CONVERSION, "unrecognized color primaries");
+	}
+}
+
+Dither tr_dither(zimg_dither_type_e d)
+{
+	switch (d) {
+	case ZIMG_DITHER_NONE: return Dither::NONE;
+	case ZIMG_DITHER_ORDERED: return Dither::ORDERED;
+	case ZIMG_DITHER_RANDOM: return Dither::RANDOM;
+	case ZIMG_DITHER_ERROR_DIFFUSION: return Dither::ERROR_DIFFUSION;
+	default: enum_error("unrecognized dither type");
+	}
+}
+
+void tr_cpu(zimg_cpu_type_e c)
+{
+	const int v = static_cast<int>(c);
+	if (!(v >= 0 && v <= 2) && !(v >= 1000 && v <= 1016))
+		enum_error("unrecognized cpu type");
+}
+
+ResampleFilter tr_filter(zimg_resample_filter_e f, double a, double b)
+{
+	// zimg.cpp:320-351
+	ResampleFilter r;
+	r.type = f;
+	switch (f) {
+	case ZIMG_RESIZE_POINT:
+	case ZIMG_RESIZE_BILINEAR:
+	case ZIMG_RESIZE_SPLINE16:
+	case ZIMG_RESIZE_SPLINE36:
+	case ZIMG_RESIZE_SPLINE64:
+		return r;
+	case ZIMG_RESIZE_BICUBIC:
+		r.param_a = std::isnan(a) ? 0.0 : a;
+		r.param_b = std::isnan(b) ? 0.5 : b;
+		return r;
+	case ZIMG_RESIZE_LANCZOS: {
+		const double taps = std::isnan(a) ? 3.0 : std::clamp(a, 1.0, static_cast<double>(UINT_MAX));
+		const unsigned t = static_cast<unsigned>(taps);
+		if (t >= 16)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "lanczos tap count too high");
+		r.param_a = t;
+		return r;
+	}
+	default:
+		if (static_cast<int>(f) == -1)
+			fail(ZIMG_ERROR_UNSUPPORTED_OPERATION, "unresize not supported");
+		enum_error("unrecognized resampling filter");
+	}
+}
+
+ImageState import_state_common(const zimg_image_format &f)
+{
+	ImageState s;
+	if (f.version >= API_2_0) {
+		s.width = f.width;
+		s.height = f.height;
+		s.type = tr_pixel(f.pixel_type);
+		s.subsample_w = f.subsample_w;
+		s.subsample_h = f.subsample_h;
+		s.color = tr_family(f.color_family);
+		s.depth = f.depth ? f.depth : pixel_depth(s.type);
+		s.fullrange = tr_range(f.pixel_range);
+		s.parity = tr_parity(f.field_parity);
+		tr_chroma(f.chroma_location, s.chroma_w, s.chroma_h);
+	}
+	if (f.version >= API_2_1) {
+		s.active_left = std::isnan(f.active_region.left) ? 0 : f.active_region.left;
+		s.active_top = std::isnan(f.active_region.top) ? 0 : f.active_region.top;
+		s.active_width = std::isnan(f.active_region.width) ? f.width : f.active_region.width;
+		s.active_height = std::isnan(f.active_region.height) ? f.height : f.active_region.height;
+	} else {
+		s.active_left = 0;
+		s.active_top = 0;
+		s.active_width = f.width;
+		s.active_height = f.height;
+	}
+	if (f.version >= API_2_4)
+		s.alpha = tr_alpha(f.alpha);
+	return s;
+}
+
+void import_states(const zimg_image_format &src, const zimg_image_format &dst, ImageState &s, ImageState &d)
+{
+	if (src.version < API_2_0 || dst.version < API_2_0)
+		fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "API version invalid");
+	s = import_state_common(src);
+	d = import_state_common(dst);
+	// Unenumerated colourspaces are accepted when they form the no-op case (zimg.cpp:414-431).
+	if (src.color_family == dst.color_family && src.matrix_coefficients == dst.matrix_coefficients &&
+	    src.transfer_characteristics == dst.transfer_characteristics && src.color_primaries == dst.color_primaries) {
+		s.colorspace = Colorspace{};
+		d.colorspace = Colorspace{};
+	} else {
+		s.colorspace = { tr_matrix(src.matrix_coefficients), tr_transfer(src.transfer_characteristics), tr_primaries(src.color_primaries) };
+		d.colorspace = { tr_matrix(dst.matrix_coefficients), tr_transfer(dst.transfer_characteristics), tr_primaries(dst.color_primaries) };
+	}
+}
+
+BuildParams import_params(const zimg_graph_builder_params &p)
+{
+	if (p.version < API_2_0)
+		fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "API version invalid");
+	BuildParams r;
+	if (p.version >= API_2_0) {
+		r.filter = tr_filter(p.resample_filter, p.filter_param_a, p.filter_param_b);
+		r.filter_uv = tr_filter(p.resample_filter_uv, p.filter_param_a_uv, p.filter_param_b_uv);
+		r.dither = tr_dither(p.dither_type);
+		tr_cpu(p.cpu_type);
+	}
+	if (p.version >= API_2_2) {
+		r.peak_luminance = p.nominal_peak_luminance;
+		r.approximate_gamma = !!p.allow_approximate_gamma;
+	}
+	if (p.version >= API_2_5) {
+		r.scene_referred = !!p.scene_referred;
+		r.chromatic_adaptation = !!p.chromatic_adaptation;
+	}
+	return r;
+}
+
+const zamd::DeviceProgram &program(const zimg_filter_graph *g)
+{
+	if (!g || !g->prog)
+		fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null graph");
+	return *g->prog;
+}
+
+template <class Buf>
+DeviceBuffers import_device_buffer(const Buf *b, const ptrdiff_t *fstride, const unsigned *plane_w, bool is_src)
+{
+	if (!b)
+		fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null buffer");
+	if (b->version < API_2_0)
+		fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "API version invalid");
+	const unsigned nplanes = b->version >= API_2_4 ? 4 : 3;
+	DeviceBuffers d;
+	for (unsigned p = 0; p < nplanes; ++p) {
+		if (!plane_w[p])
+			continue;
+		if (b->plane[p].mask != ZIMG_BUFFER_MAX)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "device buffers must be whole planes (mask = ZIMG_BUFFER_MAX)");
+		d.ptr[p] = static_cast<char *>(const_cast<void *>(static_cast<const void *>(b->plane[p].data)));
+		d.stride[p] = b->plane[p].stride;
+		d.fstride[p] = fstride ? fstride[p] : 0;
+		if (!d.ptr[p])
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, is_src ? "missing source plane" : "missing destination plane");
+	}
+	return d;
+}
+
+} // namespace
+
+extern "C" {
+
+void zimg_get_version_info(unsigned *major, unsigned *minor, unsigned *micro)
+{
+	*major = VERSION[0];
+	*minor = VERSION[1];
+	*micro = VERSION[2];
+}
+
+unsigned zimg_get_api_version(unsigned *major, unsigned *minor)
+{
+	if (major) *major = ZIMG_API_VERSION_MAJOR;
+	if (minor) *minor = ZIMG_API_VERSION_MINOR;
+	return ZIMG_API_VERSION;
+}
+
+zimg_error_code_e zimg_get_last_error(char *err_msg, size_t n)
+{
+	if (err_msg && n) {
+		std::strncpy(err_msg, g_last_message.c_str(), n);
+		err_msg[n - 1] = '\0';
+	}
+	return g_last_error;
+}
+
+void zimg_clear_last_error(void)
+{
+	g_last_error = ZIMG_ERROR_SUCCESS;
+	g_last_message.clear();
+	g_last_message.shrink_to_fit();
+}
+
+unsigned zimg_select_buffer_mask(unsigned count)
+{
+	if (count <= 1)
+		return 0;
+	return ZIMG_BUFFER_MAX >> __builtin_clz(count - 1);
+}
+
+void zimg_filter_graph_free(zimg_filter_graph *ptr) { delete ptr; }
+
+zimg_error_code_e zimg_filter_graph_get_tmp_size(const zimg_filter_graph *ptr, size_t *out)
+{
+	try {
+		const Plan &plan = program(ptr).plan();
+		size_t total = 0;
+		for (int q = 0; q < 4; ++q) {
+			total += static_cast<size_t>(plan.src_w[q]) * plan.src_h[q] * ptr->src_bpp;
+			total += static_cast<size_t>(plan.dst_w[q]) * plan.dst_h[q] * ptr->dst_bpp;
+		}
+		*out = total;
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
+zimg_error_code_e zimg_filter_graph_get_input_buffering(const zimg_filter_graph *ptr, unsigned *out)
+{
+	try {
+		program(ptr);
+		*out = 1U << ptr->src_subsample_h; // rows are copied out as soon as each unpack returns
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
+zimg_error_code_e zimg_filter_graph_get_output_buffering(const zimg_filter_graph *ptr, unsigned *out)
+{
+	try {
+		program(ptr);
+		*out = 1U << ptr->dst_subsample_h;
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
+zimg_error_code_e zimg_filter_graph_process(const zimg_filter_graph *ptr, const zimg_image_buffer_const *src, const zimg_image_buffer *dst, void *tmp,
+                                            zimg_filter_graph_callback unpack_cb, void *unpack_user,
+                                            zimg_filter_graph_callback pack_cb, void *pack_user)
+{
+	try {
+		const DeviceProgram &prog = program(ptr);
+		const Plan &plan = prog.plan();
+		if (!src || !dst)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null buffer");
+		if (src->version < API_2_0 || dst->version < API_2_0)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "API version invalid");
+
+		// Host staging: the caller's tmp holds every source and destination plane packed.
+		size_t tmp_need = 0;
+		zimg_filter_graph_get_tmp_size(ptr, &tmp_need);
+		std::vector<char> own;
+		char *host = static_cast<char *>(tmp);
+		if (!host) {
+			own.resize(tmp_need);
+			host = own.data();
+		}
+		size_t src_off[4] = {}, dst_off[4] = {};
+		size_t o = 0;
+		for (int q = 0; q < 4; ++q) {
+			src_off[q] = o;
+			o += static_cast<size_t>(plan.src_w[q]) * plan.src_h[q] * ptr->src_bpp;
+		}
+		for (int q = 0; q < 4; ++q) {
+			dst_off[q] = o;
+			o += static_cast<size_t>(plan.dst_w[q]) * plan.dst_h[q] * ptr->dst_bpp;
+		}
+		const size_t src_bytes = dst_off[0];
+		const size_t dst_bytes = o - src_bytes;
+
+		// Unpack: rows in groups of the vertical subsampling unit (zimg.h:455-482).
+		const unsigned ssh = ptr->src_subsample_h;
+		for (unsigned i = 0; i < plan.src_h[0]; i += 1U << ssh) {
+			if (unpack_cb && unpack_cb(unpack_user, i, 0, plan.src_w[0]))
+				fail(ZIMG_ERROR_USER_CALLBACK_FAILED, "user callback failed");
+			for (int q = 0; q < 4; ++q) {
+				if (!plan.src_w[q])
+					continue;
+				const unsigned sh = (q == 1 || q == 2) ? ssh : 0;
+				const size_t rowbytes = static_cast<size_t>(plan.src_w[q]) * ptr->src_bpp;
+				for (unsigned r = i >> sh; r < ((i + (1U << ssh)) >> sh) && r < plan.src_h[q]; ++r) {
+					const char *row = static_cast<const char *>(src->plane[q].data) + static_cast<ptrdiff_t>(r & src->plane[q].mask) * src->plane[q].stride;
+					std::memcpy(host + src_off[q] + r * rowbytes, row, rowbytes);
+				}
+			}
+		}
+
+		std::lock_guard<std::mutex> lock(ptr->host_mutex);
+		int device = 0;
+		if (hipGetDevice(&device) != hipSuccess)
+			fail(ZIMG_ERROR_UNKNOWN, "no HIP device");
+		const size_t dtmp = prog.tmp_size(1);
+		const size_t need = src_bytes + dst_bytes + dtmp + 1024;
+		if (!ptr->dev_mem || ptr->dev_mem_size < need || ptr->dev_mem_device != device) {
+			if (ptr->dev_mem)
+				(void)hipFree(ptr->dev_mem);
+			ptr->dev_mem = nullptr;
+			if (hipMalloc(&ptr->dev_mem, need) != hipSuccess)
+				fail(ZIMG_ERROR_OUT_OF_MEMORY, "device staging allocation failed");
+			ptr->dev_mem_size = need;
+			ptr->dev_mem_device = device;
+		}
+		char *dmem = static_cast<char *>(ptr->dev_mem);
+		const size_t dtmp_off = (src_bytes + dst_bytes + 255) / 256 * 256;
+		if (hipMemcpy(dmem, host, src_bytes, hipMemcpyHostToDevice) != hipSuccess)
+			fail(ZIMG_ERROR_UNKNOWN, "upload failed");
+
+		DeviceBuffers ds, dd;
+		for (int q = 0; q < 4; ++q) {
+			if (plan.src_w[q]) {
+				ds.ptr[q] = dmem + src_off[q];
+				ds.stride[q] = static_cast<long long>(plan.src_w[q]) * ptr->src_bpp;
+			}
+			if (plan.dst_w[q]) {
+				dd.ptr[q] = dmem + dst_off[q];
+				dd.stride[q] = static_cast<long long>(plan.dst_w[q]) * ptr->dst_bpp;
+			}
+		}
+		prog.run(ds, dd, dmem + dtmp_off, 1, nullptr);
+		if (hipDeviceSynchronize() != hipSuccess)
+			fail(ZIMG_ERROR_UNKNOWN, "device execution failed");
+		if (hipMemcpy(host + src_bytes, dmem + src_bytes, dst_bytes, hipMemcpyDeviceToHost) != hipSuccess)
+			fail(ZIMG_ERROR_UNKNOWN, "download failed");
+
+		const unsigned dsh = ptr->dst_subsample_h;
+		for (unsigned i = 0; i < plan.dst_h[0]; i += 1U << dsh) {
+			for (int q = 0; q < 4; ++q) {
+				if (!plan.dst_w[q])
+					continue;
+				const unsigned sh = (q == 1 || q == 2) ? dsh : 0;
+				const size_t rowbytes = static_cast<size_t>(plan.dst_w[q]) * ptr->dst_bpp;
+				for (unsigned r = i >> sh; r < ((i + (1U << dsh)) >> sh) && r < plan.dst_h[q]; ++r) {
+					char *row = static_cast<char *>(dst->plane[q].data) + static_cast<ptrdiff_t>(r & dst->plane[q].mask) * dst->plane[q].stride;
+					std::memcpy(row, host + dst_off[q] + r * rowbytes, rowbytes);
+				}
+			}
+			if (pack_cb && pack_cb(pack_user, i, 0, plan.dst_w[0]))
+				fail(ZIMG_ERROR_USER_CALLBACK_FAILED, "user callback failed");
+		}
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
+void zimg_image_format_default(zimg_image_format *ptr, unsigned version)
+{
+	ptr->version = version;
+	if (version >= API_2_0) {
+		ptr->width = 0;
+		ptr->height = 0;
+		ptr->pixel_type = static_cast<zimg_pixel_type_e>(-1);
+		ptr->subsample_w = 0;
+		ptr->subsample_h = 0;
+		ptr->color_family = ZIMG_COLOR_GREY;
+		ptr->matrix_coefficients = ZIMG_MATRIX_UNSPECIFIED;
+		ptr->transfer_characteristics = ZIMG_TRANSFER_UNSPECIFIED;
+		ptr->color_primaries = ZIMG_PRIMARIES_UNSPECIFIED;
+		ptr->depth = 0;
+		ptr->pixel_range = ZIMG_RANGE_LIMITED;
+		ptr->field_parity = ZIMG_FIELD_PROGRESSIVE;
+		ptr->chroma_location = ZIMG_CHROMA_LEFT;
+	}
+	if (version >= API_2_1) {
+		ptr->active_region.left = NAN;
+		ptr->active_region.top = NAN;
+		ptr->active_region.width = NAN;
+		ptr->active_region.height = NAN;
+	}
+	if (version >= API_2_4)
+		ptr->alpha = ZIMG_ALPHA_NONE;
+}
+
+void zimg_graph_builder_params_default(zimg_graph_builder_params *ptr, unsigned version)
+{
+	ptr->version = version;
+	if (version >= API_2_0) {
+		ptr->resample_filter = ZIMG_RESIZE_BICUBIC;
+		ptr->filter_param_a = NAN;
+		ptr->filter_param_b = NAN;
+		ptr->resample_filter_uv = ZIMG_RESIZE_BILINEAR;
+		ptr->filter_param_a_uv = NAN;
+		ptr->filter_param_b_uv = NAN;
+		ptr->dither_type = ZIMG_DITHER_NONE;
+		ptr->cpu_type = ZIMG_CPU_AUTO;
+	}
+	if (version >= API_2_2) {
+		ptr->nominal_peak_luminance = NAN;
+		ptr->allow_approximate_gamma = 0;
+	}
+	if (version >= API_2_5) {
+		ptr->scene_referred = 0;
+		ptr->chromatic_adaptation = 0;
+	}
+}
+
+zimg_filter_graph *zimg_filter_graph_build(const zimg_image_format *src_format, const zimg_image_format *dst_format, const zimg_graph_builder_params *params)
+{
+	try {
+		if (!src_format || !dst_format)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null format");
+		ImageState s, d;
+		import_states(*src_format, *dst_format, s, d);
+		BuildParams bp;
+		if (params)
+			bp = import_params(*params);
+		Plan plan = build_plan(s, d, bp);
+		auto g = std::make_unique<zimg_filter_graph>();
+		g->prog = std::make_unique<DeviceProgram>(plan);
+		g->src_subsample_h = s.color == ColorFamily::GREY ? 0 : s.subsample_h;
+		g->dst_subsample_h = d.color == ColorFamily::GREY ? 0 : d.subsample_h;
+		g->src_bpp = pixel_size(s.type);
+		g->dst_bpp = pixel_size(d.type);
+		return g.release();
+	} catch (...) {
+		handle_current_exception();
+		return nullptr;
+	}
+}
+
+zimg_error_code_e zimg_filter_graph_get_device_tmp_size(const zimg_filter_graph *ptr, unsigned count, size_t *out)
+{
+	try {
+		*out = program(ptr).tmp_size(count ? count : 1);
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
+zimg_error_code_e zimg_filter_graph_process_device_batch(const zimg_filter_graph *ptr, unsigned count,
+                                                         const zimg_image_buffer_const *src, const ptrdiff_t src_frame_stride[4],
+                                                         const zimg_image_buffer *dst, const ptrdiff_t dst_frame_stride[4],
+                                                         void *tmp, void *stream)
+{
+	try {
+		const DeviceProgram &prog = program(ptr);
+		const Plan &plan = prog.plan();
+		DeviceBuffers ds = import_device_buffer(src, src_frame_stride, plan.src_w, true);
+		DeviceBuffers dd = import_device_buffer(dst, dst_frame_stride, plan.dst_w, false);
+		prog.run(ds, dd, tmp, count, stream);
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
+zimg_error_code_e zimg_filter_graph_process_device(const zimg_filter_graph *ptr, const zimg_image_buffer_const *src,
+                                                   const zimg_image_buffer *dst, void *tmp, void *stream)
+{
+	return zimg_filter_graph_process_device_batch(ptr, 1, src, nullptr, dst, nullptr, tmp, stream);
+}
+
+size_t zimg_filter_graph_describe(const zimg_filter_graph *ptr, char *buf, size_t n)
+{
+	try {
+		const DeviceProgram &prog = program(ptr);
+		std::string s = "plan:\n" + prog.plan().describe() + "device:\n" + prog.describe();
+		if (buf && n) {
+			std::strncpy(buf, s.c_str(), n);
+			buf[n - 1] = '\0';
+		}
+		return s.size();
+	} catch (...) {
+		handle_current_exception();
+		return 0;
+	}
+}
+
+} // extern "C"

@@ -1,0 +1,90 @@
+// Device lowering of a planned graph: fused launch steps, HBM scratch layout,
+// per-device coefficient tables, and execution on a HIP stream.
+// Replaces the reference's graphengine row scheduler (filtergraph.cpp:96-122).
+#pragma once
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "graph.h"
+#include "kernels.h"
+
+namespace zamd {
+
+struct DeviceBuffers {
+	// Frame 0 of plane p at ptr[p]; row stride and per-frame stride in bytes.
+	char *ptr[4] = {};
+	long long stride[4] = {};
+	long long fstride[4] = {};
+};
+
+class DeviceProgram {
+public:
+	struct Buf {
+		enum Kind : int { NONE, SRC, DST, TMP } kind = NONE;
+		int idx = 0;
+	};
+
+	struct Step {
+		enum Kind : int { RESIZE, POINT, CSP, ED } kind = POINT;
+		Buf in[3], out[3];
+		dev::InXform ix[3]{};
+		dev::OutXform ox[3]{};
+		unsigned width = 0, height = 0;    // output size (all kinds)
+		unsigned in_w = 0, in_h = 0;       // input size (resize)
+		// RESIZE
+		int passes = 1;
+		bool h_first = true;
+		int tab[2] = { -1, -1 };           // node ids owning the pass tables
+		bool word = false;
+		int pixel_max = 65535;
+		unsigned mid_w = 0, mid_h = 0;
+		unsigned tile_w = 0, tile_h = 0, lds_in_rows = 0, lds_in_cols = 0, lds_mid_rows = 0, lds_mid_cols = 0;
+		// POINT
+		unsigned crop_left = 0, crop_top = 0;
+		int fill = 0;
+		uint32_t fill_bits = 0;
+		// CSP
+		CspProgram prog;
+		// ED
+		float ed_scale = 1.0f, ed_offset = 0.0f;
+		bool dead = false;
+		std::string label;
+	};
+
+	struct Slot {
+		unsigned width, height, bpp;
+		size_t pitch;
+	};
+
+	explicit DeviceProgram(const Plan &plan);
+
+	size_t tmp_size(unsigned frames) const;
+	// Enqueue the program for `frames` frames. Throws Error on invalid buffers or launch failure.
+	void run(const DeviceBuffers &src, const DeviceBuffers &dst, void *tmp, unsigned frames, void *stream) const;
+	std::string describe() const;
+
+	const Plan &plan() const { return m_plan; }
+
+private:
+	Plan m_plan;
+	std::vector<Step> m_steps;
+	std::vector<Slot> m_slots;
+
+	struct Tables {
+		void *mem = nullptr;
+		std::map<int, dev::PassArg> pass; // by node id
+	};
+	mutable std::mutex m_mutex;
+	mutable std::map<int, std::shared_ptr<Tables>> m_tables; // by HIP device
+
+	void lower();
+	void fuse();
+	void configure_resize(Step &s);
+	const Tables &tables_for_current_device() const;
+};
+
+} // namespace zamd

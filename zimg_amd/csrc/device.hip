@@ -1,0 +1,568 @@
+// Device lowering and execution. The planned node list is turned into one
+// launch per node, then adjacent launches are fused when the intermediate
+// plane has a single reader, so a typical graph runs as a few kernels whose
+// HBM traffic is close to "read the input once, write the output once":
+//   depth-in -> resize H -> resize V -> dither   => one k_resize launch
+//   to-float x3 -> colourspace -> dither x3      => one k_colorspace launch
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+
+#include "device.h"
+
+namespace zamd {
+
+namespace {
+
+constexpr size_t kPitchAlign = 256;
+constexpr size_t kLdsBudget = 64 * 1024;
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+void check_hip(hipError_t e, const char *what)
+{
+	if (e != hipSuccess)
+		fail(ZIMG_ERROR_UNKNOWN, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int type_code(PixelType t) { return static_cast<int>(t); }
+
+} // namespace
+
+DeviceProgram::DeviceProgram(const Plan &plan) : m_plan(plan)
+{
+	lower();
+	fuse();
+	for (auto &s : m_steps)
+		if (!s.dead && s.kind == Step::RESIZE)
+			configure_resize(s);
+	m_steps.erase(std::remove_if(m_steps.begin(), m_steps.end(), [](const Step &s) { return s.dead; }), m_steps.end());
+
+	// Drop scratch slots that fusion made unreachable and renumber the rest.
+	std::vector<int> remap(m_slots.size(), -1);
+	std::vector<Slot> live;
+	for (auto &s : m_steps) {
+		for (Buf *b : { &s.in[0], &s.in[1], &s.in[2], &s.out[0], &s.out[1], &s.out[2] }) {
+			if (b->kind != Buf::TMP)
+				continue;
+			if (remap[b->idx] < 0) {
+				remap[b->idx] = static_cast<int>(live.size());
+				live.push_back(m_slots[b->idx]);
+			}
+			b->idx = remap[b->idx];
+		}
+	}
+	m_slots = std::move(live);
+}
+
+void DeviceProgram::lower()
+{
+	const auto &nodes = m_plan.nodes;
+	auto key = [](int node, int out) { return node * 4 + out; };
+	std::map<int, int> readers;
+	std::map<int, std::vector<int>> sink_planes;
+
+	for (const Node &n : nodes) {
+		const int ndeps = n.kind == NodeKind::COLORSPACE ? 3 : (n.kind == NodeKind::SOURCE || n.kind == NodeKind::FILL ? 0 : 1);
+		for (int d = 0; d < ndeps; ++d)
+			readers[key(n.dep[d], n.dep_out[d])]++;
+	}
+	for (int q = 0; q < 4; ++q)
+		if (m_plan.sink[q].node >= 0)
+			sink_planes[key(m_plan.sink[q].node, m_plan.sink[q].out)].push_back(q);
+
+	std::map<int, Buf> where;
+	auto new_slot = [&](unsigned w, unsigned h, unsigned bpp) {
+		m_slots.push_back(Slot{ w, h, bpp, align_up(static_cast<size_t>(w) * bpp, kPitchAlign) });
+		return Buf{ Buf::TMP, static_cast<int>(m_slots.size()) - 1 };
+	};
+	auto place = [&](int id, int out, const PixelFormat &fmt) {
+		const Node &n = nodes[id];
+		const int k = key(id, out);
+		auto sp = sink_planes.find(k);
+		Buf b;
+		if (n.kind == NodeKind::SOURCE)
+			b = Buf{ Buf::SRC, n.plane };
+		else if (sp != sink_planes.end() && sp->second.size() == 1 && readers[k] == 0)
+			b = Buf{ Buf::DST, sp->second[0] };
+		else
+			b = new_slot(n.width, n.height, pixel_size(fmt.type));
+		where[k] = b;
+	};
+
+	for (int id = 0; id < static_cast<int>(nodes.size()); ++id) {
+		const Node &n = nodes[id];
+		if (n.kind == NodeKind::COLORSPACE) {
+			for (int o = 0; o < 3; ++o)
+				place(id, o, PixelFormat{ PixelType::FLOAT });
+		} else {
+			place(id, 0, n.out);
+		}
+		if (n.kind == NodeKind::SOURCE)
+			continue;
+
+		Step s;
+		s.width = n.width;
+		s.height = n.height;
+		s.out[0] = where[key(id, 0)];
+		char label[96];
+
+		if (n.kind == NodeKind::COLORSPACE) {
+			s.kind = Step::CSP;
+			s.prog = n.csp;
+			for (int k = 0; k < 3; ++k) {
+				s.in[k] = where[key(n.dep[k], n.dep_out[k])];
+				s.ix[k] = dev::InXform{ type_code(PixelType::FLOAT), 0, 0, 1.0f, 0.0f };
+				s.out[k] = where[key(id, k)];
+				s.ox[k] = dev::OutXform{ type_code(PixelType::FLOAT), 0, 0, 0, 32, 1.0f, 0.0f };
+			}
+			s.label = "colorspace[" + n.csp.describe() + "]";
+			m_steps.push_back(s);
+			continue;
+		}
+
+		const Node *dep = n.dep[0] >= 0 ? &nodes[n.dep[0]] : nullptr;
+		const PixelType in_type = dep ? (dep->kind == NodeKind::COLORSPACE ? PixelType::FLOAT : dep->out.type) : PixelType::BYTE;
+		if (dep)
+			s.in[0] = where[key(n.dep[0], n.dep_out[0])];
+		s.ix[0] = dev::InXform{ type_code(in_type), 0, 0, 1.0f, 0.0f };
+		s.ox[0] = dev::OutXform{ type_code(n.out.type), 0, 0, n.plane, n.out.depth, 1.0f, 0.0f };
+
+		switch (n.kind) {
+		case NodeKind::DEPTH:
+			if (n.depth_op == Node::SHIFT) {
+				s.kind = Step::POINT;
+				s.ix[0].op = 1;
+				s.ix[0].shift = n.shift;
+				std::snprintf(label, sizeof(label), "shift<<%u[%d]", n.shift, n.plane);
+			} else if (n.depth_op == Node::TO_FLOAT) {
+				s.kind = Step::POINT;
+				s.ix[0].op = 2;
+				s.ix[0].scale = n.scale;
+				s.ix[0].offset = n.offset;
+				std::snprintf(label, sizeof(label), "to_float[%d]", n.plane);
+			} else if (n.dither == Dither::ERROR_DIFFUSION) {
+				s.kind = Step::ED;
+				s.ed_scale = n.scale;
+				s.ed_offset = n.offset;
+				s.ox[0].op = 1;
+				std::snprintf(label, sizeof(label), "error_diffusion[%d]", n.plane);
+			} else {
+				s.kind = Step::POINT;
+				s.ox[0].op = 1;
+				s.ox[0].dither = n.dither == Dither::ORDERED ? 1 : 0;
+				s.ox[0].scale = n.scale;
+				s.ox[0].offset = n.offset;
+				std::snprintf(label, sizeof(label), "%s[%d]", n.dither == Dither::ORDERED ? "dither_bayer" : "quantize", n.plane);
+			}
+			s.label = label;
+			break;
+		case NodeKind::RESIZE_H:
+		case NodeKind::RESIZE_V:
+			s.kind = Step::RESIZE;
+			s.passes = 1;
+			s.h_first = n.kind == NodeKind::RESIZE_H;
+			s.tab[0] = id;
+			s.word = n.out.type == PixelType::WORD;
+			s.pixel_max = static_cast<int>((1UL << n.depth_bits) - 1);
+			s.in_w = dep->width;
+			s.in_h = dep->height;
+			s.mid_w = s.in_w;
+			s.mid_h = s.in_h;
+			std::snprintf(label, sizeof(label), "resize_%c(fw=%u)[%d]", s.h_first ? 'h' : 'v', n.table->filter_width, n.plane);
+			s.label = label;
+			break;
+		case NodeKind::COPY_RECT:
+			s.kind = Step::POINT;
+			s.crop_left = n.crop_left;
+			s.crop_top = n.crop_top;
+			s.label = "copy_rect";
+			break;
+		case NodeKind::FILL:
+			s.kind = Step::POINT;
+			s.fill = 1;
+			s.fill_bits = n.fill_bits;
+			s.label = "fill";
+			break;
+		default:
+			fail(ZIMG_ERROR_UNKNOWN, "unexpected node");
+		}
+		m_steps.push_back(s);
+	}
+
+	// Sinks whose value lives elsewhere (a source plane, a shared node) get a copy.
+	for (int q = 0; q < 4; ++q) {
+		const PlaneRef &r = m_plan.sink[q];
+		if (r.node < 0)
+			continue;
+		Buf b = where[key(r.node, r.out)];
+		if (b.kind == Buf::DST && b.idx == q)
+			continue;
+		const Node &n = nodes[r.node];
+		const PixelType t = n.kind == NodeKind::COLORSPACE ? PixelType::FLOAT : n.out.type;
+		Step s;
+		s.kind = Step::POINT;
+		s.in[0] = b;
+		s.out[0] = Buf{ Buf::DST, q };
+		s.width = n.width;
+		s.height = n.height;
+		s.ix[0] = dev::InXform{ type_code(t), 0, 0, 1.0f, 0.0f };
+		s.ox[0] = dev::OutXform{ type_code(t), 0, 0, q, 0, 1.0f, 0.0f };
+		s.label = "copy[" + std::to_string(q) + "]";
+		m_steps.push_back(s);
+	}
+}
+
+void DeviceProgram::fuse()
+{
+	auto same = [](const Buf &a, const Buf &b) { return a.kind == b.kind && a.idx == b.idx; };
+	auto readers_of = [&](const Buf &b) {
+		int n = 0;
+		for (auto &s : m_steps) {
+			if (s.dead) continue;
+			const int nin = s.kind == Step::CSP ? 3 : 1;
+			for (int k = 0; k < nin; ++k)
+				if (same(s.in[k], b)) ++n;
+		}
+		return n;
+	};
+	auto plain_in = [](const dev::InXform &x) { return x.op == 0; };
+	auto plain_out = [](const dev::OutXform &x) { return x.op == 0; };
+	auto is_point_depth_in = [](const Step &s) { return s.kind == Step::POINT && !s.fill && !s.crop_left && !s.crop_top && s.ix[0].op != 0 && s.ox[0].op == 0; };
+	auto is_point_quant = [](const Step &s) { return s.kind == Step::POINT && !s.fill && !s.crop_left && !s.crop_top && s.ix[0].op == 0 && s.ox[0].op == 1; };
+
+	bool changed = true;
+	while (changed) {
+		changed = false;
+		for (size_t ia = 0; ia < m_steps.size() && !changed; ++ia) {
+			Step &A = m_steps[ia];
+			if (A.dead)
+				continue;
+			const int nout = A.kind == Step::CSP ? 3 : 1;
+			for (int ko = 0; ko < nout && !changed; ++ko) {
+				const Buf mid = A.out[ko];
+				if (mid.kind != Buf::TMP || readers_of(mid) != 1)
+					continue;
+				for (size_t ib = ia + 1; ib < m_steps.size() && !changed; ++ib) {
+					Step &B = m_steps[ib];
+					if (B.dead)
+						continue;
+					const int nin = B.kind == Step::CSP ? 3 : 1;
+					int kin = -1;
+					for (int k = 0; k < nin; ++k)
+						if (same(B.in[k], mid)) kin = k;
+					if (kin < 0)
+						continue;
+
+					// depth-in (shift / to-float) folded into a resize or colourspace input
+					if (is_point_depth_in(A) && (B.kind == Step::RESIZE || B.kind == Step::CSP) && plain_in(B.ix[kin])) {
+						const bool ok = B.kind == Step::CSP ? A.ix[0].op == 2 : (B.word ? A.ix[0].op == 1 : A.ix[0].op == 2);
+						if (ok) {
+							B.in[kin] = A.in[0];
+							B.ix[kin] = A.ix[0];
+							B.label = A.label + "+" + B.label;
+							A.dead = true;
+							changed = true;
+						}
+					}
+					// two single resize passes in opposite directions -> one fused launch
+					else if (A.kind == Step::RESIZE && B.kind == Step::RESIZE && A.passes == 1 && B.passes == 1 && A.h_first != B.h_first &&
+					         A.word == B.word && plain_out(A.ox[0]) && plain_in(B.ix[0]) && m_plan.nodes[A.tab[0]].table->sorted() &&
+					         m_plan.nodes[B.tab[0]].table->sorted()) {
+						B.passes = 2;
+						B.h_first = A.h_first;
+						B.tab[1] = B.tab[0];
+						B.tab[0] = A.tab[0];
+						B.in[0] = A.in[0];
+						B.ix[0] = A.ix[0];
+						B.in_w = A.in_w;
+						B.in_h = A.in_h;
+						B.mid_w = A.width;
+						B.mid_h = A.height;
+						B.label = A.label + "+" + B.label;
+						A.dead = true;
+						changed = true;
+					}
+					// quantisation / dither folded into the producer's store
+					else if ((A.kind == Step::RESIZE || A.kind == Step::CSP) && plain_out(A.ox[ko]) && is_point_quant(B)) {
+						A.out[ko] = B.out[0];
+						A.ox[ko] = B.ox[0];
+						A.label += "+" + B.label;
+						B.dead = true;
+						changed = true;
+					}
+				}
+			}
+		}
+	}
+}
+
+void DeviceProgram::configure_resize(Step &s)
+{
+	const FilterTable &t1 = *m_plan.nodes[s.tab[0]].table;
+	const FilterTable *t2 = s.passes == 2 ? m_plan.nodes[s.tab[1]].table.get() : nullptr;
+	const size_t elem = s.word ? 2 : 4;
+	const bool sorted1 = t1.sorted();
+
+	static const unsigned kWidths[] = { 256, 128, 64, 32, 16, 8, 4, 2, 1 };
+	static const unsigned kHeights[] = { 32, 16, 8, 4, 2, 1 };
+
+	if (s.h_first && !sorted1) {
+		// Non-monotonic left offsets (possible with extreme shifts): single pass,
+		// taps read straight from HBM without a staged tile.
+		if (s.passes != 1)
+			fail(ZIMG_ERROR_UNKNOWN, "unsorted filter in a fused resize");
+		if (t1.filter_width > 64)
+			fail(ZIMG_ERROR_RESAMPLING_NOT_AVAILABLE, "unsorted filter wider than 64 taps");
+		s.tile_w = 256;
+		s.tile_h = 32;
+		s.lds_in_rows = s.lds_in_cols = s.lds_mid_rows = s.lds_mid_cols = 0;
+		return;
+	}
+
+	for (unsigned tw : kWidths) {
+		for (unsigned th : kHeights) {
+			unsigned in_rows = 0, in_cols = 0, mid_rows = 0, mid_cols = 0;
+			if (s.h_first) {
+				if (!sorted1)
+					break;
+				for (unsigned j0 = 0; j0 < s.width; j0 += tw) {
+					const unsigned nc = std::min(tw, s.width - j0);
+					in_cols = std::max(in_cols, t1.left[j0 + nc - 1] + t1.filter_width - t1.left[j0]);
+				}
+				for (unsigned i0 = 0; i0 < s.height; i0 += th) {
+					const unsigned nr = std::min(th, s.height - i0);
+					in_rows = std::max(in_rows, t2 ? t2->left[i0 + nr - 1] + t2->filter_width - t2->left[i0] : nr);
+				}
+				if (t2) {
+					mid_rows = in_rows;
+					mid_cols = tw;
+				}
+			} else if (t2) {
+				mid_rows = th;
+				for (unsigned j0 = 0; j0 < s.width; j0 += tw) {
+					const unsigned nc = std::min(tw, s.width - j0);
+					mid_cols = std::max(mid_cols, t2->left[j0 + nc - 1] + t2->filter_width - t2->left[j0]);
+				}
+			}
+			const size_t bytes = elem * (static_cast<size_t>(in_rows) * in_cols + static_cast<size_t>(mid_rows) * mid_cols);
+			if (bytes <= kLdsBudget) {
+				s.tile_w = tw;
+				s.tile_h = th;
+				s.lds_in_rows = in_rows;
+				s.lds_in_cols = in_cols;
+				s.lds_mid_rows = mid_rows;
+				s.lds_mid_cols = mid_cols;
+				return;
+			}
+		}
+	}
+	fail(ZIMG_ERROR_RESAMPLING_NOT_AVAILABLE, "resize tile does not fit in LDS");
+}
+
+size_t DeviceProgram::tmp_size(unsigned frames) const
+{
+	size_t total = 0;
+	for (auto &sl : m_slots)
+		total += align_up(sl.pitch * sl.height * frames, kPitchAlign);
+	return total;
+}
+
+const DeviceProgram::Tables &DeviceProgram::tables_for_current_device() const
+{
+	int device = 0;
+	check_hip(hipGetDevice(&device), "hipGetDevice");
+	std::lock_guard<std::mutex> lock(m_mutex);
+	auto it = m_tables.find(device);
+	if (it != m_tables.end())
+		return *it->second;
+
+	// One device allocation holds every pass table of the program.
+	std::vector<int> ids;
+	for (auto &s : m_steps)
+		if (s.kind == Step::RESIZE)
+			for (int p = 0; p < s.passes; ++p)
+				if (std::find(ids.begin(), ids.end(), s.tab[p]) == ids.end())
+					ids.push_back(s.tab[p]);
+
+	size_t bytes = 0;
+	std::vector<size_t> offs;
+	for (int id : ids) {
+		const FilterTable &t = *m_plan.nodes[id].table;
+		offs.push_back(bytes);
+		bytes += align_up(t.left.size() * 4, 256) + align_up(t.data_i16.size() * 2, 256) + align_up(t.data.size() * 4, 256);
+	}
+	auto tab = std::make_shared<Tables>();
+	if (bytes) {
+		std::vector<char> host(bytes, 0);
+		for (size_t n = 0; n < ids.size(); ++n) {
+			const FilterTable &t = *m_plan.nodes[ids[n]].table;
+			size_t o = offs[n];
+			std::memcpy(&host[o], t.left.data(), t.left.size() * 4);
+			const size_t o_i16 = o + align_up(t.left.size() * 4, 256);
+			std::memcpy(&host[o_i16], t.data_i16.data(), t.data_i16.size() * 2);
+			const size_t o_f32 = o_i16 + align_up(t.data_i16.size() * 2, 256);
+			std::memcpy(&host[o_f32], t.data.data(), t.data.size() * 4);
+		}
+		check_hip(hipMalloc(&tab->mem, bytes), "hipMalloc(tables)");
+		check_hip(hipMemcpy(tab->mem, host.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(tables)");
+		char *base = static_cast<char *>(tab->mem);
+		for (size_t n = 0; n < ids.size(); ++n) {
+			const FilterTable &t = *m_plan.nodes[ids[n]].table;
+			const size_t o = offs[n];
+			const size_t o_i16 = o + align_up(t.left.size() * 4, 256);
+			const size_t o_f32 = o_i16 + align_up(t.data_i16.size() * 2, 256);
+			tab->pass[ids[n]] = dev::PassArg{ reinterpret_cast<const unsigned *>(base + o), reinterpret_cast<const int16_t *>(base + o_i16),
+			                                  reinterpret_cast<const float *>(base + o_f32), static_cast<int>(t.filter_width),
+			                                  static_cast<int>(t.filter_rows) };
+		}
+	}
+	m_tables[device] = tab;
+	return *tab;
+}
+
+void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void *tmp, unsigned frames, void *stream) const
+{
+	if (!frames)
+		return;
+	if (frames > 65535)
+		fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "too many frames in one batch");
+	const Tables &tabs = tables_for_current_device();
+
+	std::vector<size_t> slot_off(m_slots.size());
+	size_t off = 0;
+	for (size_t k = 0; k < m_slots.size(); ++k) {
+		slot_off[k] = off;
+		off += align_up(m_slots[k].pitch * m_slots[k].height * frames, kPitchAlign);
+	}
+	if (off && !tmp)
+		fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "device tmp buffer required");
+
+	auto resolve = [&](const Buf &b) {
+		dev::PlaneArg p{};
+		switch (b.kind) {
+		case Buf::SRC:
+			p = dev::PlaneArg{ src.ptr[b.idx], src.stride[b.idx], src.fstride[b.idx] };
+			break;
+		case Buf::DST:
+			p = dev::PlaneArg{ dst.ptr[b.idx], dst.stride[b.idx], dst.fstride[b.idx] };
+			break;
+		case Buf::TMP: {
+			const Slot &sl = m_slots[b.idx];
+			p = dev::PlaneArg{ static_cast<char *>(tmp) + slot_off[b.idx], static_cast<long long>(sl.pitch),
+			                   static_cast<long long>(sl.pitch * sl.height) };
+			break;
+		}
+		default:
+			break;
+		}
+		if (b.kind != Buf::NONE && !p.ptr)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "missing plane buffer");
+		return p;
+	};
+
+	for (const Step &s : m_steps) {
+		int rc = 0;
+		switch (s.kind) {
+		case Step::RESIZE: {
+			dev::ResizeArg a{};
+			a.src = resolve(s.in[0]);
+			a.dst = resolve(s.out[0]);
+			a.frames = frames;
+			a.src_w = s.in_w;
+			a.src_h = s.in_h;
+			a.mid_w = s.mid_w;
+			a.mid_h = s.mid_h;
+			a.dst_w = s.width;
+			a.dst_h = s.height;
+			a.h_first = s.h_first ? 1 : 0;
+			a.passes = s.passes;
+			a.p1 = tabs.pass.at(s.tab[0]);
+			if (s.passes == 2)
+				a.p2 = tabs.pass.at(s.tab[1]);
+			a.word = s.word ? 1 : 0;
+			a.pixel_max = s.pixel_max;
+			a.in = s.ix[0];
+			a.out = s.ox[0];
+			a.tile_w = s.tile_w;
+			a.tile_h = s.tile_h;
+			a.lds_in_rows = s.lds_in_rows;
+			a.lds_in_cols = s.lds_in_cols;
+			a.lds_mid_rows = s.lds_mid_rows;
+			a.lds_mid_cols = s.lds_mid_cols;
+			rc = dev::launch_resize(a, stream);
+			break;
+		}
+		case Step::POINT: {
+			dev::PointArg a{};
+			if (!s.fill)
+				a.src = resolve(s.in[0]);
+			a.dst = resolve(s.out[0]);
+			a.frames = frames;
+			a.width = s.width;
+			a.height = s.height;
+			a.crop_left = s.crop_left;
+			a.crop_top = s.crop_top;
+			a.in = s.ix[0];
+			a.out = s.ox[0];
+			a.fill = s.fill;
+			a.fill_bits = s.fill_bits;
+			rc = dev::launch_point(a, stream);
+			break;
+		}
+		case Step::CSP: {
+			dev::CspArg a{};
+			for (int k = 0; k < 3; ++k) {
+				a.src[k] = resolve(s.in[k]);
+				a.dst[k] = resolve(s.out[k]);
+				a.in[k] = s.ix[k];
+				a.out[k] = s.ox[k];
+			}
+			a.frames = frames;
+			a.width = s.width;
+			a.height = s.height;
+			a.prog = s.prog;
+			rc = dev::launch_colorspace(a, stream);
+			break;
+		}
+		case Step::ED: {
+			dev::EdArg a{};
+			a.src = resolve(s.in[0]);
+			a.dst = resolve(s.out[0]);
+			a.frames = frames;
+			a.width = s.width;
+			a.height = s.height;
+			a.in = s.ix[0];
+			a.out = s.ox[0];
+			a.scale = s.ed_scale;
+			a.offset = s.ed_offset;
+			rc = dev::launch_error_diffusion(a, stream);
+			break;
+		}
+		}
+		if (rc != 0)
+			fail(ZIMG_ERROR_UNKNOWN, std::string("kernel launch failed: ") + s.label + ": " + hipGetErrorString(static_cast<hipError_t>(rc)));
+	}
+}
+
+std::string DeviceProgram::describe() const
+{
+	std::string r;
+	for (auto &s : m_steps) {
+		char buf[256];
+		static const char *kinds[] = { "resize", "point", "colorspace", "error_diffusion" };
+		if (s.kind == Step::RESIZE)
+			std::snprintf(buf, sizeof(buf), "%s %ux%u->%ux%u passes=%d %s tile=%ux%u lds=%zu: %s\n", kinds[s.kind], s.in_w, s.in_h, s.width,
+			              s.height, s.passes, s.h_first ? "h_first" : "v_first", s.tile_w, s.tile_h,
+			              (s.word ? 2 : 4) * (static_cast<size_t>(s.lds_in_rows) * s.lds_in_cols + static_cast<size_t>(s.lds_mid_rows) * s.lds_mid_cols),
+			              s.label.c_str());
+		else
+			std::snprintf(buf, sizeof(buf), "%s %ux%u: %s\n", kinds[s.kind], s.width, s.height, s.label.c_str());
+		r += buf;
+	}
+	return r;
+}
+
+} // namespace zamd

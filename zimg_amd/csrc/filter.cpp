@@ -1,0 +1,238 @@
+// Host-side resampling kernels and coefficient generation (FP64).
+// Reference: src/zimg/resize/filter.cpp. The steps and rounding rules are kept
+// identical because the device kernels must reproduce the reference's integer
+// output exactly, which fixes every coefficient bit.
+#include "filter.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdlib>
+
+#include "common.h"
+
+namespace zamd {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+double sinc(double x) { return x == 0.0 ? 1.0 : std::sin(x * kPi) / (x * kPi); } // filter.cpp:21-25
+
+double horner3(double x, double c0, double c1, double c2, double c3) { return c0 + x * (c1 + x * (c2 + x * c3)); }
+
+// Pixel-grid rounding that keeps round(x - 1) == round(x) - 1 (filter.cpp:32-40).
+double grid_round(double x) { return x < 0 ? std::floor(x + 0.5) : std::floor(x + 0.49999999999999994); }
+
+// One output row of the sparse resampling matrix. Matches the storage rule of
+// the reference's RowMatrix (common/matrix.cpp:14-21,104-129): a column joins
+// the stored span only when an assignment changes its value.
+class SparseRow {
+	std::vector<double> &m_dense;
+	size_t m_lo = 0, m_hi = 0;
+public:
+	explicit SparseRow(std::vector<double> &dense) : m_dense(dense) {}
+
+	double get(size_t j) const { return (j >= m_lo && j < m_hi) ? m_dense[j] : 0.0; }
+
+	void set(size_t j, double v)
+	{
+		if (get(j) == v)
+			return;
+		if (m_lo == m_hi) {
+			m_lo = j;
+			m_hi = j + 1;
+		} else {
+			m_lo = std::min(m_lo, j);
+			m_hi = std::max(m_hi, j + 1);
+		}
+		m_dense[j] = v;
+	}
+
+	size_t lo() const { return m_lo; }
+	size_t hi() const { return m_hi; }
+
+	void extract(std::vector<double> &out) const { out.assign(m_dense.begin() + m_lo, m_dense.begin() + m_hi); }
+	void clear() { std::fill(m_dense.begin() + m_lo, m_dense.begin() + m_hi, 0.0); m_lo = m_hi = 0; }
+};
+
+} // namespace
+
+unsigned ResampleFilter::support() const
+{
+	switch (type) {
+	case ZIMG_RESIZE_POINT: return 0;
+	case ZIMG_RESIZE_BILINEAR: return 1;
+	case ZIMG_RESIZE_BICUBIC: return 2;
+	case ZIMG_RESIZE_SPLINE16: return 2;
+	case ZIMG_RESIZE_SPLINE36: return 3;
+	case ZIMG_RESIZE_SPLINE64: return 4;
+	case ZIMG_RESIZE_LANCZOS: return static_cast<unsigned>(param_a);
+	default: return 0;
+	}
+}
+
+const char *ResampleFilter::name() const
+{
+	switch (type) {
+	case ZIMG_RESIZE_POINT: return "point";
+	case ZIMG_RESIZE_BILINEAR: return "bilinear";
+	case ZIMG_RESIZE_BICUBIC: return "bicubic";
+	case ZIMG_RESIZE_SPLINE16: return "spline16";
+	case ZIMG_RESIZE_SPLINE36: return "spline36";
+	case ZIMG_RESIZE_SPLINE64: return "spline64";
+	case ZIMG_RESIZE_LANCZOS: return "lanczos";
+	default: return "?";
+	}
+}
+
+double ResampleFilter::operator()(double x) const
+{
+	// Kernel definitions of filter.cpp:134-246.
+	switch (type) {
+	case ZIMG_RESIZE_POINT:
+		return 1.0;
+	case ZIMG_RESIZE_BILINEAR:
+		return std::max(1.0 - std::abs(x), 0.0);
+	case ZIMG_RESIZE_BICUBIC: {
+		const double b = param_a, c = param_b;
+		const double p0 = (6.0 - 2.0 * b) / 6.0;
+		const double p2 = (-18.0 + 12.0 * b + 6.0 * c) / 6.0;
+		const double p3 = (12.0 - 9.0 * b - 6.0 * c) / 6.0;
+		const double q0 = (8.0 * b + 24.0 * c) / 6.0;
+		const double q1 = (-12.0 * b - 48.0 * c) / 6.0;
+		const double q2 = (6.0 * b + 30.0 * c) / 6.0;
+		const double q3 = (-b - 6.0 * c) / 6.0;
+		x = std::abs(x);
+		if (x < 1.0) return horner3(x, p0, 0.0, p2, p3);
+		if (x < 2.0) return horner3(x, q0, q1, q2, q3);
+		return 0.0;
+	}
+	case ZIMG_RESIZE_SPLINE16:
+		x = std::abs(x);
+		if (x < 1.0) return horner3(x, 1.0, -1.0 / 5.0, -9.0 / 5.0, 1.0);
+		if (x < 2.0) return horner3(x - 1.0, 0.0, -7.0 / 15.0, 4.0 / 5.0, -1.0 / 3.0);
+		return 0.0;
+	case ZIMG_RESIZE_SPLINE36:
+		x = std::abs(x);
+		if (x < 1.0) return horner3(x, 1.0, -3.0 / 209.0, -453.0 / 209.0, 13.0 / 11.0);
+		if (x < 2.0) return horner3(x - 1.0, 0.0, -156.0 / 209.0, 270.0 / 209.0, -6.0 / 11.0);
+		if (x < 3.0) return horner3(x - 2.0, 0.0, 26.0 / 209.0, -45.0 / 209.0, 1.0 / 11.0);
+		return 0.0;
+	case ZIMG_RESIZE_SPLINE64:
+		x = std::abs(x);
+		if (x < 1.0) return horner3(x, 1.0, -3.0 / 2911.0, -6387.0 / 2911.0, 49.0 / 41.0);
+		if (x < 2.0) return horner3(x - 1.0, 0.0, -2328.0 / 2911.0, 4032.0 / 2911.0, -24.0 / 41.0);
+		if (x < 3.0) return horner3(x - 2.0, 0.0, 582.0 / 2911.0, -1008.0 / 2911.0, 6.0 / 41.0);
+		if (x < 4.0) return horner3(x - 3.0, 0.0, -97.0 / 2911.0, 168.0 / 2911.0, -1.0 / 41.0);
+		return 0.0;
+	case ZIMG_RESIZE_LANCZOS: {
+		const unsigned taps = static_cast<unsigned>(param_a);
+		x = std::abs(x);
+		return x < taps ? sinc(x) * sinc(x / taps) : 0.0;
+	}
+	default:
+		return 0.0;
+	}
+}
+
+bool FilterTable::sorted() const { return std::is_sorted(left.begin(), left.end()); }
+
+FilterTable compute_filter(const ResampleFilter &f, unsigned src_dim, unsigned dst_dim, double shift, double width)
+{
+	const double scale = static_cast<double>(dst_dim) / width;
+	const double step = std::min(scale, 1.0);
+	const double support = static_cast<double>(f.support()) / step;
+	const unsigned taps = std::max(static_cast<unsigned>(std::ceil(support)) * 2U, 1U);
+
+	if (support > static_cast<unsigned>(UINT_MAX / 2))
+		fail(ZIMG_ERROR_RESAMPLING_NOT_AVAILABLE, "filter width too great");
+
+	// Pass 1: the FP64 resampling matrix, one sparse row per output sample.
+	std::vector<double> dense(src_dim, 0.0);
+	std::vector<size_t> row_lo(dst_dim), row_hi(dst_dim);
+	std::vector<std::vector<double>> rows(dst_dim);
+	const double clamp_hi = std::nextafter(static_cast<double>(src_dim), -INFINITY);
+
+	for (unsigned i = 0; i < dst_dim; ++i) {
+		SparseRow row(dense);
+		const double pos = (i + 0.5) / scale + shift;
+		const double begin = grid_round(pos - taps / 2.0) + 0.5;
+
+		double total = 0.0;
+		for (unsigned j = 0; j < taps; ++j)
+			total += f((begin + j - pos) * step);
+
+		size_t leftmost = SIZE_MAX;
+		for (unsigned j = 0; j < taps; ++j) {
+			const double xpos = begin + j;
+			// Mirror at the image edges, then clamp what is still outside.
+			double src_pos = xpos < 0.0 ? -xpos : xpos >= src_dim ? 2.0 * src_dim - xpos : xpos;
+			src_pos = std::clamp(src_pos, 0.0, clamp_hi);
+
+			const size_t idx = static_cast<size_t>(std::floor(src_pos));
+			row.set(idx, row.get(idx) + f((xpos - pos) * step) / total);
+			leftmost = std::min(leftmost, idx);
+		}
+		// The leftmost tap always belongs to the stored span (filter.cpp:295-299).
+		if (row.get(leftmost) == 0.0) {
+			row.set(leftmost, DBL_EPSILON);
+			row.set(leftmost, 0.0);
+		}
+
+		row_lo[i] = row.lo();
+		row_hi[i] = row.hi();
+		row.extract(rows[i]);
+		row.clear();
+	}
+
+	// Pass 2: pack to a fixed width, rounding to f32 and Q1.14 with error feedback.
+	size_t fw = 0;
+	for (unsigned i = 0; i < dst_dim; ++i)
+		fw = std::max(fw, row_hi[i] - row_lo[i]);
+	if (!fw)
+		fail(ZIMG_ERROR_UNKNOWN, "empty resampling matrix");
+
+	FilterTable t;
+	t.filter_width = static_cast<unsigned>(fw);
+	t.filter_rows = dst_dim;
+	t.input_width = src_dim;
+	t.data.resize(fw * dst_dim);
+	t.data_i16.resize(fw * dst_dim);
+	t.left.resize(dst_dim);
+
+	for (unsigned i = 0; i < dst_dim; ++i) {
+		const size_t left = std::min(row_lo[i], static_cast<size_t>(src_dim) - fw);
+		double err_f32 = 0.0, err_i16 = 0.0;
+		int16_t sum_i16 = 0, peak_i16 = 0;
+		size_t peak_idx = 0;
+		float *drow = &t.data[i * fw];
+		int16_t *irow = &t.data_i16[i * fw];
+
+		for (size_t j = 0; j < fw; ++j) {
+			const size_t col = left + j;
+			const double c = (col >= row_lo[i] && col < row_hi[i]) ? rows[i][col - row_lo[i]] : 0.0;
+			const double want_f32 = c - err_f32;
+			const double want_i16 = c * (1 << 14) - err_i16;
+			const float q_f32 = static_cast<float>(want_f32);
+			const int16_t q_i16 = static_cast<int16_t>(std::lrint(want_i16));
+			err_f32 = static_cast<double>(q_f32) - want_f32;
+			err_i16 = static_cast<double>(q_i16) - want_i16;
+
+			if (std::abs(q_i16) > std::abs(peak_i16)) {
+				peak_i16 = q_i16;
+				peak_idx = j;
+			}
+			sum_i16 = static_cast<int16_t>(sum_i16 + q_i16);
+			drow[j] = q_f32;
+			irow[j] = q_i16;
+		}
+		// Integer taps sum to exactly 1.0 in Q1.14: the residue goes to the largest tap.
+		irow[peak_idx] = static_cast<int16_t>(irow[peak_idx] + ((1 << 14) - sum_i16));
+		t.left[i] = static_cast<unsigned>(left);
+	}
+	return t;
+}
+
+} // namespace zamd

@@ -1,0 +1,99 @@
+// Kernel launch interfaces of the gfx950 device path (implemented in kernels.hip).
+// All argument structs are plain data passed by value as kernel arguments.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#include "colorspace.h"
+
+namespace zamd {
+namespace dev {
+
+// A batch of planes in HBM: row r of frame f starts at ptr + f * fstride + r * stride.
+struct PlaneArg {
+	char *ptr;
+	long long stride;
+	long long fstride;
+};
+
+// Depth conversion applied while loading a stage's input.
+struct InXform {
+	int type;        // 0 BYTE, 1 WORD, 3 FLOAT (source storage)
+	int op;          // 0 none, 1 left shift into WORD, 2 integer -> float (x * scale + offset)
+	unsigned shift;
+	float scale, offset;
+};
+
+// Depth conversion applied while storing a stage's output.
+struct OutXform {
+	int type;        // storage type of the output plane
+	int op;          // 0 store as is, 1 quantise (x * scale + offset + dither, clamp, round to even)
+	int dither;      // 0 none, 1 ordered (Bayer 16x16)
+	int plane;       // Bayer table variant (plane index mod 4)
+	unsigned bits;   // output depth
+	float scale, offset;
+};
+
+// One resize pass on the device: packed coefficient rows (stride fw) and left offsets.
+struct PassArg {
+	const unsigned *left;
+	const int16_t *ci;
+	const float *cf;
+	int fw;
+	int rows;
+};
+
+struct ResizeArg {
+	PlaneArg src, dst;
+	unsigned frames;
+	unsigned src_w, src_h, mid_w, mid_h, dst_w, dst_h;
+	int h_first;        // 1: horizontal pass first
+	int passes;         // 1 or 2
+	PassArg p1, p2;
+	int word;           // 1: integer WORD arithmetic (Q1.14), 0: FLOAT
+	int pixel_max;      // WORD clamp
+	InXform in;
+	OutXform out;
+	unsigned tile_w, tile_h;
+	unsigned lds_in_rows, lds_in_cols;   // staged input tile (H-first)
+	unsigned lds_mid_rows, lds_mid_cols; // intermediate tile
+};
+
+struct PointArg {
+	PlaneArg src, dst;
+	unsigned frames, width, height;
+	unsigned crop_left, crop_top;
+	InXform in;
+	OutXform out;
+	int fill;           // 1: ignore src, write fill_bits
+	uint32_t fill_bits;
+};
+
+struct CspArg {
+	PlaneArg src[3], dst[3];
+	unsigned frames, width, height;
+	InXform in[3];
+	OutXform out[3];
+	CspProgram prog;
+};
+
+struct EdArg {
+	PlaneArg src, dst;
+	unsigned frames, width, height;
+	InXform in;        // load only (op must be 0); scale/offset of the dither below
+	OutXform out;      // op 1, dither ignored
+	float scale, offset;
+};
+
+// Launchers. Return hipSuccess (0) or the HIP error of the launch.
+int launch_resize(const ResizeArg &a, void *stream);
+int launch_point(const PointArg &a, void *stream);
+int launch_colorspace(const CspArg &a, void *stream);
+int launch_error_diffusion(const EdArg &a, void *stream);
+
+// LDS bytes the resize kernel needs for a tile configuration (host-side sizing).
+size_t resize_lds_bytes(const ResizeArg &a);
+
+} // namespace dev
+} // namespace zamd
