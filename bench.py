@@ -37,7 +37,8 @@ def parse():
     ap.add_argument("--frames", type=int, default=64, help="distinct frames per step (>256 MB of input+output defeats the MALL)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="bounded CPU baseline sample (seconds of work)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="fhd_709_to_2020", choices=["fhd_709_to_2020", "uhd_lanczos"])
+    ap.add_argument("--workload", default="fhd_709_to_2020", choices=sorted(WORKLOADS))
+    ap.add_argument("--secondary", default="uhd_lanczos", help="second workload reported in the same line ('' for none)")
     return ap.parse_args()
 
 
@@ -89,9 +90,25 @@ def time_steps(torch, launch, steps, warmup):
     return start.elapsed_time(end) / 1000.0  # seconds (events on the launch stream)
 
 
+WORKLOADS = {"fhd_709_to_2020", "uhd_lanczos", "fp32_down", "fp32_up", "uhd_pq_to_hlg"}
+
+
 def build_workload(name, frames):
     import zimg_amd as z
     from zimg_amd import workloads as W
+    if name in ("fp32_down", "fp32_up"):
+        sw, sh, dw, dh, filt = (3840, 2160, 1920, 1080, z.RESIZE_LANCZOS) if name == "fp32_down" else (1280, 720, 3840, 2160, z.RESIZE_BICUBIC)
+        s, d, p = W.fp32_formats(sw, sh, dw, dh, filt, 3.0 if filt == z.RESIZE_LANCZOS else float("nan"))
+        g = z.FilterGraph(s, d, p)
+        base = W.fp32_plane(sw, sh, seed=8)
+        planes = [np.stack([np.roll(base, 17 * k, axis=1) for k in range(frames)]), None, None, None]
+        return g, planes, [(dh, dw, np.float32), None, None, None], (sw * sh + dw * dh) * 4, "f32"
+    if name == "uhd_pq_to_hlg":
+        s, d, p = W.uhd_pq_to_hlg_formats()
+        g = z.FilterGraph(s, d, p)
+        R, G, B = W.pq_rgb(3840, 2160, seed=9)
+        planes = [np.stack([np.roll(c, 29 * k, axis=1) for k in range(frames)]) for c in (R, G, B)] + [None]
+        return g, planes, [(2160, 3840, np.uint16)] * 3 + [None], 3840 * 2160 * 3 * (4 + 2), "f32 in, u16 (12-bit) out"
     if name == "fhd_709_to_2020":
         s, d, p = W.fhd_709_to_2020_formats()
         g = z.FilterGraph(s, d, p)
@@ -162,10 +179,10 @@ def main():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
     primary = args.workload
-    secondary = "uhd_lanczos" if primary == "fhd_709_to_2020" else "fhd_709_to_2020"
+    secondary = args.secondary if args.secondary and args.secondary != primary else None
     m = measure(torch, primary, args.frames, args.steps, args.warmup)
-    m2 = measure(torch, secondary, min(args.frames, 32), max(args.steps // 2, 5), args.warmup)
-    cpu = None if args.no_cpu else cpu_baseline(primary, args.cpu_seconds)
+    m2 = measure(torch, secondary, min(args.frames, 32), max(args.steps // 2, 5), args.warmup) if secondary else None
+    cpu = None if args.no_cpu or primary not in ("fhd_709_to_2020", "uhd_lanczos") else cpu_baseline(primary, args.cpu_seconds)
 
     line = {
         "metric": "frames/s + HBM GB/s, UHD Lanczos & 709→2020; MFMA/learned vs plain at ≤0.5 LSB",
@@ -185,8 +202,9 @@ def main():
                      "frac": round(m["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": None,
                      "kernel": "whole graph per launch sequence" if m["launches"] > 1 else m["graph"][0]},
         "cpu_baseline": cpu,
-        "secondary": {"workload": secondary, "fps": round(m2["fps"], 1), "achieved_gbs": round(m2["achieved_gbs"], 1),
-                      "frac": round(m2["achieved_gbs"] / PEAK_HBM_GBS, 4), "launches": m2["launches"]},
+        "secondary": None if not m2 else {
+            "workload": secondary, "fps": round(m2["fps"], 1), "achieved_gbs": round(m2["achieved_gbs"], 1),
+            "frac": round(m2["achieved_gbs"] / PEAK_HBM_GBS, 4), "launches": m2["launches"]},
     }
     print(json.dumps(line))
 

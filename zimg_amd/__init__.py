@@ -115,6 +115,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built: run `make -C zimg_amd` (or __graft_entry__.build())")
+        # One HIP runtime per process: when torch (which bundles its own
+        # libamdhip64.so.7) provides the device buffers, load it first so the
+        # library binds to the same runtime instance by soname.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         vp, sz = ctypes.c_void_p, ctypes.c_size_t
         L.zimg_get_version_info.argtypes = [ctypes.POINTER(ctypes.c_uint)] * 3

@@ -43,6 +43,9 @@ public:
 		int pixel_max = 65535;
 		unsigned mid_w = 0, mid_h = 0;
 		unsigned tile_w = 0, tile_h = 0, lds_in_rows = 0, lds_in_cols = 0, lds_mid_rows = 0, lds_mid_cols = 0;
+		// fast kernel configuration (fast_fwh == 0: not eligible)
+		int fast_fwh = 0;
+		unsigned fast_tile_h = 0, fast_in_rows = 0, fast_in_cols = 0, fast_mid_rows = 0, fast_mid_cols = 0;
 		// POINT
 		unsigned crop_left = 0, crop_top = 0;
 		int fill = 0;

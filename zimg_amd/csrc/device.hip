@@ -306,6 +306,61 @@ void DeviceProgram::configure_resize(Step &s)
 	const size_t elem = s.word ? 2 : 4;
 	const bool sorted1 = t1.sorted();
 
+	// Fast kernel: sorted tables, horizontal taps within a register bucket.
+	const FilterTable *th_tab = s.h_first ? &t1 : t2;
+	const FilterTable *tv_tab = s.h_first ? t2 : &t1;
+	const bool sorted_all = sorted1 && (!t2 || t2->sorted());
+	int fwh = 0;
+	if (sorted_all) {
+		const unsigned need = th_tab ? th_tab->filter_width : 1;
+		for (int b : dev::kFastTapBuckets) {
+			if (need <= static_cast<unsigned>(b)) {
+				fwh = b;
+				break;
+			}
+		}
+	}
+	if (fwh) {
+		const unsigned tw = dev::kFastTileW;
+		auto even = [](unsigned x) { return (x + 1) & ~1u; };
+		for (unsigned th : { 32u, 16u, 8u, 4u, 2u, 1u }) {
+			unsigned in_rows = 0, in_cols = 0, mid_rows = 0, mid_cols = 0;
+			unsigned hcols = 0;
+			if (th_tab) {
+				for (unsigned j0 = 0; j0 < s.width; j0 += tw) {
+					const unsigned nc = std::min(tw, s.width - j0);
+					hcols = std::max(hcols, even(th_tab->left[j0 + nc - 1] + th_tab->filter_width - (th_tab->left[j0] & ~1u)));
+				}
+			}
+			if (s.h_first) {
+				unsigned span = 0;
+				for (unsigned i0 = 0; i0 < s.height; i0 += th) {
+					const unsigned nr = std::min(th, s.height - i0);
+					span = std::max(span, tv_tab ? tv_tab->left[i0 + nr - 1] + tv_tab->filter_width - tv_tab->left[i0] : nr);
+				}
+				in_rows = span;
+				in_cols = hcols;
+				if (t2) {
+					mid_rows = span;
+					mid_cols = tw;
+				}
+			} else if (t2) {
+				mid_rows = th;
+				mid_cols = hcols;
+			}
+			const size_t bytes = elem * (static_cast<size_t>(in_rows) * in_cols + static_cast<size_t>(mid_rows) * mid_cols);
+			if (bytes <= kLdsBudget) {
+				s.fast_fwh = fwh;
+				s.fast_tile_h = th;
+				s.fast_in_rows = in_rows;
+				s.fast_in_cols = in_cols;
+				s.fast_mid_rows = mid_rows;
+				s.fast_mid_cols = mid_cols;
+				break;
+			}
+		}
+	}
+
 	static const unsigned kWidths[] = { 256, 128, 64, 32, 16, 8, 4, 2, 1 };
 	static const unsigned kHeights[] = { 32, 16, 8, 4, 2, 1 };
 
@@ -486,12 +541,26 @@ void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void
 			a.pixel_max = s.pixel_max;
 			a.in = s.ix[0];
 			a.out = s.ox[0];
-			a.tile_w = s.tile_w;
-			a.tile_h = s.tile_h;
-			a.lds_in_rows = s.lds_in_rows;
-			a.lds_in_cols = s.lds_in_cols;
-			a.lds_mid_rows = s.lds_mid_rows;
-			a.lds_mid_cols = s.lds_mid_cols;
+			auto aligned = [](const dev::PlaneArg &p) {
+				return (reinterpret_cast<uintptr_t>(p.ptr) % 16) == 0 && (p.stride % 16) == 0 && (p.fstride % 16) == 0;
+			};
+			if (s.fast_fwh && aligned(a.src) && aligned(a.dst)) {
+				a.fwh = s.fast_fwh;
+				a.tile_w = dev::kFastTileW;
+				a.tile_h = s.fast_tile_h;
+				a.lds_in_rows = s.fast_in_rows;
+				a.lds_in_cols = s.fast_in_cols;
+				a.lds_mid_rows = s.fast_mid_rows;
+				a.lds_mid_cols = s.fast_mid_cols;
+			} else {
+				a.fwh = 0;
+				a.tile_w = s.tile_w;
+				a.tile_h = s.tile_h;
+				a.lds_in_rows = s.lds_in_rows;
+				a.lds_in_cols = s.lds_in_cols;
+				a.lds_mid_rows = s.lds_mid_rows;
+				a.lds_mid_cols = s.lds_mid_cols;
+			}
 			rc = dev::launch_resize(a, stream);
 			break;
 		}
@@ -554,9 +623,11 @@ std::string DeviceProgram::describe() const
 		char buf[256];
 		static const char *kinds[] = { "resize", "point", "colorspace", "error_diffusion" };
 		if (s.kind == Step::RESIZE)
-			std::snprintf(buf, sizeof(buf), "%s %ux%u->%ux%u passes=%d %s tile=%ux%u lds=%zu: %s\n", kinds[s.kind], s.in_w, s.in_h, s.width,
-			              s.height, s.passes, s.h_first ? "h_first" : "v_first", s.tile_w, s.tile_h,
+			std::snprintf(buf, sizeof(buf), "%s %ux%u->%ux%u passes=%d %s tile=%ux%u lds=%zu fast=%d(tile_h=%u lds=%zu): %s\n", kinds[s.kind],
+			              s.in_w, s.in_h, s.width, s.height, s.passes, s.h_first ? "h_first" : "v_first", s.tile_w, s.tile_h,
 			              (s.word ? 2 : 4) * (static_cast<size_t>(s.lds_in_rows) * s.lds_in_cols + static_cast<size_t>(s.lds_mid_rows) * s.lds_mid_cols),
+			              s.fast_fwh, s.fast_tile_h,
+			              (s.word ? 2 : 4) * (static_cast<size_t>(s.fast_in_rows) * s.fast_in_cols + static_cast<size_t>(s.fast_mid_rows) * s.fast_mid_cols),
 			              s.label.c_str());
 		else
 			std::snprintf(buf, sizeof(buf), "%s %ux%u: %s\n", kinds[s.kind], s.width, s.height, s.label.c_str());

@@ -58,7 +58,12 @@ struct ResizeArg {
 	unsigned tile_w, tile_h;
 	unsigned lds_in_rows, lds_in_cols;   // staged input tile (H-first)
 	unsigned lds_mid_rows, lds_mid_cols; // intermediate tile
+	int fwh;            // 0: generic kernel; else the fast kernel's horizontal tap capacity
 };
+
+// Horizontal tap capacities of the fast resize kernel instantiations.
+constexpr int kFastTapBuckets[] = { 4, 8, 12, 16, 24, 32 };
+constexpr unsigned kFastTileW = 512;
 
 struct PointArg {
 	PlaneArg src, dst;

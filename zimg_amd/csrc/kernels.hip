@@ -283,6 +283,343 @@ __global__ void __launch_bounds__(kThreads) k_resize(ResizeArg a)
 	}
 }
 
+// ---------------------------------------------------------------------------
+// Fast fused resize. Same arithmetic as k_resize, organised for throughput:
+// each thread owns two adjacent output columns for the whole tile, so the
+// horizontal taps of both columns live in registers and are reused on every
+// row; vertical taps are uniform per output row (scalar loads); LDS and HBM
+// are touched in column pairs (4 or 8 bytes per lane). Loops carry no integer
+// division. Requires sorted left offsets, <= FWH horizontal taps and planes
+// whose base pointers and row strides are multiples of 16 bytes.
+// ---------------------------------------------------------------------------
+
+template <bool WORD>
+struct Pair;
+template <>
+struct Pair<true> {
+	uint16_t a, b;
+};
+template <>
+struct Pair<false> {
+	float a, b;
+};
+
+template <bool WORD>
+__device__ __forceinline__ Pair<WORD> load_pair(const char *row, unsigned c, unsigned limit, const InXform &in)
+{
+	// Columns c (even) and c + 1; the second is read only when c + 1 < limit.
+	Pair<WORD> p;
+	const bool two = c + 1 < limit;
+	if constexpr (WORD) {
+		unsigned x0, x1 = 0;
+		if (in.type == 0) {
+			if (two) {
+				const uint16_t v = *reinterpret_cast<const uint16_t *>(row + c);
+				x0 = v & 0xFF;
+				x1 = v >> 8;
+			} else {
+				x0 = reinterpret_cast<const uint8_t *>(row)[c];
+			}
+		} else {
+			if (two) {
+				const uint32_t v = *reinterpret_cast<const uint32_t *>(row + 2 * static_cast<size_t>(c));
+				x0 = v & 0xFFFF;
+				x1 = v >> 16;
+			} else {
+				x0 = reinterpret_cast<const uint16_t *>(row)[c];
+			}
+		}
+		if (in.op == 1) {
+			x0 <<= in.shift;
+			x1 <<= in.shift;
+		}
+		p.a = static_cast<uint16_t>(x0);
+		p.b = static_cast<uint16_t>(x1);
+	} else {
+		float x0, x1 = 0.0f;
+		if (in.type == 3) {
+			if (two) {
+				const float2 v = *reinterpret_cast<const float2 *>(row + 4 * static_cast<size_t>(c));
+				x0 = v.x;
+				x1 = v.y;
+			} else {
+				x0 = reinterpret_cast<const float *>(row)[c];
+			}
+		} else {
+			x0 = load_as_float(row, c, in.type);
+			if (two)
+				x1 = load_as_float(row, c + 1, in.type);
+		}
+		if (in.op == 2) {
+			x0 = x0 * in.scale + in.offset;
+			x1 = x1 * in.scale + in.offset;
+		}
+		p.a = x0;
+		p.b = x1;
+	}
+	return p;
+}
+
+template <bool WORD>
+__device__ __forceinline__ void store_pair(char *row, unsigned j, lds_t<WORD> v0, lds_t<WORD> v1, bool two, const OutXform &o, unsigned i)
+{
+	if (o.op == 1) {
+		const unsigned q0 = quantize(static_cast<float>(v0), o, i, j);
+		if (!two) {
+			store_uint(row, j, o.type, q0);
+			return;
+		}
+		const unsigned q1 = quantize(static_cast<float>(v1), o, i, j + 1);
+		if (o.type == 0)
+			*reinterpret_cast<uint16_t *>(row + j) = static_cast<uint16_t>(q0 | (q1 << 8));
+		else
+			*reinterpret_cast<uint32_t *>(row + 2 * static_cast<size_t>(j)) = q0 | (q1 << 16);
+		return;
+	}
+	if constexpr (WORD) {
+		if (two)
+			*reinterpret_cast<uint32_t *>(row + 2 * static_cast<size_t>(j)) = static_cast<uint32_t>(v0) | (static_cast<uint32_t>(v1) << 16);
+		else
+			reinterpret_cast<uint16_t *>(row)[j] = v0;
+	} else {
+		if (two)
+			*reinterpret_cast<float2 *>(row + 4 * static_cast<size_t>(j)) = make_float2(v0, v1);
+		else
+			reinterpret_cast<float *>(row)[j] = v0;
+	}
+}
+
+// Per-thread horizontal taps of two columns held in registers.
+template <bool WORD, int FWH>
+struct HTaps {
+	using C = std::conditional_t<WORD, int32_t, float>;
+	C c0[FWH], c1[FWH];
+	int l0, l1;
+
+	__device__ __forceinline__ void load(const PassArg &p, unsigned j, bool two, int base)
+	{
+		const unsigned j1 = two ? j + 1 : j;
+		l0 = static_cast<int>(p.left[j]) - base;
+		l1 = static_cast<int>(p.left[j1]) - base;
+#pragma unroll
+		for (int k = 0; k < FWH; ++k) {
+			if (k < p.fw) {
+				if constexpr (WORD) {
+					c0[k] = p.ci[static_cast<size_t>(j) * p.fw + k];
+					c1[k] = p.ci[static_cast<size_t>(j1) * p.fw + k];
+				} else {
+					c0[k] = p.cf[static_cast<size_t>(j) * p.fw + k];
+					c1[k] = p.cf[static_cast<size_t>(j1) * p.fw + k];
+				}
+			}
+		}
+	}
+
+	__device__ __forceinline__ void apply(const lds_t<WORD> *x, int fw, int pixel_max, lds_t<WORD> &v0, lds_t<WORD> &v1) const
+	{
+		if constexpr (WORD) {
+			int32_t a0 = 0, a1 = 0;
+#pragma unroll
+			for (int k = 0; k < FWH; ++k) {
+				if (k < fw) {
+					a0 += c0[k] * (static_cast<int32_t>(x[l0 + k]) - 32768);
+					a1 += c1[k] * (static_cast<int32_t>(x[l1 + k]) - 32768);
+				}
+			}
+			v0 = pack_word(a0, pixel_max);
+			v1 = pack_word(a1, pixel_max);
+		} else {
+			float a0 = 0, a1 = 0;
+#pragma unroll
+			for (int k = 0; k < FWH; ++k) {
+				if (k < fw) {
+					a0 += c0[k] * x[l0 + k];
+					a1 += c1[k] * x[l1 + k];
+				}
+			}
+			v0 = a0;
+			v1 = a1;
+		}
+	}
+};
+
+template <bool WORD>
+__device__ __forceinline__ void v_taps_pair(const lds_t<WORD> *x, unsigned pitch, const PassArg &p, unsigned row, int pixel_max,
+                                            lds_t<WORD> &v0, lds_t<WORD> &v1)
+{
+	if constexpr (WORD) {
+		const int16_t *c = p.ci + static_cast<size_t>(row) * p.fw;
+		int32_t a0 = 0, a1 = 0;
+		for (int k = 0; k < p.fw; ++k) {
+			const uint32_t pr = *reinterpret_cast<const uint32_t *>(x + static_cast<size_t>(k) * pitch);
+			const int32_t ck = c[k];
+			a0 += ck * (static_cast<int32_t>(pr & 0xFFFF) - 32768);
+			a1 += ck * (static_cast<int32_t>(pr >> 16) - 32768);
+		}
+		v0 = pack_word(a0, pixel_max);
+		v1 = pack_word(a1, pixel_max);
+	} else {
+		const float *c = p.cf + static_cast<size_t>(row) * p.fw;
+		float a0 = 0, a1 = 0;
+		for (int k = 0; k < p.fw; ++k) {
+			const float2 pr = *reinterpret_cast<const float2 *>(x + static_cast<size_t>(k) * pitch);
+			a0 += c[k] * pr.x;
+			a1 += c[k] * pr.y;
+		}
+		v0 = a0;
+		v1 = a1;
+	}
+}
+
+template <bool WORD, int FWH>
+__global__ void __launch_bounds__(kThreads) k_resize_fast(ResizeArg a)
+{
+	using T = lds_t<WORD>;
+	extern __shared__ __align__(16) unsigned char smem[];
+	T *xin = reinterpret_cast<T *>(smem);
+	T *mid = xin + static_cast<size_t>(a.lds_in_rows) * a.lds_in_cols;
+
+	const unsigned t = threadIdx.x;
+	const unsigned f = blockIdx.z;
+	const unsigned j0 = blockIdx.x * a.tile_w, i0 = blockIdx.y * a.tile_h;
+	const unsigned nrows = min(a.tile_h, a.dst_h - i0);
+	const unsigned ncols = min(a.tile_w, a.dst_w - j0);
+	const char *src = frame_ptr(a.src, f);
+	char *dst = frame_ptr_w(a.dst, f);
+	const unsigned jc = 2 * t;            // this thread's column pair inside the tile
+	const bool active = jc < ncols;
+	const bool two = jc + 1 < ncols;
+	const unsigned j = j0 + (active ? jc : 0);
+
+	if (a.h_first) {
+		const PassArg &ph = a.p1;
+		const unsigned r0 = a.passes == 2 ? a.p2.left[i0] : i0;
+		const unsigned r1 = a.passes == 2 ? a.p2.left[i0 + nrows - 1] + a.p2.fw : i0 + nrows;
+		const unsigned c0 = ph.left[j0] & ~1u;
+		const unsigned c1 = ph.left[j0 + ncols - 1] + ph.fw;
+		const unsigned pin = a.lds_in_cols;
+		const unsigned npairs = (c1 - c0 + 1) / 2;
+
+		// Stage source rows [r0, r1) x cols [c0, c1) (c0 even) in LDS, in column pairs.
+		for (unsigned r = 0; r < r1 - r0; ++r) {
+			const char *srow = row_ptr(src, a.src.stride, r0 + r);
+			for (unsigned q = t; q < npairs; q += kThreads) {
+				const Pair<WORD> p = load_pair<WORD>(srow, c0 + 2 * q, a.src_w, a.in);
+				T *d = xin + r * pin + 2 * q;
+				d[0] = p.a;
+				d[1] = p.b;
+			}
+		}
+		__syncthreads();
+
+		HTaps<WORD, FWH> taps;
+		taps.load(ph, j, two, static_cast<int>(c0));
+		const unsigned span = r1 - r0;
+		const unsigned pm = a.lds_mid_cols;
+		for (unsigned r = 0; r < span; ++r) {
+			T v0, v1;
+			taps.apply(xin + r * pin, ph.fw, a.pixel_max, v0, v1);
+			if (!active)
+				continue;
+			if (a.passes == 2) {
+				mid[r * pm + jc] = v0;
+				mid[r * pm + jc + 1] = v1;
+			} else {
+				store_pair<WORD>(row_ptr(dst, a.dst.stride, r0 + r), j, v0, v1, two, a.out, r0 + r);
+			}
+		}
+		if (a.passes == 2) {
+			__syncthreads();
+			const PassArg &pv = a.p2;
+			for (unsigned i = 0; i < nrows; ++i) {
+				const unsigned top = pv.left[i0 + i] - r0;
+				T v0, v1;
+				v_taps_pair<WORD>(mid + top * pm + jc, pm, pv, i0 + i, a.pixel_max, v0, v1);
+				if (active)
+					store_pair<WORD>(row_ptr(dst, a.dst.stride, i0 + i), j, v0, v1, two, a.out, i0 + i);
+			}
+		}
+	} else if (a.passes == 1) {
+		// Vertical pass only: column pairs straight from HBM to HBM.
+		const PassArg &pv = a.p1;
+		for (unsigned i = 0; i < nrows; ++i) {
+			const unsigned top = pv.left[i0 + i];
+			T v0, v1;
+			if constexpr (WORD) {
+				const int16_t *cv = pv.ci + static_cast<size_t>(i0 + i) * pv.fw;
+				int32_t a0 = 0, a1 = 0;
+				for (int k = 0; k < pv.fw; ++k) {
+					const Pair<true> p = load_pair<true>(row_ptr(src, a.src.stride, top + k), j, j0 + ncols, a.in);
+					a0 += static_cast<int32_t>(cv[k]) * (static_cast<int32_t>(p.a) - 32768);
+					a1 += static_cast<int32_t>(cv[k]) * (static_cast<int32_t>(p.b) - 32768);
+				}
+				v0 = pack_word(a0, a.pixel_max);
+				v1 = pack_word(a1, a.pixel_max);
+			} else {
+				const float *cv = pv.cf + static_cast<size_t>(i0 + i) * pv.fw;
+				float a0 = 0, a1 = 0;
+				for (int k = 0; k < pv.fw; ++k) {
+					const Pair<false> p = load_pair<false>(row_ptr(src, a.src.stride, top + k), j, j0 + ncols, a.in);
+					a0 += cv[k] * p.a;
+					a1 += cv[k] * p.b;
+				}
+				v0 = a0;
+				v1 = a1;
+			}
+			if (active)
+				store_pair<WORD>(row_ptr(dst, a.dst.stride, i0 + i), j, v0, v1, two, a.out, i0 + i);
+		}
+	} else {
+		// V first: vertical pass from HBM into the intermediate tile (the source
+		// columns the horizontal pass needs), then horizontal pass to HBM.
+		const PassArg &pv = a.p1;
+		const PassArg &ph = a.p2;
+		const unsigned c0 = ph.left[j0] & ~1u;
+		const unsigned c1 = ph.left[j0 + ncols - 1] + ph.fw;
+		const unsigned pm = a.lds_mid_cols;
+		const unsigned npairs = (c1 - c0 + 1) / 2;
+		for (unsigned q = t; q < npairs; q += kThreads) {
+			const unsigned c = c0 + 2 * q;
+			for (unsigned i = 0; i < nrows; ++i) {
+				const unsigned top = pv.left[i0 + i];
+				T v0, v1;
+				if constexpr (WORD) {
+					const int16_t *cv = pv.ci + static_cast<size_t>(i0 + i) * pv.fw;
+					int32_t a0 = 0, a1 = 0;
+					for (int k = 0; k < pv.fw; ++k) {
+						const Pair<true> p = load_pair<true>(row_ptr(src, a.src.stride, top + k), c, a.src_w, a.in);
+						a0 += static_cast<int32_t>(cv[k]) * (static_cast<int32_t>(p.a) - 32768);
+						a1 += static_cast<int32_t>(cv[k]) * (static_cast<int32_t>(p.b) - 32768);
+					}
+					v0 = pack_word(a0, a.pixel_max);
+					v1 = pack_word(a1, a.pixel_max);
+				} else {
+					const float *cv = pv.cf + static_cast<size_t>(i0 + i) * pv.fw;
+					float a0 = 0, a1 = 0;
+					for (int k = 0; k < pv.fw; ++k) {
+						const Pair<false> p = load_pair<false>(row_ptr(src, a.src.stride, top + k), c, a.src_w, a.in);
+						a0 += cv[k] * p.a;
+						a1 += cv[k] * p.b;
+					}
+					v0 = a0;
+					v1 = a1;
+				}
+				mid[i * pm + 2 * q] = v0;
+				mid[i * pm + 2 * q + 1] = v1;
+			}
+		}
+		__syncthreads();
+		HTaps<WORD, FWH> taps;
+		taps.load(ph, j, two, static_cast<int>(c0));
+		for (unsigned i = 0; i < nrows; ++i) {
+			T v0, v1;
+			taps.apply(mid + i * pm, ph.fw, a.pixel_max, v0, v1);
+			if (active)
+				store_pair<WORD>(row_ptr(dst, a.dst.stride, i0 + i), j, v0, v1, two, a.out, i0 + i);
+		}
+	}
+}
+
 // Point stages: depth conversion, ordered dither, sub-rectangle copy, fill.
 __global__ void __launch_bounds__(kThreads) k_point(PointArg a)
 {
@@ -505,7 +842,17 @@ int launch_resize_t(const ResizeArg &a, void *stream)
 {
 	dim3 grid((a.dst_w + a.tile_w - 1) / a.tile_w, (a.dst_h + a.tile_h - 1) / a.tile_h, a.frames);
 	const size_t lds = resize_lds_bytes(a);
-	hipLaunchKernelGGL(k_resize<WORD>, grid, dim3(kThreads), lds, static_cast<hipStream_t>(stream), a);
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	switch (a.fwh) {
+	case 0: hipLaunchKernelGGL(k_resize<WORD>, grid, dim3(kThreads), lds, s, a); break;
+	case 4: hipLaunchKernelGGL((k_resize_fast<WORD, 4>), grid, dim3(kThreads), lds, s, a); break;
+	case 8: hipLaunchKernelGGL((k_resize_fast<WORD, 8>), grid, dim3(kThreads), lds, s, a); break;
+	case 12: hipLaunchKernelGGL((k_resize_fast<WORD, 12>), grid, dim3(kThreads), lds, s, a); break;
+	case 16: hipLaunchKernelGGL((k_resize_fast<WORD, 16>), grid, dim3(kThreads), lds, s, a); break;
+	case 24: hipLaunchKernelGGL((k_resize_fast<WORD, 24>), grid, dim3(kThreads), lds, s, a); break;
+	case 32: hipLaunchKernelGGL((k_resize_fast<WORD, 32>), grid, dim3(kThreads), lds, s, a); break;
+	default: return static_cast<int>(hipErrorInvalidValue);
+	}
 	return static_cast<int>(hipGetLastError());
 }
 

@@ -15,7 +15,9 @@
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
-#define ZHD __host__ __device__ inline
+// Always inlined: the transfer dispatch is on a wave-uniform id, so inlining
+// turns it into one scalar branch instead of a call with spilled registers.
+#define ZHD __host__ __device__ __attribute__((always_inline)) inline
 #define ZCONST __constant__
 #else
 #define ZHD inline
