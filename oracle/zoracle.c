@@ -743,6 +743,23 @@ void orc_convert_to_float(int type_in, const void *src, ptrdiff_t src_stride,
 	}
 }
 
+void orc_convert_to_half(int type_in, const void *src, ptrdiff_t src_stride,
+                         uint16_t *dst, ptrdiff_t dst_stride, unsigned width, unsigned height,
+                         float scale, float offset)
+{
+	/* depth_convert.cpp:32-38 (integer_to_half) and 58-63 (float_to_half_n) */
+	for (unsigned i = 0; i < height; ++i) {
+		const void *s = CROWP(char, src, src_stride, i);
+		uint16_t *d = ROWP(uint16_t, dst, dst_stride, i);
+		for (unsigned j = 0; j < width; ++j) {
+			if (type_in == 3)
+				d[j] = orc_float_to_half(orc_load(3, s, j));
+			else
+				d[j] = orc_float_to_half(orc_load(type_in, s, j) * scale + offset);
+		}
+	}
+}
+
 void orc_left_shift(int type_in, const void *src, ptrdiff_t src_stride,
                     int type_out, void *dst, ptrdiff_t dst_stride,
                     unsigned width, unsigned height, unsigned shift)

@@ -90,6 +90,11 @@ uint16_t orc_float_to_half(float f);
 void orc_convert_to_float(int type_in, const void *src, ptrdiff_t src_stride,
                           float *dst, ptrdiff_t dst_stride, unsigned width, unsigned height,
                           float scale, float offset);
+/* Conversion into HALF storage (depth_convert.cpp:32-63): integers take
+ * float_to_half(x * scale + offset), FLOAT input float_to_half(x). */
+void orc_convert_to_half(int type_in, const void *src, ptrdiff_t src_stride,
+                         uint16_t *dst, ptrdiff_t dst_stride, unsigned width, unsigned height,
+                         float scale, float offset);
 void orc_left_shift(int type_in, const void *src, ptrdiff_t src_stride,
                     int type_out, void *dst, ptrdiff_t dst_stride,
                     unsigned width, unsigned height, unsigned shift);

@@ -91,6 +91,9 @@ def oracle():
         lib.orc_half_to_float.argtypes = [ctypes.c_uint16]
         lib.orc_float_to_half.restype = ctypes.c_uint16
         lib.orc_float_to_half.argtypes = [ctypes.c_float]
+        lib.orc_convert_to_half.restype = None
+        lib.orc_convert_to_half.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_ssize_t, ctypes.c_void_p,
+                                            ctypes.c_ssize_t, ctypes.c_uint, ctypes.c_uint, ctypes.c_float, ctypes.c_float]
         lib.orc_convert_to_float.restype = None
         lib.orc_convert_to_float.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_ssize_t, c_float_p, ctypes.c_ssize_t,
                                              ctypes.c_uint, ctypes.c_uint, ctypes.c_float, ctypes.c_float]
@@ -260,6 +263,25 @@ def to_float(img, type_in, scale, offset):
     oracle().orc_convert_to_float(type_in, img.ctypes.data, img.strides[0], ptr(out, c_float_p), out.strides[0],
                                   img.shape[1], img.shape[0], scale, offset)
     return out
+
+
+def to_half(img, type_in, scale=1.0, offset=0.0):
+    """HALF storage bits (uint16) of an integer or FLOAT plane."""
+    img = np.ascontiguousarray(img)
+    out = np.zeros(img.shape, np.uint16)
+    oracle().orc_convert_to_half(type_in, img.ctypes.data, img.strides[0], out.ctypes.data, out.strides[0],
+                                 img.shape[1], img.shape[0], scale, offset)
+    return out
+
+
+def half_to_float(bits):
+    """FLOAT values of HALF storage bits (uint16 array), as the reference widens them."""
+    return to_float(np.ascontiguousarray(bits, np.uint16), HALF, 1.0, 0.0)
+
+
+def blue_noise():
+    """The 64x64 RANDOM-dither threshold map (fixture taken from the reference table)."""
+    return np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "blue_noise_64x64.npy"))
 
 
 def dither(img, type_in, type_out, scale, offset, bits, dither_type=0, plane=0, blue_noise=None):

@@ -10,7 +10,7 @@ import pytest
 
 import pipelines as P
 import zimg_amd as z
-from oracle_lib import (BICUBIC, BILINEAR, BYTE, FLOAT, LANCZOS, POINT, SPLINE16, SPLINE36, SPLINE64, WORD, dither,
+from oracle_lib import (BICUBIC, BILINEAR, BYTE, FLOAT, LANCZOS, POINT, SPLINE16, SPLINE36, SPLINE64, WORD, blue_noise, dither,
                         left_shift, resize_plane, scale_offset)
 from zimg_amd import workloads as W
 
@@ -127,10 +127,11 @@ def _yuv420_byte(w, h, seed):
     return Y, U, V
 
 
-@pytest.mark.parametrize("src_type", ["byte", "word10"])
-def test_yuv420_resize_ordered_dither_to_byte(src_type):
-    """4:2:0 u8 or 10-bit -> resized -> 8-bit with Bayer ordered dither (batched workload chain)."""
-    print(f"case: 4:2:0 {src_type} 1280x720->1920x1080 spline36 ordered dither", flush=True)
+@pytest.mark.parametrize("src_type,dither_type", [("byte", 1), ("word10", 1), ("word10", 2)])
+def test_yuv420_resize_ordered_dither_to_byte(src_type, dither_type):
+    """4:2:0 u8 or 10-bit -> resized -> 8-bit with Bayer (1) or blue-noise (2)
+    dither folded into the resize store (batched workload chain)."""
+    print(f"case: 4:2:0 {src_type} 1280x720->1920x1080 spline36 dither {dither_type}", flush=True)
     sw, sh, dw, dh = 1280, 720, 1920, 1080
     Y8, U8, V8 = _yuv420_byte(sw, sh, 5)
     if src_type == "byte":
@@ -141,7 +142,7 @@ def test_yuv420_resize_ordered_dither_to_byte(src_type):
                   transfer_characteristics=z.TRANSFER_BT709, color_primaries=z.PRIMARIES_BT709)
     s = z.image_format(width=sw, height=sh, pixel_type=fmt_t, depth=depth, **common)
     d = z.image_format(width=dw, height=dh, pixel_type=z.PIXEL_BYTE, depth=8, **common)
-    p = z.builder_params(resample_filter=z.RESIZE_SPLINE36, resample_filter_uv=z.RESIZE_SPLINE36, dither_type=z.DITHER_ORDERED)
+    p = z.builder_params(resample_filter=z.RESIZE_SPLINE36, resample_filter_uv=z.RESIZE_SPLINE36, dither_type=dither_type)
     g = z.FilterGraph(s, d, p)
     outs = gpu_util.run_graph(g, list(planes) + [None], [(dh, dw, np.uint8), (dh // 2, dw // 2, np.uint8), (dh // 2, dw // 2, np.uint8), None])
 
@@ -164,7 +165,7 @@ def test_yuv420_resize_ordered_dither_to_byte(src_type):
         r = resize_plane(x, SPLINE36, 0.0, 0.0, cw_d, ch_d, shift_w=shift_w, shift_h=0.0, subwidth=float(cw_s),
                          subheight=float(ch_s), depth=wdepth)
         sc, of = scale_offset((WORD, wdepth, 0, int(chroma), 0), (BYTE, 8, 0, int(chroma), 0))
-        want = dither(r, WORD, BYTE, sc, of, 8, dither_type=1, plane=q)
+        want = dither(r, WORD, BYTE, sc, of, 8, dither_type=dither_type, plane=q, blue_noise=blue_noise())
         _assert_equal(got, want, f"plane {q}")
 
 

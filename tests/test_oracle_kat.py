@@ -12,8 +12,11 @@ import re
 import numpy as np
 import pytest
 
-from oracle_lib import (BICUBIC, BILINEAR, BYTE, FLOAT, LANCZOS, POINT, SPLINE16, SPLINE36, SPLINE64, WORD, c_u8_p, dither,
-                        oracle, oracle_filter, ptr, resize_h, resize_v, scale_offset, to_float)
+from oracle_lib import (BICUBIC, BILINEAR, BYTE, FLOAT, HALF, LANCZOS, POINT, SPLINE16, SPLINE36, SPLINE64, WORD, blue_noise,
+                        c_u8_p, dither, half_to_float, oracle, oracle_filter, ptr, resize_h, resize_v, scale_offset, to_float,
+                        to_half)
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 f_eval = lambda t, a, b, x: oracle().orc_filter_eval(t, a, b, x)
 
@@ -143,6 +146,59 @@ def test_half_conversions_ieee():
     f = np.concatenate([rng.standard_normal(50000) * 100, rng.standard_normal(50000) * 1e-5, [65504, 65519, 1e-8, -0.0]]).astype(np.float32)
     got = np.array([o.orc_float_to_half(float(v)) for v in f], np.uint16)
     assert np.array_equal(got, f.astype(np.float16).view(np.uint16))
+
+
+def test_half_nan_and_plane_conversions():
+    """NaN rules of quantize.cpp (payload kept, made quiet) and the plane forms:
+    integer_to_half = float_to_half(x * scale + offset) (two float roundings,
+    then one half rounding), float_to_half_n ignores scale/offset."""
+    h = np.arange(65536, dtype=np.uint16)
+    f = half_to_float(h[None, :])[0].view(np.uint32)
+    nan = ((h & 0x7C00) == 0x7C00) & ((h & 0x3FF) != 0)
+    want = ((h.astype(np.uint32) & 0x8000) << 16) | 0x7FC00000 | ((h.astype(np.uint32) & 0x3FF) << 13)
+    assert np.array_equal(f[nan], want[nan])
+    assert np.array_equal(f[~nan], h[~nan].view(np.float16).astype(np.float32).view(np.uint32))
+    # float -> half on NaNs with payloads (including signalling ones, read from memory)
+    bits = np.array([0x7F800001, 0x7FA00000, 0xFFC00001, 0x7FFFFFFF, 0x7F802000, 0xFF800000, 0x7F800000], np.uint32)
+    got = to_half(bits.view(np.float32)[None, :], FLOAT)[0]
+    exp = ((bits >> 16) & 0x8000) | 0x7C00 | np.where((bits & 0x7FFFFF) != 0, 0x200 | ((bits & 0x7FFFFF) >> 13), 0)
+    assert got.tolist() == exp.astype(np.uint16).tolist()
+    rng = np.random.default_rng(11)
+    ints = rng.integers(0, 1024, (7, 300)).astype(np.uint16)
+    s, o = scale_offset((WORD, 10, 0, 1, 0), (HALF, 16, 0, 1, 0))
+    got = to_half(ints, WORD, s, o)
+    x = (ints.astype(np.float32) * np.float32(s)).astype(np.float32) + np.float32(o)
+    assert np.array_equal(got, x.astype(np.float32).astype(np.float16).view(np.uint16))
+    fl = (rng.standard_normal((5, 333)) * 3).astype(np.float32)
+    assert np.array_equal(to_half(fl, FLOAT, 7.0, 5.0), fl.astype(np.float16).view(np.uint16))
+
+
+def test_blue_noise_fixture_and_product_table():
+    """The RANDOM-dither map: 64x64, every byte value exactly 16 times (a
+    rank-ordered threshold map), and the product's copy is the same data."""
+    t = blue_noise()
+    assert t.shape == (64, 64) and t.dtype == np.uint8
+    assert np.array_equal(np.bincount(t.ravel(), minlength=256), np.full(256, 16))
+    text = open(os.path.join(ROOT, "zimg_amd", "csrc", "bluenoise.h")).read()
+    body = text[text.index("#define ZAMD_BLUE_NOISE_BYTES"):].split("\\\n", 1)[1]
+    nums = [int(v) for v in re.findall(r"\d+", body)]
+    assert nums == t.ravel().tolist()
+
+
+def test_random_dither_formula():
+    """dither.cpp:219-232 with dither_ordered: variant p starts the map at
+    column offset (0, 32, 16, 48)[p] and row offset (0, 12, 55, 26)[p]."""
+    rng = np.random.default_rng(5)
+    img = rng.random((131, 150), dtype=np.float32)
+    s, o = scale_offset((FLOAT, 32, 1, 0, 0), (BYTE, 8, 1, 0, 0))
+    t = blue_noise()
+    tab = ((t.astype(np.float32) + np.float32(1)) / np.float32(257) - np.float32(0.5)).astype(np.float32)
+    for plane, (hoff, voff) in enumerate([(0, 0), (32, 12), (16, 55), (48, 26)]):
+        got = dither(img, FLOAT, BYTE, s, o, 8, dither_type=2, plane=plane, blue_noise=t)
+        d = tab[(np.arange(131)[:, None] + voff) % 64, (np.arange(150)[None, :] + hoff) % 64]
+        x = (img * np.float32(s)).astype(np.float32) + np.float32(o)
+        x = np.clip((x + d).astype(np.float32), 0, 255)
+        assert np.array_equal(got, np.rint(x).astype(np.uint8)), plane
 
 
 def test_bayer_table():

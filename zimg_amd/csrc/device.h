@@ -52,8 +52,9 @@ public:
 		uint32_t fill_bits = 0;
 		// CSP
 		CspProgram prog;
-		// ED
-		float ed_scale = 1.0f, ed_offset = 0.0f;
+		// ED (planes of equal size share one launch)
+		int ed_planes = 1;
+		float ed_scale[3] = { 1.0f, 1.0f, 1.0f }, ed_offset[3] = { 0.0f, 0.0f, 0.0f };
 		bool dead = false;
 		std::string label;
 	};

@@ -138,24 +138,33 @@ void DeviceProgram::lower()
 				s.ix[0].shift = n.shift;
 				std::snprintf(label, sizeof(label), "shift<<%u[%d]", n.shift, n.plane);
 			} else if (n.depth_op == Node::TO_FLOAT) {
+				// select_depth_convert_func (depth_convert.cpp:80-100): integers take
+				// x * scale + offset; HALF <-> FLOAT is an exact conversion.
 				s.kind = Step::POINT;
-				s.ix[0].op = 2;
-				s.ix[0].scale = n.scale;
-				s.ix[0].offset = n.offset;
-				std::snprintf(label, sizeof(label), "to_float[%d]", n.plane);
+				if (pixel_is_float(in_type)) {
+					if (in_type == n.out.type)
+						fail(ZIMG_ERROR_UNKNOWN, "no conversion between pixel types");
+					s.ix[0].op = 3;
+				} else {
+					s.ix[0].op = 2;
+					s.ix[0].scale = n.scale;
+					s.ix[0].offset = n.offset;
+				}
+				std::snprintf(label, sizeof(label), "%s[%d]", n.out.type == PixelType::HALF ? "to_half" : "to_float", n.plane);
 			} else if (n.dither == Dither::ERROR_DIFFUSION) {
 				s.kind = Step::ED;
-				s.ed_scale = n.scale;
-				s.ed_offset = n.offset;
+				s.ed_scale[0] = n.scale;
+				s.ed_offset[0] = n.offset;
 				s.ox[0].op = 1;
 				std::snprintf(label, sizeof(label), "error_diffusion[%d]", n.plane);
 			} else {
 				s.kind = Step::POINT;
 				s.ox[0].op = 1;
-				s.ox[0].dither = n.dither == Dither::ORDERED ? 1 : 0;
+				s.ox[0].dither = n.dither == Dither::ORDERED ? 1 : n.dither == Dither::RANDOM ? 2 : 0;
 				s.ox[0].scale = n.scale;
 				s.ox[0].offset = n.offset;
-				std::snprintf(label, sizeof(label), "%s[%d]", n.dither == Dither::ORDERED ? "dither_bayer" : "quantize", n.plane);
+				std::snprintf(label, sizeof(label), "%s[%d]",
+				              n.dither == Dither::ORDERED ? "dither_bayer" : n.dither == Dither::RANDOM ? "dither_blue" : "quantize", n.plane);
 			}
 			s.label = label;
 			break;
@@ -230,7 +239,15 @@ void DeviceProgram::fuse()
 	};
 	auto plain_in = [](const dev::InXform &x) { return x.op == 0; };
 	auto plain_out = [](const dev::OutXform &x) { return x.op == 0; };
-	auto is_point_depth_in = [](const Step &s) { return s.kind == Step::POINT && !s.fill && !s.crop_left && !s.crop_top && s.ix[0].op != 0 && s.ox[0].op == 0; };
+	constexpr int kFloat = 3, kHalf = 2;
+	auto is_point_depth_in = [&](const Step &s) {
+		return s.kind == Step::POINT && !s.fill && !s.crop_left && !s.crop_top && s.ix[0].op != 0 && s.ox[0].op == 0 &&
+		       (s.ix[0].op == 1 || s.ox[0].type == kFloat);
+	};
+	auto is_point_to_half = [&](const Step &s) {
+		return s.kind == Step::POINT && !s.fill && !s.crop_left && !s.crop_top && s.ix[0].op == 3 && s.ix[0].type == kFloat &&
+		       s.ox[0].op == 0 && s.ox[0].type == kHalf;
+	};
 	auto is_point_quant = [](const Step &s) { return s.kind == Step::POINT && !s.fill && !s.crop_left && !s.crop_top && s.ix[0].op == 0 && s.ox[0].op == 1; };
 
 	bool changed = true;
@@ -258,7 +275,8 @@ void DeviceProgram::fuse()
 
 					// depth-in (shift / to-float) folded into a resize or colourspace input
 					if (is_point_depth_in(A) && (B.kind == Step::RESIZE || B.kind == Step::CSP) && plain_in(B.ix[kin])) {
-						const bool ok = B.kind == Step::CSP ? A.ix[0].op == 2 : (B.word ? A.ix[0].op == 1 : A.ix[0].op == 2);
+						const bool to_float = A.ix[0].op == 2 || A.ix[0].op == 3;
+						const bool ok = B.kind == Step::CSP ? to_float : (B.word ? A.ix[0].op == 1 : to_float);
 						if (ok) {
 							B.in[kin] = A.in[0];
 							B.ix[kin] = A.ix[0];
@@ -285,8 +303,9 @@ void DeviceProgram::fuse()
 						A.dead = true;
 						changed = true;
 					}
-					// quantisation / dither folded into the producer's store
-					else if ((A.kind == Step::RESIZE || A.kind == Step::CSP) && plain_out(A.ox[ko]) && is_point_quant(B)) {
+					// quantisation / dither / FLOAT -> HALF folded into the producer's store
+					else if ((A.kind == Step::RESIZE || A.kind == Step::CSP) && plain_out(A.ox[ko]) &&
+					         (is_point_quant(B) || (is_point_to_half(B) && A.ox[ko].type == kFloat))) {
 						A.out[ko] = B.out[0];
 						A.ox[ko] = B.ox[0];
 						A.label += "+" + B.label;
@@ -295,6 +314,38 @@ void DeviceProgram::fuse()
 					}
 				}
 			}
+		}
+	}
+
+	// Error diffusion of equally sized planes runs as one launch (one workgroup
+	// per plane and frame), provided nothing in between touches their buffers.
+	for (size_t ia = 0; ia < m_steps.size(); ++ia) {
+		Step &A = m_steps[ia];
+		if (A.dead || A.kind != Step::ED)
+			continue;
+		for (size_t ib = ia + 1; ib < m_steps.size() && A.ed_planes < 3; ++ib) {
+			Step &B = m_steps[ib];
+			if (B.dead || B.kind != Step::ED || B.width != A.width || B.height != A.height || B.ix[0].type != A.ix[0].type ||
+			    B.ox[0].type != A.ox[0].type || B.ox[0].bits != A.ox[0].bits)
+				continue;
+			bool clash = false;
+			for (size_t k = ia + 1; k < ib && !clash; ++k) {
+				const Step &M = m_steps[k];
+				if (M.dead)
+					continue;
+				for (int o = 0; o < 3; ++o)
+					if (same(M.out[o], B.in[0]) || same(M.out[o], B.out[0]) || same(M.in[o], B.out[0]))
+						clash = true;
+			}
+			if (clash)
+				continue;
+			const int p = A.ed_planes++;
+			A.in[p] = B.in[0];
+			A.out[p] = B.out[0];
+			A.ed_scale[p] = B.ed_scale[0];
+			A.ed_offset[p] = B.ed_offset[0];
+			A.label += "+" + B.label;
+			B.dead = true;
 		}
 	}
 }
@@ -598,15 +649,18 @@ void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void
 		}
 		case Step::ED: {
 			dev::EdArg a{};
-			a.src = resolve(s.in[0]);
-			a.dst = resolve(s.out[0]);
+			for (int p = 0; p < s.ed_planes; ++p) {
+				a.src[p] = resolve(s.in[p]);
+				a.dst[p] = resolve(s.out[p]);
+				a.scale[p] = s.ed_scale[p];
+				a.offset[p] = s.ed_offset[p];
+			}
+			a.planes = static_cast<unsigned>(s.ed_planes);
 			a.frames = frames;
 			a.width = s.width;
 			a.height = s.height;
 			a.in = s.ix[0];
 			a.out = s.ox[0];
-			a.scale = s.ed_scale;
-			a.offset = s.ed_offset;
 			rc = dev::launch_error_diffusion(a, stream);
 			break;
 		}

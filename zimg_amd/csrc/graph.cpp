@@ -470,14 +470,10 @@ class Planner {
 			base.depth_op = Node::SHIFT;
 			base.shift = fmt.depth - in.depth;
 		} else if (pixel_is_float(fmt.type)) {
-			if (fmt.type == PixelType::HALF)
-				fail(ZIMG_ERROR_UNSUPPORTED_OPERATION, "HALF output not supported");
 			base.depth_op = Node::TO_FLOAT;
 		} else {
 			base.depth_op = Node::DITHER;
 			base.dither = m_params.dither;
-			if (base.dither == Dither::RANDOM)
-				fail(ZIMG_ERROR_UNSUPPORTED_OPERATION, "random (blue-noise) dither not supported");
 		}
 		each(mask, [&](int r) {
 			Node n = base;

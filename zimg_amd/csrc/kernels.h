@@ -19,18 +19,19 @@ struct PlaneArg {
 
 // Depth conversion applied while loading a stage's input.
 struct InXform {
-	int type;        // 0 BYTE, 1 WORD, 3 FLOAT (source storage)
-	int op;          // 0 none, 1 left shift into WORD, 2 integer -> float (x * scale + offset)
+	int type;        // 0 BYTE, 1 WORD, 2 HALF, 3 FLOAT (source storage)
+	int op;          // 0 none, 1 left shift into WORD, 2 integer -> float (x * scale + offset),
+	                 // 3 exact conversion between floating-point storage types (no scale/offset)
 	unsigned shift;
 	float scale, offset;
 };
 
 // Depth conversion applied while storing a stage's output.
 struct OutXform {
-	int type;        // storage type of the output plane
+	int type;        // storage type of the output plane (HALF stores round to nearest even)
 	int op;          // 0 store as is, 1 quantise (x * scale + offset + dither, clamp, round to even)
-	int dither;      // 0 none, 1 ordered (Bayer 16x16)
-	int plane;       // Bayer table variant (plane index mod 4)
+	int dither;      // 0 none, 1 ordered (Bayer 16x16), 2 random (64x64 blue noise)
+	int plane;       // dither table variant (plane index mod 4)
 	unsigned bits;   // output depth
 	float scale, offset;
 };
@@ -83,12 +84,13 @@ struct CspArg {
 	CspProgram prog;
 };
 
+// Error diffusion over up to 4 planes of equal size in one launch (grid.y = plane).
 struct EdArg {
-	PlaneArg src, dst;
-	unsigned frames, width, height;
-	InXform in;        // load only (op must be 0); scale/offset of the dither below
+	PlaneArg src[4], dst[4];
+	unsigned frames, width, height, planes;
+	InXform in;        // load only (op must be 0)
 	OutXform out;      // op 1, dither ignored
-	float scale, offset;
+	float scale[4], offset[4];
 };
 
 // Launchers. Return hipSuccess (0) or the HIP error of the launch.

@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <type_traits>
 
+#include "bluenoise.h"
 #include "kernels.h"
 #include "zmath.h"
 
@@ -27,25 +28,54 @@ __device__ __forceinline__ char *frame_ptr_w(const PlaneArg &p, unsigned f) { re
 __device__ __forceinline__ const char *row_ptr(const char *base, long long stride, unsigned r) { return base + static_cast<long long>(r) * stride; }
 __device__ __forceinline__ char *row_ptr(char *base, long long stride, unsigned r) { return base + static_cast<long long>(r) * stride; }
 
+// half_to_float (quantize.cpp:24-77): exact widening; a NaN keeps its payload
+// and is made quiet.
+__device__ __forceinline__ float half_to_float(uint16_t h)
+{
+	if ((h & 0x7C00u) == 0x7C00u && (h & 0x03FFu))
+		return __uint_as_float((static_cast<uint32_t>(h & 0x8000u) << 16) | 0x7FC00000u | (static_cast<uint32_t>(h & 0x03FFu) << 13));
+	return static_cast<float>(__builtin_bit_cast(_Float16, h));
+}
+
+// float_to_half (quantize.cpp:79-144): IEEE binary16 round-to-nearest-even,
+// overflow to infinity; a NaN keeps the top of its payload and is made quiet.
+__device__ __forceinline__ uint16_t float_to_half(float x)
+{
+	const uint32_t u = __float_as_uint(x);
+	if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x007FFFFFu))
+		return static_cast<uint16_t>(((u >> 16) & 0x8000u) | 0x7E00u | ((u & 0x007FFFFFu) >> 13));
+	return __builtin_bit_cast(uint16_t, static_cast<_Float16>(x));
+}
+
+// Storage types: 0 BYTE, 1 WORD, 2 HALF, 3 FLOAT.
 __device__ __forceinline__ float load_as_float(const char *row, unsigned j, int type)
 {
 	if (type == 0) return static_cast<float>(reinterpret_cast<const uint8_t *>(row)[j]);
 	if (type == 1) return static_cast<float>(reinterpret_cast<const uint16_t *>(row)[j]);
+	if (type == 2) return half_to_float(reinterpret_cast<const uint16_t *>(row)[j]);
 	return reinterpret_cast<const float *>(row)[j];
 }
 
+// Raw storage bits (for copies and integer shifts).
 __device__ __forceinline__ unsigned load_as_uint(const char *row, unsigned j, int type)
 {
 	if (type == 0) return reinterpret_cast<const uint8_t *>(row)[j];
-	if (type == 1) return reinterpret_cast<const uint16_t *>(row)[j];
+	if (type == 1 || type == 2) return reinterpret_cast<const uint16_t *>(row)[j];
 	return reinterpret_cast<const uint32_t *>(row)[j];
 }
 
 __device__ __forceinline__ void store_uint(char *row, unsigned j, int type, unsigned v)
 {
 	if (type == 0) reinterpret_cast<uint8_t *>(row)[j] = static_cast<uint8_t>(v);
-	else if (type == 1) reinterpret_cast<uint16_t *>(row)[j] = static_cast<uint16_t>(v);
+	else if (type == 1 || type == 2) reinterpret_cast<uint16_t *>(row)[j] = static_cast<uint16_t>(v);
 	else reinterpret_cast<uint32_t *>(row)[j] = v;
+}
+
+// An unquantised value into a floating-point plane (HALF or FLOAT).
+__device__ __forceinline__ void store_float(char *row, unsigned j, int type, float v)
+{
+	if (type == 2) reinterpret_cast<uint16_t *>(row)[j] = float_to_half(v);
+	else reinterpret_cast<float *>(row)[j] = v;
 }
 
 // Bayer 16x16 ordered-dither offset for output pixel (i, j) of plane variant p
@@ -70,11 +100,24 @@ __device__ __forceinline__ float bayer_offset(int plane, unsigned i, unsigned j)
 	return static_cast<float>(static_cast<int>(v) + 1) / static_cast<float>(255u + 2u) - 0.5f;
 }
 
+__constant__ uint8_t c_blue_noise[ZAMD_BLUE_NOISE_LEN * ZAMD_BLUE_NOISE_LEN] = { ZAMD_BLUE_NOISE_BYTES };
+
+// Blue-noise offset for output pixel (i, j) of plane variant p (dither.cpp:219-232):
+// each variant starts the 64x64 map at its own (column, row) offset.
+__device__ __forceinline__ float blue_noise_offset(int plane, unsigned i, unsigned j)
+{
+	const unsigned variant = static_cast<unsigned>(plane) & 3;
+	const unsigned hoff = variant == 0 ? 0 : variant == 1 ? 32 : variant == 2 ? 16 : 48;
+	const unsigned voff = variant == 0 ? 0 : variant == 1 ? 12 : variant == 2 ? 55 : 26;
+	const unsigned t = c_blue_noise[((i + voff) % ZAMD_BLUE_NOISE_LEN) * ZAMD_BLUE_NOISE_LEN + ((hoff + j) & (ZAMD_BLUE_NOISE_LEN - 1))];
+	return static_cast<float>(static_cast<int>(t) + 1) / static_cast<float>(255u + 2u) - 0.5f;
+}
+
 // dither_ordered (dither.cpp:37-55) for one value already converted to float.
 __device__ __forceinline__ unsigned quantize(float x, const OutXform &o, unsigned i, unsigned j)
 {
 	x = x * o.scale + o.offset;
-	const float d = o.dither ? bayer_offset(o.plane, i, j) : 0.0f;
+	const float d = o.dither == 1 ? bayer_offset(o.plane, i, j) : o.dither == 2 ? blue_noise_offset(o.plane, i, j) : 0.0f;
 	x += d;
 	const float maxval = static_cast<float>(1UL << o.bits) - 1;
 	x = x < 0.0f ? 0.0f : (maxval < x ? maxval : x);
@@ -115,7 +158,7 @@ __device__ __forceinline__ void store_out(char *row, unsigned j, lds_t<WORD> v, 
 	} else if constexpr (WORD) {
 		reinterpret_cast<uint16_t *>(row)[j] = v;
 	} else {
-		reinterpret_cast<float *>(row)[j] = v;
+		store_float(row, j, o.type, v);
 	}
 }
 
@@ -381,6 +424,11 @@ __device__ __forceinline__ void store_pair(char *row, unsigned j, lds_t<WORD> v0
 			*reinterpret_cast<uint32_t *>(row + 2 * static_cast<size_t>(j)) = static_cast<uint32_t>(v0) | (static_cast<uint32_t>(v1) << 16);
 		else
 			reinterpret_cast<uint16_t *>(row)[j] = v0;
+	} else if (o.type == 2) {
+		if (two)
+			*reinterpret_cast<uint32_t *>(row + 2 * static_cast<size_t>(j)) = static_cast<uint32_t>(float_to_half(v0)) | (static_cast<uint32_t>(float_to_half(v1)) << 16);
+		else
+			reinterpret_cast<uint16_t *>(row)[j] = float_to_half(v0);
 	} else {
 		if (two)
 			*reinterpret_cast<float2 *>(row + 4 * static_cast<size_t>(j)) = make_float2(v0, v1);
@@ -649,7 +697,7 @@ __global__ void __launch_bounds__(kThreads) k_point(PointArg a)
 	if (a.out.op == 1)
 		store_uint(drow, j, a.out.type, quantize(x, a.out, i, j));
 	else
-		reinterpret_cast<float *>(drow)[j] = x;
+		store_float(drow, j, a.out.type, x);
 }
 
 // ColorspaceConversionImpl::process (colorspace.cpp:45-77): up to six in-place
@@ -739,7 +787,7 @@ __global__ void __launch_bounds__(kThreads) k_colorspace(CspArg a)
 		if (a.out[p].op == 1)
 			store_uint(drow, j, a.out[p].type, quantize(v[p], a.out[p], i, j));
 		else
-			reinterpret_cast<float *>(drow)[j] = v[p];
+			store_float(drow, j, a.out[p].type, v[p]);
 	}
 }
 
@@ -757,19 +805,29 @@ constexpr int kEdChunk = 16;
 constexpr int kEdDelay = 2 * 63 + 2 + kEdChunk;
 constexpr int kEdRing = 256;
 
+__device__ __forceinline__ float ed_load(const char *srow, int j, int W, int type, bool row_ok)
+{
+	return (row_ok && j >= 0 && j < W) ? load_as_float(srow, static_cast<unsigned>(j), type) : 0.0f;
+}
+
 __global__ void __launch_bounds__(kEdWaves * 64) k_error_diffusion(EdArg a)
 {
 	extern __shared__ __align__(16) float ed_smem[];
 	float *ring = ed_smem;                      // [kEdWaves][kEdRing]
 	float *prev_row = ring + kEdWaves * kEdRing; // [width + 2], padded row of the previous band's last row
 
+	// blockIdx.x: frame, blockIdx.y: plane of this launch.
 	const unsigned f = blockIdx.x;
+	const unsigned pl = blockIdx.y;
+	const PlaneArg &sp = a.src[pl];
+	const PlaneArg &dp = a.dst[pl];
 	const int lane = threadIdx.x & 63;
 	const int w = threadIdx.x >> 6;
 	const int W = static_cast<int>(a.width);
-	const char *src = frame_ptr(a.src, f);
-	char *dst = frame_ptr_w(a.dst, f);
+	const char *src = frame_ptr(sp, f);
+	char *dst = frame_ptr_w(dp, f);
 	const float maxval = static_cast<float>(1UL << a.out.bits) - 1;
+	const float scale = a.scale[pl], offset = a.offset[pl];
 
 	for (int t = threadIdx.x; t < W + 2; t += kEdWaves * 64)
 		prev_row[t] = 0.0f;
@@ -782,11 +840,26 @@ __global__ void __launch_bounds__(kEdWaves * 64) k_error_diffusion(EdArg a)
 	for (int base = 0; base < static_cast<int>(a.height); base += band_rows) {
 		const int r = base + w * 64 + lane;
 		const bool row_ok = r < static_cast<int>(a.height);
-		const char *srow = row_ptr(src, a.src.stride, row_ok ? r : 0);
-		char *drow = row_ptr(dst, a.dst.stride, row_ok ? r : 0);
+		const char *srow = row_ptr(src, sp.stride, row_ok ? r : 0);
+		char *drow = row_ptr(dst, dp.stride, row_ok ? r : 0);
 		float e_left = 0.0f, t_m1 = 0.0f, t_0 = 0.0f, t_p1 = 0.0f, prod = 0.0f;
 
+		// Inputs of a chunk are fetched one chunk ahead, so the HBM latency of
+		// this lane's row is off the step-to-step critical path.
+		const int jbase = -2 * lane - w * kEdDelay;
+		float xin[kEdChunk], xnext[kEdChunk];
+#pragma unroll
+		for (int s = 0; s < kEdChunk; ++s)
+			xnext[s] = ed_load(srow, jbase + s, W, a.in.type, row_ok);
+
 		for (int ch = 0; ch < nchunks; ++ch) {
+#pragma unroll
+			for (int s = 0; s < kEdChunk; ++s)
+				xin[s] = xnext[s];
+#pragma unroll
+			for (int s = 0; s < kEdChunk; ++s)
+				xnext[s] = ed_load(srow, jbase + (ch + 1) * kEdChunk + s, W, a.in.type, row_ok);
+#pragma unroll
 			for (int s = 0; s < kEdChunk; ++s) {
 				const int T = ch * kEdChunk + s;
 				const int j = T - 2 * lane - w * kEdDelay;
@@ -808,7 +881,7 @@ __global__ void __launch_bounds__(kEdWaves * 64) k_error_diffusion(EdArg a)
 				t_p1 = recv;
 				prod = 0.0f;
 				if (row_ok && j >= 0 && j < W) {
-					float x = load_as_float(srow, static_cast<unsigned>(j), a.in.type) * a.scale + a.offset;
+					float x = xin[s] * scale + offset;
 					float err = 0;
 					err += e_left * (7.0f / 16.0f);
 					err += t_p1 * (3.0f / 16.0f);
@@ -886,7 +959,7 @@ int launch_colorspace(const CspArg &a, void *stream)
 int launch_error_diffusion(const EdArg &a, void *stream)
 {
 	const size_t lds = sizeof(float) * (static_cast<size_t>(kEdWaves) * kEdRing + a.width + 2);
-	hipLaunchKernelGGL(k_error_diffusion, dim3(a.frames), dim3(kEdWaves * 64), lds, static_cast<hipStream_t>(stream), a);
+	hipLaunchKernelGGL(k_error_diffusion, dim3(a.frames, a.planes), dim3(kEdWaves * 64), lds, static_cast<hipStream_t>(stream), a);
 	return static_cast<int>(hipGetLastError());
 }
 
