@@ -192,3 +192,24 @@ def test_single_plane_lanczos_is_one_fused_launch():
     dev = [l for l in g.describe().split("device:\n")[1].splitlines() if l]
     assert len(dev) == 1 and "passes=2 h_first" in dev[0]
     assert g.device_tmp_size(8) == 0
+
+
+def test_tiled_resize_configurations_in_bounds():
+    """Graph build re-derives every LDS and tap-window index of the tiled
+    resize kernel on the host and fails on any that would leave its staged
+    window; random sizes, filters, types and active regions all build."""
+    import random
+    rng = random.Random(7)
+    for _ in range(400):
+        sw, sh = rng.randint(1, 2500), rng.randint(1, 1500)
+        dw, dh = rng.randint(1, 4000), rng.randint(1, 2200)
+        t = rng.choice([z.PIXEL_WORD, z.PIXEL_FLOAT, z.PIXEL_BYTE])
+        depth = {z.PIXEL_BYTE: 8, z.PIXEL_WORD: 16, z.PIXEL_FLOAT: 32}[t]
+        filt = rng.choice(range(7))
+        s = z.image_format(width=sw, height=sh, pixel_type=t, depth=depth)
+        d = z.image_format(width=dw, height=dh, pixel_type=t, depth=depth)
+        if rng.random() < 0.3:
+            s.active_region.left, s.active_region.top = rng.uniform(-3, 3), rng.uniform(-3, 3)
+            s.active_region.width, s.active_region.height = sw * rng.uniform(0.5, 1.0), sh * rng.uniform(0.5, 1.0)
+        p = z.builder_params(resample_filter=filt, filter_param_a=rng.choice([2.0, 3.0, 4.0]) if filt == z.RESIZE_LANCZOS else float("nan"))
+        z.FilterGraph(s, d, p).describe()

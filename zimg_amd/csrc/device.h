@@ -45,7 +45,7 @@ public:
 		unsigned tile_w = 0, tile_h = 0, lds_in_rows = 0, lds_in_cols = 0, lds_mid_rows = 0, lds_mid_cols = 0;
 		// fast kernel configuration (fast_fwh == 0: not eligible)
 		int fast_fwh = 0;
-		unsigned fast_tile_h = 0, fast_in_rows = 0, fast_in_cols = 0, fast_mid_rows = 0, fast_mid_cols = 0;
+		unsigned fast_tile_w = 0, fast_tile_h = 0, fast_in_rows = 0, fast_in_cols = 0, fast_mid_rows = 0, fast_mid_cols = 0;
 		// POINT
 		unsigned crop_left = 0, crop_top = 0;
 		int fill = 0;
@@ -88,6 +88,7 @@ private:
 	void lower();
 	void fuse();
 	void configure_resize(Step &s);
+	void check_tiled(const Step &s, const FilterTable *th_tab, const FilterTable *tv_tab) const;
 	const Tables &tables_for_current_device() const;
 };
 

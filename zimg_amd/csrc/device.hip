@@ -350,6 +350,66 @@ void DeviceProgram::fuse()
 	}
 }
 
+namespace {
+
+// Even-aligned start of a WORD pass row and its pair count (the tiled kernel's table form).
+unsigned pair_left(const FilterTable &t, unsigned o) { return t.left[o] & ~1u; }
+unsigned pair_count(const FilterTable &t) { return t.filter_width / 2 + 1; }
+
+} // namespace
+
+// Every LDS and table index the tiled kernel forms for this configuration,
+// recomputed on the host with the kernel's formulas: the staged windows fit
+// the LDS arrays and every tap window lies inside its staged window.
+void DeviceProgram::check_tiled(const Step &s, const FilterTable *th_tab, const FilterTable *tv_tab) const
+{
+	auto bad = [](const char *what) { fail(ZIMG_ERROR_UNKNOWN, std::string("tiled resize configuration: ") + what); };
+	const unsigned tw = s.fast_tile_w, th = s.fast_tile_h;
+	if (s.h_first && tw != dev::kFastTileW)
+		bad("tile width");
+	if (!s.h_first && !th_tab && tw % 16)
+		bad("vertical-only tile width");
+	const bool w = s.word;
+	for (unsigned j0 = 0; j0 < s.width; j0 += tw) {
+		const unsigned nc_out = std::min(tw, s.width - j0), jl = j0 + nc_out - 1;
+		unsigned cb = j0, nc = (nc_out + 15) & ~15u;
+		if (th_tab) {
+			cb = (w ? pair_left(*th_tab, j0) : th_tab->left[j0]) & ~15u;
+			nc = ((w ? pair_left(*th_tab, jl) + 2 * pair_count(*th_tab) : th_tab->left[jl] + th_tab->filter_width) - cb + 15) & ~15u;
+			for (unsigned j = j0; j <= jl; ++j) {
+				const unsigned lo = w ? pair_left(*th_tab, j) : th_tab->left[j];
+				const unsigned width = w ? 2 * pair_count(*th_tab) : th_tab->filter_width;
+				if (lo < cb || lo - cb + width > nc || (w && (lo - cb) % 2))
+					bad("horizontal window");
+			}
+			if (static_cast<int>(w ? pair_count(*th_tab) : th_tab->filter_width) > s.fast_fwh)
+				bad("horizontal register capacity");
+		}
+		if (nc > s.fast_in_cols || (th_tab && !s.h_first && nc > s.fast_mid_cols) || cb % 16 || nc % 16)
+			bad("staged columns");
+	}
+	for (unsigned i0 = 0; i0 < s.height; i0 += th) {
+		const unsigned nr = std::min(th, s.height - i0), il = i0 + nr - 1;
+		unsigned r0 = i0, span = nr;
+		if (tv_tab) {
+			const bool pairs = w && s.h_first;
+			r0 = pairs ? pair_left(*tv_tab, i0) : tv_tab->left[i0];
+			span = (pairs ? pair_left(*tv_tab, il) + 2 * pair_count(*tv_tab) : tv_tab->left[il] + tv_tab->filter_width) - r0;
+			for (unsigned i = i0; i <= il; ++i) {
+				const unsigned top = pairs ? pair_left(*tv_tab, i) : tv_tab->left[i];
+				if (top < r0 || top - r0 + (pairs ? 2 * pair_count(*tv_tab) : tv_tab->filter_width) > span || (pairs && (top - r0) % 2))
+					bad("vertical window");
+			}
+			if (pairs && (span % 2 || r0 % 2))
+				bad("row pairs");
+		}
+		if (span > s.fast_in_rows || (s.h_first && tv_tab && span > s.fast_mid_rows) || (!s.h_first && th_tab && nr > s.fast_mid_rows))
+			bad("staged rows");
+	}
+	if (s.h_first && tv_tab && s.fast_mid_cols < dev::kFastTileW)
+		bad("intermediate width");
+}
+
 void DeviceProgram::configure_resize(Step &s)
 {
 	const FilterTable &t1 = *m_plan.nodes[s.tab[0]].table;
@@ -357,13 +417,13 @@ void DeviceProgram::configure_resize(Step &s)
 	const size_t elem = s.word ? 2 : 4;
 	const bool sorted1 = t1.sorted();
 
-	// Fast kernel: sorted tables, horizontal taps within a register bucket.
+	// Tiled kernel: sorted tables, horizontal taps within a register bucket.
 	const FilterTable *th_tab = s.h_first ? &t1 : t2;
 	const FilterTable *tv_tab = s.h_first ? t2 : &t1;
 	const bool sorted_all = sorted1 && (!t2 || t2->sorted());
 	int fwh = 0;
 	if (sorted_all) {
-		const unsigned need = th_tab ? th_tab->filter_width : 1;
+		const unsigned need = th_tab ? (s.word ? pair_count(*th_tab) : th_tab->filter_width) : 1;
 		for (int b : dev::kFastTapBuckets) {
 			if (need <= static_cast<unsigned>(b)) {
 				fwh = b;
@@ -372,44 +432,77 @@ void DeviceProgram::configure_resize(Step &s)
 		}
 	}
 	if (fwh) {
-		const unsigned tw = dev::kFastTileW;
-		auto even = [](unsigned x) { return (x + 1) & ~1u; };
-		for (unsigned th : { 32u, 16u, 8u, 4u, 2u, 1u }) {
-			unsigned in_rows = 0, in_cols = 0, mid_rows = 0, mid_cols = 0;
-			unsigned hcols = 0;
-			if (th_tab) {
+		// Horizontal window [cb, cb + nc) of output columns [j0, j0 + nc_out) exactly as the kernel computes it.
+		auto h_cols = [&](const FilterTable &t, unsigned j0, unsigned ncols) {
+			const unsigned jl = j0 + ncols - 1;
+			const unsigned cb = (s.word ? pair_left(t, j0) : t.left[j0]) & ~15u;
+			const unsigned ce = s.word ? pair_left(t, jl) + 2 * pair_count(t) : t.left[jl] + t.filter_width;
+			return (ce - cb + 15) & ~15u;
+		};
+		auto ceil_div = [](size_t a, size_t b) { return (a + b - 1) / b; };
+		double best = 0;
+		std::vector<unsigned> widths = s.h_first ? std::vector<unsigned>{ dev::kFastTileW } : std::vector<unsigned>{ 512, 256, 128, 64, 32, 16 };
+		for (unsigned tw : widths) {
+			for (unsigned th : { 64u, 32u, 16u, 8u, 4u, 2u, 1u }) {
+				unsigned in_rows = 0, in_cols = 0, mid_rows = 0, mid_cols = 0;
 				for (unsigned j0 = 0; j0 < s.width; j0 += tw) {
 					const unsigned nc = std::min(tw, s.width - j0);
-					hcols = std::max(hcols, even(th_tab->left[j0 + nc - 1] + th_tab->filter_width - (th_tab->left[j0] & ~1u)));
+					in_cols = std::max(in_cols, th_tab ? h_cols(*th_tab, j0, nc) : (nc + 15) & ~15u);
 				}
-			}
-			if (s.h_first) {
-				unsigned span = 0;
 				for (unsigned i0 = 0; i0 < s.height; i0 += th) {
 					const unsigned nr = std::min(th, s.height - i0);
-					span = std::max(span, tv_tab ? tv_tab->left[i0 + nr - 1] + tv_tab->filter_width - tv_tab->left[i0] : nr);
+					const unsigned il = i0 + nr - 1;
+					unsigned span;
+					if (!tv_tab)
+						span = nr;
+					else if (s.h_first && s.word)
+						span = pair_left(*tv_tab, il) + 2 * pair_count(*tv_tab) - pair_left(*tv_tab, i0);
+					else
+						span = tv_tab->left[il] + tv_tab->filter_width - tv_tab->left[i0];
+					in_rows = std::max(in_rows, span);
 				}
-				in_rows = span;
-				in_cols = hcols;
 				if (t2) {
-					mid_rows = span;
-					mid_cols = tw;
+					mid_rows = s.h_first ? in_rows : th;
+					mid_cols = s.h_first ? tw : in_cols;
 				}
-			} else if (t2) {
-				mid_rows = th;
-				mid_cols = hcols;
-			}
-			const size_t bytes = elem * (static_cast<size_t>(in_rows) * in_cols + static_cast<size_t>(mid_rows) * mid_cols);
-			if (bytes <= kLdsBudget) {
-				s.fast_fwh = fwh;
-				s.fast_tile_h = th;
-				s.fast_in_rows = in_rows;
-				s.fast_in_cols = in_cols;
-				s.fast_mid_rows = mid_rows;
-				s.fast_mid_cols = mid_cols;
-				break;
+				const size_t bytes = align_up(elem * in_rows * in_cols, 16) + elem * mid_rows * mid_cols;
+				if (bytes > kLdsBudget)
+					continue;
+				// Cost model: per-block work in thread iterations (staging, first and
+				// second pass, weighted by taps), scaled by how many blocks share a CU.
+				const size_t blocks = ceil_div(s.width, tw) * ceil_div(s.height, th);
+				const double stage = static_cast<double>(ceil_div(static_cast<size_t>(in_rows) * in_cols * elem / 16, 256)) * 6.0 + 40.0;
+				const double fwv = tv_tab ? (s.word && s.h_first ? pair_count(*tv_tab) : tv_tab->filter_width) : 0;
+				const double fwhh = th_tab ? fwh : 0;
+				double work;
+				if (s.h_first) {
+					work = in_rows * (fwhh + 4) + (t2 ? th * (fwv + 4) : 0);
+				} else {
+					const size_t pairs = in_cols / 2;
+					const size_t groups = pairs >= 256 ? 1 : 256 / pairs;
+					work = static_cast<double>(ceil_div(pairs, 256) * ceil_div(th, groups)) * (fwv + 4);
+					if (t2) {
+						const size_t opairs = (tw + 1) / 2;
+						const size_t og = opairs >= 256 ? 1 : 256 / opairs;
+						work += static_cast<double>(ceil_div(opairs, 256) * ceil_div(th, og)) * (fwhh + 4);
+					}
+				}
+				const size_t per_cu = std::min<size_t>(4, (160 * 1024) / std::max<size_t>(bytes, 1));
+				const double cost = static_cast<double>(blocks) * (stage + work) / static_cast<double>(std::max<size_t>(per_cu, 1));
+				if (best == 0 || cost < best) {
+					best = cost;
+					s.fast_fwh = fwh;
+					s.fast_tile_w = tw;
+					s.fast_tile_h = th;
+					s.fast_in_rows = in_rows;
+					s.fast_in_cols = in_cols;
+					s.fast_mid_rows = mid_rows;
+					s.fast_mid_cols = mid_cols;
+				}
 			}
 		}
+		if (s.fast_fwh)
+			check_tiled(s, th_tab, tv_tab);
 	}
 
 	static const unsigned kWidths[] = { 256, 128, 64, 32, 16, 8, 4, 2, 1 };
@@ -498,7 +591,8 @@ const DeviceProgram::Tables &DeviceProgram::tables_for_current_device() const
 	for (int id : ids) {
 		const FilterTable &t = *m_plan.nodes[id].table;
 		offs.push_back(bytes);
-		bytes += align_up(t.left.size() * 4, 256) + align_up(t.data_i16.size() * 2, 256) + align_up(t.data.size() * 4, 256);
+		bytes += align_up(t.left.size() * 4, 256) + align_up(t.data_i16.size() * 2, 256) + align_up(t.data.size() * 4, 256) +
+		         align_up(t.left.size() * 4, 256) + align_up(static_cast<size_t>(t.filter_rows) * pair_count(t) * 4, 256);
 	}
 	auto tab = std::make_shared<Tables>();
 	if (bytes) {
@@ -511,6 +605,22 @@ const DeviceProgram::Tables &DeviceProgram::tables_for_current_device() const
 			std::memcpy(&host[o_i16], t.data_i16.data(), t.data_i16.size() * 2);
 			const size_t o_f32 = o_i16 + align_up(t.data_i16.size() * 2, 256);
 			std::memcpy(&host[o_f32], t.data.data(), t.data.size() * 4);
+			// Pair form: window from left & ~1, a zero tap first when left is odd.
+			const size_t o_pl = o_f32 + align_up(t.data.size() * 4, 256);
+			const size_t o_pc = o_pl + align_up(t.left.size() * 4, 256);
+			const unsigned fwp = pair_count(t);
+			for (unsigned r = 0; r < t.filter_rows; ++r) {
+				const unsigned pl = pair_left(t, r);
+				const unsigned d = t.left[r] - pl;
+				std::memcpy(&host[o_pl + 4 * r], &pl, 4);
+				auto tap = [&](int k) -> uint16_t {
+					return (k >= 0 && k < static_cast<int>(t.filter_width)) ? static_cast<uint16_t>(t.data_i16[static_cast<size_t>(r) * t.filter_width + k]) : 0;
+				};
+				for (unsigned p = 0; p < fwp; ++p) {
+					const uint32_t w = tap(static_cast<int>(2 * p) - static_cast<int>(d)) | (static_cast<uint32_t>(tap(static_cast<int>(2 * p + 1) - static_cast<int>(d))) << 16);
+					std::memcpy(&host[o_pc + 4 * (static_cast<size_t>(r) * fwp + p)], &w, 4);
+				}
+			}
 		}
 		check_hip(hipMalloc(&tab->mem, bytes), "hipMalloc(tables)");
 		check_hip(hipMemcpy(tab->mem, host.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(tables)");
@@ -520,9 +630,12 @@ const DeviceProgram::Tables &DeviceProgram::tables_for_current_device() const
 			const size_t o = offs[n];
 			const size_t o_i16 = o + align_up(t.left.size() * 4, 256);
 			const size_t o_f32 = o_i16 + align_up(t.data_i16.size() * 2, 256);
+			const size_t o_pl = o_f32 + align_up(t.data.size() * 4, 256);
+			const size_t o_pc = o_pl + align_up(t.left.size() * 4, 256);
 			tab->pass[ids[n]] = dev::PassArg{ reinterpret_cast<const unsigned *>(base + o), reinterpret_cast<const int16_t *>(base + o_i16),
 			                                  reinterpret_cast<const float *>(base + o_f32), static_cast<int>(t.filter_width),
-			                                  static_cast<int>(t.filter_rows) };
+			                                  static_cast<int>(t.filter_rows), reinterpret_cast<const unsigned *>(base + o_pl),
+			                                  reinterpret_cast<const int32_t *>(base + o_pc), static_cast<int>(pair_count(t)) };
 		}
 	}
 	m_tables[device] = tab;
@@ -597,7 +710,7 @@ void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void
 			};
 			if (s.fast_fwh && aligned(a.src) && aligned(a.dst)) {
 				a.fwh = s.fast_fwh;
-				a.tile_w = dev::kFastTileW;
+				a.tile_w = s.fast_tile_w;
 				a.tile_h = s.fast_tile_h;
 				a.lds_in_rows = s.fast_in_rows;
 				a.lds_in_cols = s.fast_in_cols;
@@ -677,10 +790,10 @@ std::string DeviceProgram::describe() const
 		char buf[256];
 		static const char *kinds[] = { "resize", "point", "colorspace", "error_diffusion" };
 		if (s.kind == Step::RESIZE)
-			std::snprintf(buf, sizeof(buf), "%s %ux%u->%ux%u passes=%d %s tile=%ux%u lds=%zu fast=%d(tile_h=%u lds=%zu): %s\n", kinds[s.kind],
+			std::snprintf(buf, sizeof(buf), "%s %ux%u->%ux%u passes=%d %s tile=%ux%u lds=%zu tiled=%d(%ux%u lds=%zu): %s\n", kinds[s.kind],
 			              s.in_w, s.in_h, s.width, s.height, s.passes, s.h_first ? "h_first" : "v_first", s.tile_w, s.tile_h,
 			              (s.word ? 2 : 4) * (static_cast<size_t>(s.lds_in_rows) * s.lds_in_cols + static_cast<size_t>(s.lds_mid_rows) * s.lds_mid_cols),
-			              s.fast_fwh, s.fast_tile_h,
+			              s.fast_fwh, s.fast_tile_w, s.fast_tile_h,
 			              (s.word ? 2 : 4) * (static_cast<size_t>(s.fast_in_rows) * s.fast_in_cols + static_cast<size_t>(s.fast_mid_rows) * s.fast_mid_cols),
 			              s.label.c_str());
 		else

@@ -36,13 +36,19 @@ struct OutXform {
 	float scale, offset;
 };
 
-// One resize pass on the device: packed coefficient rows (stride fw) and left offsets.
+// One resize pass on the device: packed coefficient rows (stride fw) and left
+// offsets; for WORD also the same taps as int16 pairs over a window that
+// starts on an even sample (pl = left & ~1, a zero tap in front when left is
+// odd, zeros behind), fwp = fw / 2 + 1 pairs per row.
 struct PassArg {
 	const unsigned *left;
 	const int16_t *ci;
 	const float *cf;
 	int fw;
 	int rows;
+	const unsigned *pl;
+	const int32_t *pc;
+	int fwp;
 };
 
 struct ResizeArg {
@@ -59,11 +65,13 @@ struct ResizeArg {
 	unsigned tile_w, tile_h;
 	unsigned lds_in_rows, lds_in_cols;   // staged input tile (H-first)
 	unsigned lds_mid_rows, lds_mid_cols; // intermediate tile
-	int fwh;            // 0: generic kernel; else the fast kernel's horizontal tap capacity
+	int fwh;            // 0: generic kernel; else the tiled kernel's horizontal register capacity
+	                    // (WORD: tap pairs, FLOAT: taps)
 };
 
-// Horizontal tap capacities of the fast resize kernel instantiations.
-constexpr int kFastTapBuckets[] = { 4, 8, 12, 16, 24, 32 };
+// Horizontal register capacities of the tiled resize kernel instantiations.
+constexpr int kFastTapBuckets[] = { 2, 3, 4, 5, 6, 8, 12, 16, 24, 32 };
+// Output columns of a horizontal-first tile (two per thread).
 constexpr unsigned kFastTileW = 512;
 
 struct PointArg {

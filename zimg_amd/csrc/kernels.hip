@@ -326,83 +326,6 @@ __global__ void __launch_bounds__(kThreads) k_resize(ResizeArg a)
 	}
 }
 
-// ---------------------------------------------------------------------------
-// Fast fused resize. Same arithmetic as k_resize, organised for throughput:
-// each thread owns two adjacent output columns for the whole tile, so the
-// horizontal taps of both columns live in registers and are reused on every
-// row; vertical taps are uniform per output row (scalar loads); LDS and HBM
-// are touched in column pairs (4 or 8 bytes per lane). Loops carry no integer
-// division. Requires sorted left offsets, <= FWH horizontal taps and planes
-// whose base pointers and row strides are multiples of 16 bytes.
-// ---------------------------------------------------------------------------
-
-template <bool WORD>
-struct Pair;
-template <>
-struct Pair<true> {
-	uint16_t a, b;
-};
-template <>
-struct Pair<false> {
-	float a, b;
-};
-
-template <bool WORD>
-__device__ __forceinline__ Pair<WORD> load_pair(const char *row, unsigned c, unsigned limit, const InXform &in)
-{
-	// Columns c (even) and c + 1; the second is read only when c + 1 < limit.
-	Pair<WORD> p;
-	const bool two = c + 1 < limit;
-	if constexpr (WORD) {
-		unsigned x0, x1 = 0;
-		if (in.type == 0) {
-			if (two) {
-				const uint16_t v = *reinterpret_cast<const uint16_t *>(row + c);
-				x0 = v & 0xFF;
-				x1 = v >> 8;
-			} else {
-				x0 = reinterpret_cast<const uint8_t *>(row)[c];
-			}
-		} else {
-			if (two) {
-				const uint32_t v = *reinterpret_cast<const uint32_t *>(row + 2 * static_cast<size_t>(c));
-				x0 = v & 0xFFFF;
-				x1 = v >> 16;
-			} else {
-				x0 = reinterpret_cast<const uint16_t *>(row)[c];
-			}
-		}
-		if (in.op == 1) {
-			x0 <<= in.shift;
-			x1 <<= in.shift;
-		}
-		p.a = static_cast<uint16_t>(x0);
-		p.b = static_cast<uint16_t>(x1);
-	} else {
-		float x0, x1 = 0.0f;
-		if (in.type == 3) {
-			if (two) {
-				const float2 v = *reinterpret_cast<const float2 *>(row + 4 * static_cast<size_t>(c));
-				x0 = v.x;
-				x1 = v.y;
-			} else {
-				x0 = reinterpret_cast<const float *>(row)[c];
-			}
-		} else {
-			x0 = load_as_float(row, c, in.type);
-			if (two)
-				x1 = load_as_float(row, c + 1, in.type);
-		}
-		if (in.op == 2) {
-			x0 = x0 * in.scale + in.offset;
-			x1 = x1 * in.scale + in.offset;
-		}
-		p.a = x0;
-		p.b = x1;
-	}
-	return p;
-}
-
 template <bool WORD>
 __device__ __forceinline__ void store_pair(char *row, unsigned j, lds_t<WORD> v0, lds_t<WORD> v1, bool two, const OutXform &o, unsigned i)
 {
@@ -437,95 +360,203 @@ __device__ __forceinline__ void store_pair(char *row, unsigned j, lds_t<WORD> v0
 	}
 }
 
-// Per-thread horizontal taps of two columns held in registers.
-template <bool WORD, int FWH>
-struct HTaps {
+// ---------------------------------------------------------------------------
+// Tiled resize. Same arithmetic as k_resize, organised for throughput:
+//  * the source rectangle a tile needs is staged in LDS with 16-byte loads,
+//    several in flight per thread, with the load transform applied once;
+//  * WORD samples are kept as x - 32768 in int16 and the Q1.14 taps are
+//    applied in pairs with v_dot2_i32_i16 (exact: the int32 sum of the
+//    products is the reference's accumulator, resize_impl.cpp:44-95), using
+//    tables whose rows start on an even sample and carry a zero tap where
+//    the real window starts on an odd one;
+//  * horizontal taps of a thread's two columns live in registers for the
+//    whole tile; vertical taps are uniform per row (scalar loads);
+//  * between two passes the intermediate stays in LDS; for WORD the
+//    horizontal-first intermediate interleaves row pairs so that the
+//    vertical pass also runs on dot2.
+// Requires sorted left offsets and planes whose base pointers and row
+// strides are multiples of 16 bytes.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int32_t dot2(uint32_t x, int32_t c, int32_t acc)
+{
+	typedef short v2i16 __attribute__((ext_vector_type(2)));
+	return __builtin_amdgcn_sdot2(__builtin_bit_cast(v2i16, x), __builtin_bit_cast(v2i16, c), acc, false);
+}
+
+// pack_word, returned as the biased int16 bits the next pass consumes.
+__device__ __forceinline__ uint32_t pack_biased(int32_t acc, int32_t pixel_max)
+{
+	return static_cast<uint32_t>(pack_word(acc, pixel_max)) ^ 0x8000u;
+}
+
+// Shift every 16-bit half of w left by s (s < 16), then bias to int16.
+__device__ __forceinline__ uint32_t shift_bias2(uint32_t w, unsigned s)
+{
+	const uint32_t m = (0xFFFFu << s) & 0xFFFFu;
+	return ((w << s) & (m | (m << 16))) ^ 0x80008000u;
+}
+
+// Stage source rows [r0, r0 + nr) x columns [c0, c0 + nc) into LDS with row
+// pitch `pin` samples. c0 and nc are multiples of 16, pin of 16. Rows at or
+// past `rows_valid` and columns at or past `src_w` are stored as zero
+// samples; every tap that could reach them has a zero coefficient.
+template <bool WORD>
+__device__ __forceinline__ void stage_tile(lds_t<WORD> *xin, unsigned pin, const char *src, long long stride, unsigned r0,
+                                           unsigned nr, unsigned rows_valid, unsigned c0, unsigned nc, unsigned src_w,
+                                           const InXform &in)
+{
+	const unsigned esz = in.type == 0 ? 1 : in.type == 3 ? 4 : 2;
+	const unsigned ce = 16 / esz; // samples per 16-byte chunk
+	const unsigned cpr = nc / ce; // chunks per row
+	const unsigned total = nr * cpr;
+	constexpr int U = 4;
+	for (unsigned base = 0; base < total; base += U * kThreads) {
+		uint4 raw[U];
+		unsigned rr[U], cc[U];
+		bool full[U];
+#pragma unroll
+		for (int u = 0; u < U; ++u) {
+			const unsigned e = base + u * kThreads + threadIdx.x;
+			rr[u] = e / cpr;
+			cc[u] = c0 + (e - rr[u] * cpr) * ce;
+			full[u] = e < total && r0 + rr[u] < rows_valid && cc[u] + ce <= src_w;
+			raw[u] = full[u] ? *reinterpret_cast<const uint4 *>(row_ptr(src, stride, r0 + rr[u]) + static_cast<size_t>(cc[u]) * esz)
+			                 : make_uint4(0, 0, 0, 0);
+		}
+#pragma unroll
+		for (int u = 0; u < U; ++u) {
+			const unsigned e = base + u * kThreads + threadIdx.x;
+			if (e >= total)
+				continue;
+			const unsigned r = r0 + rr[u];
+			if (!full[u] && r < rows_valid) {
+				// Chunk straddling the right edge: per-sample loads inside the row.
+				const char *srow = row_ptr(src, stride, r);
+				uint32_t w[4] = { 0, 0, 0, 0 };
+				for (unsigned k = 0; k < ce; ++k) {
+					const unsigned c = cc[u] + k;
+					if (c >= src_w)
+						break;
+					const uint32_t v = load_as_uint(srow, c, in.type);
+					w[(k * esz) / 4] |= esz == 4 ? v : v << (8 * ((k * esz) % 4));
+				}
+				raw[u] = make_uint4(w[0], w[1], w[2], w[3]);
+			}
+			lds_t<WORD> *d = xin + static_cast<size_t>(rr[u]) * pin + (cc[u] - c0);
+			const uint32_t wv[4] = { raw[u].x, raw[u].y, raw[u].z, raw[u].w };
+			if constexpr (WORD) {
+				const unsigned s = in.op == 1 ? in.shift : 0;
+				if (esz == 2) {
+					*reinterpret_cast<uint4 *>(d) = make_uint4(shift_bias2(wv[0], s), shift_bias2(wv[1], s), shift_bias2(wv[2], s), shift_bias2(wv[3], s));
+				} else {
+					uint32_t o[8];
+#pragma unroll
+					for (int k = 0; k < 4; ++k) {
+						o[2 * k] = shift_bias2((wv[k] & 0xFFu) | ((wv[k] & 0xFF00u) << 8), s);
+						o[2 * k + 1] = shift_bias2(((wv[k] >> 16) & 0xFFu) | ((wv[k] >> 8) & 0xFF0000u), s);
+					}
+					reinterpret_cast<uint4 *>(d)[0] = make_uint4(o[0], o[1], o[2], o[3]);
+					reinterpret_cast<uint4 *>(d)[1] = make_uint4(o[4], o[5], o[6], o[7]);
+				}
+			} else {
+				if (esz == 4) {
+					float4 v = make_float4(__uint_as_float(wv[0]), __uint_as_float(wv[1]), __uint_as_float(wv[2]), __uint_as_float(wv[3]));
+					if (in.op == 2) {
+						v.x = v.x * in.scale + in.offset;
+						v.y = v.y * in.scale + in.offset;
+						v.z = v.z * in.scale + in.offset;
+						v.w = v.w * in.scale + in.offset;
+					}
+					*reinterpret_cast<float4 *>(d) = v;
+				} else {
+					for (unsigned k = 0; k < ce; k += 4) {
+						float f[4];
+#pragma unroll
+						for (unsigned q = 0; q < 4; ++q) {
+							const unsigned kk = k + q;
+							const uint32_t word = wv[(kk * esz) / 4];
+							const uint32_t bits = esz == 1 ? (word >> (8 * (kk % 4))) & 0xFFu : (word >> (16 * (kk % 2))) & 0xFFFFu;
+							float x = in.type == 2 ? half_to_float(static_cast<uint16_t>(bits)) : static_cast<float>(bits);
+							if (in.op == 2)
+								x = x * in.scale + in.offset;
+							f[q] = x;
+						}
+						*reinterpret_cast<float4 *>(d + k) = make_float4(f[0], f[1], f[2], f[3]);
+					}
+				}
+			}
+		}
+	}
+}
+
+// Horizontal taps of a thread's two output columns, in registers. WORD:
+// N packed int16 pairs per column over the even-aligned window (dot2), dword
+// offsets into an LDS row. FLOAT: N float taps, sample offsets.
+template <bool WORD, int N>
+struct HRegs {
 	using C = std::conditional_t<WORD, int32_t, float>;
-	C c0[FWH], c1[FWH];
+	C c0[N], c1[N];
 	int l0, l1;
 
-	__device__ __forceinline__ void load(const PassArg &p, unsigned j, bool two, int base)
-	{
-		const unsigned j1 = two ? j + 1 : j;
-		l0 = static_cast<int>(p.left[j]) - base;
-		l1 = static_cast<int>(p.left[j1]) - base;
-#pragma unroll
-		for (int k = 0; k < FWH; ++k) {
-			if (k < p.fw) {
-				if constexpr (WORD) {
-					c0[k] = p.ci[static_cast<size_t>(j) * p.fw + k];
-					c1[k] = p.ci[static_cast<size_t>(j1) * p.fw + k];
-				} else {
-					c0[k] = p.cf[static_cast<size_t>(j) * p.fw + k];
-					c1[k] = p.cf[static_cast<size_t>(j1) * p.fw + k];
-				}
-			}
-		}
-	}
-
-	__device__ __forceinline__ void apply(const lds_t<WORD> *x, int fw, int pixel_max, lds_t<WORD> &v0, lds_t<WORD> &v1) const
+	// j, j1: the two columns (equal for a lone last column); c0: first staged sample.
+	__device__ __forceinline__ void load(const PassArg &p, unsigned j, unsigned j1, unsigned cbase)
 	{
 		if constexpr (WORD) {
-			int32_t a0 = 0, a1 = 0;
+			l0 = static_cast<int>(p.pl[j] - cbase) / 2;
+			l1 = static_cast<int>(p.pl[j1] - cbase) / 2;
 #pragma unroll
-			for (int k = 0; k < FWH; ++k) {
-				if (k < fw) {
-					a0 += c0[k] * (static_cast<int32_t>(x[l0 + k]) - 32768);
-					a1 += c1[k] * (static_cast<int32_t>(x[l1 + k]) - 32768);
-				}
+			for (int k = 0; k < N; ++k) {
+				c0[k] = k < p.fwp ? p.pc[static_cast<size_t>(j) * p.fwp + k] : 0;
+				c1[k] = k < p.fwp ? p.pc[static_cast<size_t>(j1) * p.fwp + k] : 0;
 			}
-			v0 = pack_word(a0, pixel_max);
-			v1 = pack_word(a1, pixel_max);
 		} else {
-			float a0 = 0, a1 = 0;
+			l0 = static_cast<int>(p.left[j] - cbase);
+			l1 = static_cast<int>(p.left[j1] - cbase);
 #pragma unroll
-			for (int k = 0; k < FWH; ++k) {
-				if (k < fw) {
-					a0 += c0[k] * x[l0 + k];
-					a1 += c1[k] * x[l1 + k];
-				}
+			for (int k = 0; k < N; ++k) {
+				c0[k] = k < p.fw ? p.cf[static_cast<size_t>(j) * p.fw + k] : 0.0f;
+				c1[k] = k < p.fw ? p.cf[static_cast<size_t>(j1) * p.fw + k] : 0.0f;
 			}
-			v0 = a0;
-			v1 = a1;
 		}
 	}
-};
 
-template <bool WORD>
-__device__ __forceinline__ void v_taps_pair(const lds_t<WORD> *x, unsigned pitch, const PassArg &p, unsigned row, int pixel_max,
-                                            lds_t<WORD> &v0, lds_t<WORD> &v1)
-{
-	if constexpr (WORD) {
-		const int16_t *c = p.ci + static_cast<size_t>(row) * p.fw;
-		int32_t a0 = 0, a1 = 0;
-		for (int k = 0; k < p.fw; ++k) {
-			const uint32_t pr = *reinterpret_cast<const uint32_t *>(x + static_cast<size_t>(k) * pitch);
-			const int32_t ck = c[k];
-			a0 += ck * (static_cast<int32_t>(pr & 0xFFFF) - 32768);
-			a1 += ck * (static_cast<int32_t>(pr >> 16) - 32768);
+	// WORD: accumulators of one LDS row (dwords); FLOAT: sequential sums.
+	__device__ __forceinline__ void apply_word(const uint32_t *row, int fwp, int32_t &a0, int32_t &a1) const
+	{
+		a0 = 0;
+		a1 = 0;
+#pragma unroll
+		for (int k = 0; k < N; ++k) {
+			if (k < fwp) {
+				a0 = dot2(row[l0 + k], c0[k], a0);
+				a1 = dot2(row[l1 + k], c1[k], a1);
+			}
 		}
-		v0 = pack_word(a0, pixel_max);
-		v1 = pack_word(a1, pixel_max);
-	} else {
-		const float *c = p.cf + static_cast<size_t>(row) * p.fw;
-		float a0 = 0, a1 = 0;
-		for (int k = 0; k < p.fw; ++k) {
-			const float2 pr = *reinterpret_cast<const float2 *>(x + static_cast<size_t>(k) * pitch);
-			a0 += c[k] * pr.x;
-			a1 += c[k] * pr.y;
+	}
+
+	__device__ __forceinline__ void apply_float(const float *row, int fw, float &v0, float &v1) const
+	{
+		float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll
+		for (int k = 0; k < N; ++k) {
+			if (k < fw) {
+				a0 += c0[k] * row[l0 + k];
+				a1 += c1[k] * row[l1 + k];
+			}
 		}
 		v0 = a0;
 		v1 = a1;
 	}
-}
+};
 
-template <bool WORD, int FWH>
-__global__ void __launch_bounds__(kThreads) k_resize_fast(ResizeArg a)
+template <bool WORD, int N>
+__global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 {
 	using T = lds_t<WORD>;
 	extern __shared__ __align__(16) unsigned char smem[];
 	T *xin = reinterpret_cast<T *>(smem);
-	T *mid = xin + static_cast<size_t>(a.lds_in_rows) * a.lds_in_cols;
+	unsigned char *mid_base = smem + ((static_cast<size_t>(a.lds_in_rows) * a.lds_in_cols * sizeof(T) + 15) & ~static_cast<size_t>(15));
 
 	const unsigned t = threadIdx.x;
 	const unsigned f = blockIdx.z;
@@ -534,136 +565,207 @@ __global__ void __launch_bounds__(kThreads) k_resize_fast(ResizeArg a)
 	const unsigned ncols = min(a.tile_w, a.dst_w - j0);
 	const char *src = frame_ptr(a.src, f);
 	char *dst = frame_ptr_w(a.dst, f);
-	const unsigned jc = 2 * t;            // this thread's column pair inside the tile
-	const bool active = jc < ncols;
-	const bool two = jc + 1 < ncols;
-	const unsigned j = j0 + (active ? jc : 0);
+	const unsigned pin = a.lds_in_cols;
 
 	if (a.h_first) {
+		// Tile width 2 * kThreads: thread t owns output columns j0 + 2t, +1.
 		const PassArg &ph = a.p1;
-		const unsigned r0 = a.passes == 2 ? a.p2.left[i0] : i0;
-		const unsigned r1 = a.passes == 2 ? a.p2.left[i0 + nrows - 1] + a.p2.fw : i0 + nrows;
-		const unsigned c0 = ph.left[j0] & ~1u;
-		const unsigned c1 = ph.left[j0 + ncols - 1] + ph.fw;
-		const unsigned pin = a.lds_in_cols;
-		const unsigned npairs = (c1 - c0 + 1) / 2;
-
-		// Stage source rows [r0, r1) x cols [c0, c1) (c0 even) in LDS, in column pairs.
-		for (unsigned r = 0; r < r1 - r0; ++r) {
-			const char *srow = row_ptr(src, a.src.stride, r0 + r);
-			for (unsigned q = t; q < npairs; q += kThreads) {
-				const Pair<WORD> p = load_pair<WORD>(srow, c0 + 2 * q, a.src_w, a.in);
-				T *d = xin + r * pin + 2 * q;
-				d[0] = p.a;
-				d[1] = p.b;
-			}
+		const unsigned jc = 2 * t;
+		const bool active = jc < ncols;
+		const bool two = jc + 1 < ncols;
+		const unsigned j = j0 + (active ? jc : 0);
+		const unsigned j1 = two ? j + 1 : j;
+		const unsigned jl = j0 + ncols - 1;
+		const unsigned cb = (WORD ? ph.pl[j0] : ph.left[j0]) & ~15u;
+		const unsigned ce = WORD ? ph.pl[jl] + 2 * ph.fwp : ph.left[jl] + ph.fw;
+		const unsigned nc = (ce - cb + 15) & ~15u;
+		unsigned r0, r1;
+		if (a.passes == 2) {
+			const PassArg &pv = a.p2;
+			const unsigned il = i0 + nrows - 1;
+			r0 = WORD ? pv.pl[i0] : pv.left[i0];
+			r1 = WORD ? pv.pl[il] + 2 * pv.fwp : pv.left[il] + pv.fw;
+		} else {
+			r0 = i0;
+			r1 = i0 + nrows;
 		}
+		const unsigned span = r1 - r0;
+		stage_tile<WORD>(xin, pin, src, a.src.stride, r0, span, a.src_h, cb, nc, a.src_w, a.in);
 		__syncthreads();
 
-		HTaps<WORD, FWH> taps;
-		taps.load(ph, j, two, static_cast<int>(c0));
-		const unsigned span = r1 - r0;
-		const unsigned pm = a.lds_mid_cols;
-		for (unsigned r = 0; r < span; ++r) {
-			T v0, v1;
-			taps.apply(xin + r * pin, ph.fw, a.pixel_max, v0, v1);
-			if (!active)
-				continue;
+		HRegs<WORD, N> h;
+		h.load(ph, j, j1, cb);
+		if constexpr (WORD) {
+			const uint32_t *xw = reinterpret_cast<const uint32_t *>(xin);
+			const unsigned pw = pin / 2;
 			if (a.passes == 2) {
-				mid[r * pm + jc] = v0;
-				mid[r * pm + jc + 1] = v1;
-			} else {
-				store_pair<WORD>(row_ptr(dst, a.dst.stride, r0 + r), j, v0, v1, two, a.out, r0 + r);
-			}
-		}
-		if (a.passes == 2) {
-			__syncthreads();
-			const PassArg &pv = a.p2;
-			for (unsigned i = 0; i < nrows; ++i) {
-				const unsigned top = pv.left[i0 + i] - r0;
-				T v0, v1;
-				v_taps_pair<WORD>(mid + top * pm + jc, pm, pv, i0 + i, a.pixel_max, v0, v1);
-				if (active)
-					store_pair<WORD>(row_ptr(dst, a.dst.stride, i0 + i), j, v0, v1, two, a.out, i0 + i);
-			}
-		}
-	} else if (a.passes == 1) {
-		// Vertical pass only: column pairs straight from HBM to HBM.
-		const PassArg &pv = a.p1;
-		for (unsigned i = 0; i < nrows; ++i) {
-			const unsigned top = pv.left[i0 + i];
-			T v0, v1;
-			if constexpr (WORD) {
-				const int16_t *cv = pv.ci + static_cast<size_t>(i0 + i) * pv.fw;
-				int32_t a0 = 0, a1 = 0;
-				for (int k = 0; k < pv.fw; ++k) {
-					const Pair<true> p = load_pair<true>(row_ptr(src, a.src.stride, top + k), j, j0 + ncols, a.in);
-					a0 += static_cast<int32_t>(cv[k]) * (static_cast<int32_t>(p.a) - 32768);
-					a1 += static_cast<int32_t>(cv[k]) * (static_cast<int32_t>(p.b) - 32768);
+				// Row pairs (r0 even, span even) -> interleaved intermediate.
+				uint32_t *mid = reinterpret_cast<uint32_t *>(mid_base);
+				const unsigned pm = a.lds_mid_cols;
+				for (unsigned r = 0; r < span; r += 2) {
+					int32_t e0, e1, o0, o1;
+					h.apply_word(xw + r * pw, ph.fwp, e0, e1);
+					h.apply_word(xw + (r + 1) * pw, ph.fwp, o0, o1);
+					const uint32_t m0 = pack_biased(e0, a.pixel_max) | (pack_biased(o0, a.pixel_max) << 16);
+					const uint32_t m1 = pack_biased(e1, a.pixel_max) | (pack_biased(o1, a.pixel_max) << 16);
+					*reinterpret_cast<uint2 *>(mid + (r / 2) * pm + jc) = make_uint2(m0, m1);
 				}
-				v0 = pack_word(a0, a.pixel_max);
-				v1 = pack_word(a1, a.pixel_max);
-			} else {
-				const float *cv = pv.cf + static_cast<size_t>(i0 + i) * pv.fw;
-				float a0 = 0, a1 = 0;
-				for (int k = 0; k < pv.fw; ++k) {
-					const Pair<false> p = load_pair<false>(row_ptr(src, a.src.stride, top + k), j, j0 + ncols, a.in);
-					a0 += cv[k] * p.a;
-					a1 += cv[k] * p.b;
+				__syncthreads();
+				const PassArg &pv = a.p2;
+				for (unsigned i = 0; i < nrows; ++i) {
+					const unsigned row = i0 + i;
+					const unsigned rp = (pv.pl[row] - r0) / 2;
+					const int32_t *cv = pv.pc + static_cast<size_t>(row) * pv.fwp;
+					int32_t a0 = 0, a1 = 0;
+					for (int k = 0; k < pv.fwp; ++k) {
+						const uint2 m = *reinterpret_cast<const uint2 *>(mid + (rp + k) * pm + jc);
+						a0 = dot2(m.x, cv[k], a0);
+						a1 = dot2(m.y, cv[k], a1);
+					}
+					if (active)
+						store_pair<true>(row_ptr(dst, a.dst.stride, row), j, pack_word(a0, a.pixel_max), pack_word(a1, a.pixel_max), two, a.out, row);
 				}
-				v0 = a0;
-				v1 = a1;
+			} else {
+				for (unsigned r = 0; r < span; ++r) {
+					int32_t a0, a1;
+					h.apply_word(xw + r * pw, ph.fwp, a0, a1);
+					if (active)
+						store_pair<true>(row_ptr(dst, a.dst.stride, r0 + r), j, pack_word(a0, a.pixel_max), pack_word(a1, a.pixel_max), two, a.out, r0 + r);
+				}
 			}
-			if (active)
-				store_pair<WORD>(row_ptr(dst, a.dst.stride, i0 + i), j, v0, v1, two, a.out, i0 + i);
+		} else {
+			if (a.passes == 2) {
+				float *mid = reinterpret_cast<float *>(mid_base);
+				const unsigned pm = a.lds_mid_cols;
+				for (unsigned r = 0; r < span; ++r) {
+					float v0, v1;
+					h.apply_float(xin + r * pin, ph.fw, v0, v1);
+					*reinterpret_cast<float2 *>(mid + r * pm + jc) = make_float2(v0, v1);
+				}
+				__syncthreads();
+				const PassArg &pv = a.p2;
+				for (unsigned i = 0; i < nrows; ++i) {
+					const unsigned row = i0 + i;
+					const unsigned top = pv.left[row] - r0;
+					const float *cv = pv.cf + static_cast<size_t>(row) * pv.fw;
+					float a0 = 0.0f, a1 = 0.0f;
+					for (int k = 0; k < pv.fw; ++k) {
+						const float2 m = *reinterpret_cast<const float2 *>(mid + (top + k) * pm + jc);
+						a0 += cv[k] * m.x;
+						a1 += cv[k] * m.y;
+					}
+					if (active)
+						store_pair<false>(row_ptr(dst, a.dst.stride, row), j, a0, a1, two, a.out, row);
+				}
+			} else {
+				for (unsigned r = 0; r < span; ++r) {
+					float v0, v1;
+					h.apply_float(xin + r * pin, ph.fw, v0, v1);
+					if (active)
+						store_pair<false>(row_ptr(dst, a.dst.stride, r0 + r), j, v0, v1, two, a.out, r0 + r);
+				}
+			}
 		}
-	} else {
-		// V first: vertical pass from HBM into the intermediate tile (the source
-		// columns the horizontal pass needs), then horizontal pass to HBM.
-		const PassArg &pv = a.p1;
+		return;
+	}
+
+	// Vertical pass first (or alone): stage the input rows of the tile's
+	// output rows over the columns the horizontal pass (if any) reads.
+	const PassArg &pv = a.p1;
+	const unsigned il = i0 + nrows - 1;
+	const unsigned r0 = pv.left[i0];
+	const unsigned span = pv.left[il] + pv.fw - r0;
+	unsigned cb, nc;
+	if (a.passes == 2) {
 		const PassArg &ph = a.p2;
-		const unsigned c0 = ph.left[j0] & ~1u;
-		const unsigned c1 = ph.left[j0 + ncols - 1] + ph.fw;
-		const unsigned pm = a.lds_mid_cols;
-		const unsigned npairs = (c1 - c0 + 1) / 2;
-		for (unsigned q = t; q < npairs; q += kThreads) {
-			const unsigned c = c0 + 2 * q;
-			for (unsigned i = 0; i < nrows; ++i) {
-				const unsigned top = pv.left[i0 + i];
+		const unsigned jl = j0 + ncols - 1;
+		cb = (WORD ? ph.pl[j0] : ph.left[j0]) & ~15u;
+		const unsigned ce = WORD ? ph.pl[jl] + 2 * ph.fwp : ph.left[jl] + ph.fw;
+		nc = (ce - cb + 15) & ~15u;
+	} else {
+		cb = j0;
+		nc = (ncols + 15) & ~15u;
+	}
+	stage_tile<WORD>(xin, pin, src, a.src.stride, r0, span, a.src_h, cb, nc, a.src_w, a.in);
+	__syncthreads();
+
+	// Vertical pass over column pairs: groups of threads split the rows.
+	const unsigned pairs = nc / 2;
+	const unsigned groups = pairs >= kThreads ? 1 : kThreads / pairs;
+	const unsigned grp = t / (pairs >= kThreads ? kThreads : pairs);
+	const unsigned pm = a.lds_mid_cols;
+	if (grp < groups) {
+		for (unsigned q = t - grp * (pairs >= kThreads ? kThreads : pairs); q < pairs; q += kThreads) {
+			for (unsigned i = grp; i < nrows; i += groups) {
+				const unsigned row = i0 + i;
+				const unsigned top = pv.left[row] - r0;
 				T v0, v1;
 				if constexpr (WORD) {
-					const int16_t *cv = pv.ci + static_cast<size_t>(i0 + i) * pv.fw;
+					const uint32_t *xw = reinterpret_cast<const uint32_t *>(xin);
+					const int16_t *cv = pv.ci + static_cast<size_t>(row) * pv.fw;
 					int32_t a0 = 0, a1 = 0;
 					for (int k = 0; k < pv.fw; ++k) {
-						const Pair<true> p = load_pair<true>(row_ptr(src, a.src.stride, top + k), c, a.src_w, a.in);
-						a0 += static_cast<int32_t>(cv[k]) * (static_cast<int32_t>(p.a) - 32768);
-						a1 += static_cast<int32_t>(cv[k]) * (static_cast<int32_t>(p.b) - 32768);
+						const uint32_t w = xw[(top + k) * (pin / 2) + q];
+						const int32_t c = cv[k];
+						a0 += c * static_cast<int32_t>(static_cast<int16_t>(w & 0xFFFFu));
+						a1 += c * (static_cast<int32_t>(w) >> 16);
+					}
+					if (a.passes == 2) {
+						reinterpret_cast<uint32_t *>(mid_base)[i * (pm / 2) + q] = pack_biased(a0, a.pixel_max) | (pack_biased(a1, a.pixel_max) << 16);
+						continue;
 					}
 					v0 = pack_word(a0, a.pixel_max);
 					v1 = pack_word(a1, a.pixel_max);
 				} else {
-					const float *cv = pv.cf + static_cast<size_t>(i0 + i) * pv.fw;
-					float a0 = 0, a1 = 0;
+					const float *cv = pv.cf + static_cast<size_t>(row) * pv.fw;
+					float a0 = 0.0f, a1 = 0.0f;
 					for (int k = 0; k < pv.fw; ++k) {
-						const Pair<false> p = load_pair<false>(row_ptr(src, a.src.stride, top + k), c, a.src_w, a.in);
-						a0 += cv[k] * p.a;
-						a1 += cv[k] * p.b;
+						const float2 m = *reinterpret_cast<const float2 *>(xin + (top + k) * pin + 2 * q);
+						a0 += cv[k] * m.x;
+						a1 += cv[k] * m.y;
+					}
+					if (a.passes == 2) {
+						*reinterpret_cast<float2 *>(reinterpret_cast<float *>(mid_base) + i * pm + 2 * q) = make_float2(a0, a1);
+						continue;
 					}
 					v0 = a0;
 					v1 = a1;
 				}
-				mid[i * pm + 2 * q] = v0;
-				mid[i * pm + 2 * q + 1] = v1;
+				const unsigned jc = 2 * q;
+				if (jc < ncols)
+					store_pair<WORD>(row_ptr(dst, a.dst.stride, row), j0 + jc, v0, v1, jc + 1 < ncols, a.out, row);
 			}
 		}
-		__syncthreads();
-		HTaps<WORD, FWH> taps;
-		taps.load(ph, j, two, static_cast<int>(c0));
-		for (unsigned i = 0; i < nrows; ++i) {
-			T v0, v1;
-			taps.apply(mid + i * pm, ph.fw, a.pixel_max, v0, v1);
-			if (active)
-				store_pair<WORD>(row_ptr(dst, a.dst.stride, i0 + i), j, v0, v1, two, a.out, i0 + i);
+	}
+	if (a.passes == 1)
+		return;
+	__syncthreads();
+
+	// Horizontal pass from the intermediate rows.
+	const PassArg &ph = a.p2;
+	const unsigned opairs = (ncols + 1) / 2;
+	const unsigned ogroups = opairs >= kThreads ? 1 : kThreads / opairs;
+	const unsigned ostride = opairs >= kThreads ? kThreads : opairs;
+	const unsigned og = t / ostride;
+	if (og >= ogroups)
+		return;
+	for (unsigned q = t - og * ostride; q < opairs; q += kThreads) {
+		const unsigned jc = 2 * q;
+		const bool two = jc + 1 < ncols;
+		const unsigned j = j0 + jc;
+		HRegs<WORD, N> h;
+		h.load(ph, j, two ? j + 1 : j, cb);
+		for (unsigned i = og; i < nrows; i += ogroups) {
+			const unsigned row = i0 + i;
+			if constexpr (WORD) {
+				int32_t a0, a1;
+				h.apply_word(reinterpret_cast<const uint32_t *>(mid_base) + i * (pm / 2), ph.fwp, a0, a1);
+				store_pair<true>(row_ptr(dst, a.dst.stride, row), j, pack_word(a0, a.pixel_max), pack_word(a1, a.pixel_max), two, a.out, row);
+			} else {
+				float v0, v1;
+				h.apply_float(reinterpret_cast<const float *>(mid_base) + i * pm, ph.fw, v0, v1);
+				store_pair<false>(row_ptr(dst, a.dst.stride, row), j, v0, v1, two, a.out, row);
+			}
 		}
 	}
 }
@@ -918,12 +1020,10 @@ int launch_resize_t(const ResizeArg &a, void *stream)
 	hipStream_t s = static_cast<hipStream_t>(stream);
 	switch (a.fwh) {
 	case 0: hipLaunchKernelGGL(k_resize<WORD>, grid, dim3(kThreads), lds, s, a); break;
-	case 4: hipLaunchKernelGGL((k_resize_fast<WORD, 4>), grid, dim3(kThreads), lds, s, a); break;
-	case 8: hipLaunchKernelGGL((k_resize_fast<WORD, 8>), grid, dim3(kThreads), lds, s, a); break;
-	case 12: hipLaunchKernelGGL((k_resize_fast<WORD, 12>), grid, dim3(kThreads), lds, s, a); break;
-	case 16: hipLaunchKernelGGL((k_resize_fast<WORD, 16>), grid, dim3(kThreads), lds, s, a); break;
-	case 24: hipLaunchKernelGGL((k_resize_fast<WORD, 24>), grid, dim3(kThreads), lds, s, a); break;
-	case 32: hipLaunchKernelGGL((k_resize_fast<WORD, 32>), grid, dim3(kThreads), lds, s, a); break;
+#define ZAMD_TILED(n) case n: hipLaunchKernelGGL((k_resize_tiled<WORD, n>), grid, dim3(kThreads), lds, s, a); break;
+	ZAMD_TILED(2) ZAMD_TILED(3) ZAMD_TILED(4) ZAMD_TILED(5) ZAMD_TILED(6)
+	ZAMD_TILED(8) ZAMD_TILED(12) ZAMD_TILED(16) ZAMD_TILED(24) ZAMD_TILED(32)
+#undef ZAMD_TILED
 	default: return static_cast<int>(hipErrorInvalidValue);
 	}
 	return static_cast<int>(hipGetLastError());
@@ -934,7 +1034,8 @@ int launch_resize_t(const ResizeArg &a, void *stream)
 size_t resize_lds_bytes(const ResizeArg &a)
 {
 	const size_t elem = a.word ? 2 : 4;
-	return elem * (static_cast<size_t>(a.lds_in_rows) * a.lds_in_cols + static_cast<size_t>(a.lds_mid_rows) * a.lds_mid_cols);
+	const size_t in = (elem * a.lds_in_rows * a.lds_in_cols + 15) & ~static_cast<size_t>(15);
+	return in + elem * a.lds_mid_rows * a.lds_mid_cols;
 }
 
 int launch_resize(const ResizeArg &a, void *stream)
