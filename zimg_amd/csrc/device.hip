@@ -375,10 +375,10 @@ void DeviceProgram::check_tiled(const Step &s, const FilterTable *th_tab, const 
 		unsigned cb = j0, nc = (nc_out + 15) & ~15u;
 		if (th_tab) {
 			cb = (w ? pair_left(*th_tab, j0) : th_tab->left[j0]) & ~15u;
-			nc = ((w ? pair_left(*th_tab, jl) + 2 * pair_count(*th_tab) : th_tab->left[jl] + th_tab->filter_width) - cb + 15) & ~15u;
+			nc = ((w ? pair_left(*th_tab, jl) + 2 * static_cast<unsigned>(s.fast_fwh) : th_tab->left[jl] + th_tab->filter_width) - cb + 15) & ~15u;
 			for (unsigned j = j0; j <= jl; ++j) {
 				const unsigned lo = w ? pair_left(*th_tab, j) : th_tab->left[j];
-				const unsigned width = w ? 2 * pair_count(*th_tab) : th_tab->filter_width;
+				const unsigned width = w ? 2 * static_cast<unsigned>(s.fast_fwh) : th_tab->filter_width;
 				if (lo < cb || lo - cb + width > nc || (w && (lo - cb) % 2))
 					bad("horizontal window");
 			}
@@ -436,9 +436,11 @@ void DeviceProgram::configure_resize(Step &s)
 		auto h_cols = [&](const FilterTable &t, unsigned j0, unsigned ncols) {
 			const unsigned jl = j0 + ncols - 1;
 			const unsigned cb = (s.word ? pair_left(t, j0) : t.left[j0]) & ~15u;
-			const unsigned ce = s.word ? pair_left(t, jl) + 2 * pair_count(t) : t.left[jl] + t.filter_width;
+			const unsigned ce = s.word ? pair_left(t, jl) + 2 * static_cast<unsigned>(fwh) : t.left[jl] + t.filter_width;
 			return (ce - cb + 15) & ~15u;
 		};
+		// Per-row vertical taps held in LDS for a tile row (start row + taps).
+		const unsigned vtaps = (s.h_first && !t2) ? 0 : (s.word && s.h_first ? pair_count(*tv_tab) : tv_tab->filter_width);
 		auto ceil_div = [](size_t a, size_t b) { return (a + b - 1) / b; };
 		double best = 0;
 		std::vector<unsigned> widths = s.h_first ? std::vector<unsigned>{ dev::kFastTileW } : std::vector<unsigned>{ 512, 256, 128, 64, 32, 16 };
@@ -465,7 +467,8 @@ void DeviceProgram::configure_resize(Step &s)
 					mid_rows = s.h_first ? in_rows : th;
 					mid_cols = s.h_first ? tw : in_cols;
 				}
-				const size_t bytes = align_up(elem * in_rows * in_cols, 16) + elem * mid_rows * mid_cols;
+				const size_t vtab = vtaps ? align_up(static_cast<size_t>(th) * (1 + vtaps) * 4, 16) : 0;
+				const size_t bytes = vtab + align_up(elem * in_rows * in_cols, 16) + elem * mid_rows * mid_cols;
 				if (bytes > kLdsBudget)
 					continue;
 				// Cost model: per-block work in thread iterations (staging, first and

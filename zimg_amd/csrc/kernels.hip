@@ -492,14 +492,16 @@ __device__ __forceinline__ void stage_tile(lds_t<WORD> *xin, unsigned pin, const
 
 // Horizontal taps of a thread's two output columns, in registers. WORD:
 // N packed int16 pairs per column over the even-aligned window (dot2), dword
-// offsets into an LDS row. FLOAT: N float taps, sample offsets.
+// offsets into an LDS row; pairs past the table's fwp are zero, so all N are
+// applied without a branch (the staged window covers 2N samples). FLOAT: the
+// first fw of N float taps, sample offsets; taps past fw leave the sum as is.
 template <bool WORD, int N>
 struct HRegs {
 	using C = std::conditional_t<WORD, int32_t, float>;
 	C c0[N], c1[N];
 	int l0, l1;
 
-	// j, j1: the two columns (equal for a lone last column); c0: first staged sample.
+	// j, j1: the two columns (equal for a lone last column); cbase: first staged sample.
 	__device__ __forceinline__ void load(const PassArg &p, unsigned j, unsigned j1, unsigned cbase)
 	{
 		if constexpr (WORD) {
@@ -521,18 +523,16 @@ struct HRegs {
 		}
 	}
 
-	// WORD: accumulators of one LDS row (dwords); FLOAT: sequential sums.
-	__device__ __forceinline__ void apply_word(const uint32_t *row, int fwp, int32_t &a0, int32_t &a1) const
+	__device__ __forceinline__ void apply_word(const uint32_t *row, int32_t &a0, int32_t &a1) const
 	{
-		a0 = 0;
-		a1 = 0;
+		int32_t s0 = 0, s1 = 0;
 #pragma unroll
 		for (int k = 0; k < N; ++k) {
-			if (k < fwp) {
-				a0 = dot2(row[l0 + k], c0[k], a0);
-				a1 = dot2(row[l1 + k], c1[k], a1);
-			}
+			s0 = dot2(row[l0 + k], c0[k], s0);
+			s1 = dot2(row[l1 + k], c1[k], s1);
 		}
+		a0 = s0;
+		a1 = s1;
 	}
 
 	__device__ __forceinline__ void apply_float(const float *row, int fw, float &v0, float &v1) const
@@ -540,23 +540,51 @@ struct HRegs {
 		float a0 = 0.0f, a1 = 0.0f;
 #pragma unroll
 		for (int k = 0; k < N; ++k) {
-			if (k < fw) {
-				a0 += c0[k] * row[l0 + k];
-				a1 += c1[k] * row[l1 + k];
-			}
+			const int kk = k < fw ? k : fw - 1; // stays inside the window; the sum ignores it
+			const float x0 = row[l0 + kk], x1 = row[l1 + kk];
+			const float s0 = a0 + c0[k] * x0, s1 = a1 + c1[k] * x1;
+			a0 = k < fw ? s0 : a0;
+			a1 = k < fw ? s1 : a1;
 		}
 		v0 = a0;
 		v1 = a1;
 	}
 };
 
+// LDS layout of the tiled kernel: per-row vertical table of the tile (start
+// row, then the row's taps; copied once so the row loops read LDS, not HBM),
+// the staged source window, then the intermediate.
+__host__ __device__ inline unsigned tiled_vtab_words(const ResizeArg &a)
+{
+	if (a.h_first && a.passes == 1)
+		return 0;
+	const PassArg &pv = a.h_first ? a.p2 : a.p1;
+	const unsigned taps = a.word && a.h_first ? pv.fwp : pv.fw;
+	return a.tile_h * (1 + taps);
+}
+
+__host__ __device__ inline size_t tiled_lds_bytes(const ResizeArg &a, size_t *in_off, size_t *mid_off)
+{
+	const size_t elem = a.word ? 2 : 4;
+	const size_t tab = (static_cast<size_t>(tiled_vtab_words(a)) * 4 + 15) & ~static_cast<size_t>(15);
+	const size_t in = (elem * a.lds_in_rows * a.lds_in_cols + 15) & ~static_cast<size_t>(15);
+	if (in_off)
+		*in_off = tab;
+	if (mid_off)
+		*mid_off = tab + in;
+	return tab + in + elem * a.lds_mid_rows * a.lds_mid_cols;
+}
+
 template <bool WORD, int N>
 __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 {
 	using T = lds_t<WORD>;
 	extern __shared__ __align__(16) unsigned char smem[];
-	T *xin = reinterpret_cast<T *>(smem);
-	unsigned char *mid_base = smem + ((static_cast<size_t>(a.lds_in_rows) * a.lds_in_cols * sizeof(T) + 15) & ~static_cast<size_t>(15));
+	size_t in_off, mid_off;
+	tiled_lds_bytes(a, &in_off, &mid_off);
+	uint32_t *vtab = reinterpret_cast<uint32_t *>(smem);
+	T *xin = reinterpret_cast<T *>(smem + in_off);
+	unsigned char *mid_base = smem + mid_off;
 
 	const unsigned t = threadIdx.x;
 	const unsigned f = blockIdx.z;
@@ -566,6 +594,28 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 	const char *src = frame_ptr(a.src, f);
 	char *dst = frame_ptr_w(a.dst, f);
 	const unsigned pin = a.lds_in_cols;
+
+	// Vertical table rows of this tile: vtab[i] = first source row,
+	// vtab[nrows + i * taps + k] = tap k (int16 pairs for WORD after a
+	// horizontal pass, else int16 or float bits).
+	unsigned vtaps = 0;
+	if (!(a.h_first && a.passes == 1)) {
+		const PassArg &pv = a.h_first ? a.p2 : a.p1;
+		const bool pairs = WORD && a.h_first;
+		vtaps = pairs ? pv.fwp : pv.fw;
+		for (unsigned e = t; e < nrows * (1 + vtaps); e += kThreads) {
+			uint32_t v;
+			if (e < nrows) {
+				v = pairs ? pv.pl[i0 + e] : pv.left[i0 + e];
+			} else {
+				const size_t k = static_cast<size_t>(i0) * vtaps + (e - nrows);
+				v = pairs ? static_cast<uint32_t>(pv.pc[k]) : WORD ? static_cast<uint32_t>(static_cast<int32_t>(pv.ci[k])) : __float_as_uint(pv.cf[k]);
+			}
+			vtab[e] = v;
+		}
+	}
+	const uint32_t *vtop = vtab;
+	const uint32_t *vcoef = vtab + nrows;
 
 	if (a.h_first) {
 		// Tile width 2 * kThreads: thread t owns output columns j0 + 2t, +1.
@@ -577,7 +627,7 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 		const unsigned j1 = two ? j + 1 : j;
 		const unsigned jl = j0 + ncols - 1;
 		const unsigned cb = (WORD ? ph.pl[j0] : ph.left[j0]) & ~15u;
-		const unsigned ce = WORD ? ph.pl[jl] + 2 * ph.fwp : ph.left[jl] + ph.fw;
+		const unsigned ce = WORD ? ph.pl[jl] + 2 * N : ph.left[jl] + ph.fw;
 		const unsigned nc = (ce - cb + 15) & ~15u;
 		unsigned r0, r1;
 		if (a.passes == 2) {
@@ -604,23 +654,22 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 				const unsigned pm = a.lds_mid_cols;
 				for (unsigned r = 0; r < span; r += 2) {
 					int32_t e0, e1, o0, o1;
-					h.apply_word(xw + r * pw, ph.fwp, e0, e1);
-					h.apply_word(xw + (r + 1) * pw, ph.fwp, o0, o1);
+					h.apply_word(xw + r * pw, e0, e1);
+					h.apply_word(xw + (r + 1) * pw, o0, o1);
 					const uint32_t m0 = pack_biased(e0, a.pixel_max) | (pack_biased(o0, a.pixel_max) << 16);
 					const uint32_t m1 = pack_biased(e1, a.pixel_max) | (pack_biased(o1, a.pixel_max) << 16);
 					*reinterpret_cast<uint2 *>(mid + (r / 2) * pm + jc) = make_uint2(m0, m1);
 				}
 				__syncthreads();
-				const PassArg &pv = a.p2;
 				for (unsigned i = 0; i < nrows; ++i) {
 					const unsigned row = i0 + i;
-					const unsigned rp = (pv.pl[row] - r0) / 2;
-					const int32_t *cv = pv.pc + static_cast<size_t>(row) * pv.fwp;
+					const uint32_t *m = mid + ((vtop[i] - r0) / 2) * pm + jc;
+					const uint32_t *cv = vcoef + i * vtaps;
 					int32_t a0 = 0, a1 = 0;
-					for (int k = 0; k < pv.fwp; ++k) {
-						const uint2 m = *reinterpret_cast<const uint2 *>(mid + (rp + k) * pm + jc);
-						a0 = dot2(m.x, cv[k], a0);
-						a1 = dot2(m.y, cv[k], a1);
+					for (unsigned k = 0; k < vtaps; ++k) {
+						const uint2 mm = *reinterpret_cast<const uint2 *>(m + k * pm);
+						a0 = dot2(mm.x, static_cast<int32_t>(cv[k]), a0);
+						a1 = dot2(mm.y, static_cast<int32_t>(cv[k]), a1);
 					}
 					if (active)
 						store_pair<true>(row_ptr(dst, a.dst.stride, row), j, pack_word(a0, a.pixel_max), pack_word(a1, a.pixel_max), two, a.out, row);
@@ -628,7 +677,7 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 			} else {
 				for (unsigned r = 0; r < span; ++r) {
 					int32_t a0, a1;
-					h.apply_word(xw + r * pw, ph.fwp, a0, a1);
+					h.apply_word(xw + r * pw, a0, a1);
 					if (active)
 						store_pair<true>(row_ptr(dst, a.dst.stride, r0 + r), j, pack_word(a0, a.pixel_max), pack_word(a1, a.pixel_max), two, a.out, r0 + r);
 				}
@@ -643,16 +692,16 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 					*reinterpret_cast<float2 *>(mid + r * pm + jc) = make_float2(v0, v1);
 				}
 				__syncthreads();
-				const PassArg &pv = a.p2;
 				for (unsigned i = 0; i < nrows; ++i) {
 					const unsigned row = i0 + i;
-					const unsigned top = pv.left[row] - r0;
-					const float *cv = pv.cf + static_cast<size_t>(row) * pv.fw;
+					const float *m = mid + (vtop[i] - r0) * pm + jc;
+					const uint32_t *cv = vcoef + i * vtaps;
 					float a0 = 0.0f, a1 = 0.0f;
-					for (int k = 0; k < pv.fw; ++k) {
-						const float2 m = *reinterpret_cast<const float2 *>(mid + (top + k) * pm + jc);
-						a0 += cv[k] * m.x;
-						a1 += cv[k] * m.y;
+					for (unsigned k = 0; k < vtaps; ++k) {
+						const float2 mm = *reinterpret_cast<const float2 *>(m + k * pm);
+						const float c = __uint_as_float(cv[k]);
+						a0 += c * mm.x;
+						a1 += c * mm.y;
 					}
 					if (active)
 						store_pair<false>(row_ptr(dst, a.dst.stride, row), j, a0, a1, two, a.out, row);
@@ -680,7 +729,7 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 		const PassArg &ph = a.p2;
 		const unsigned jl = j0 + ncols - 1;
 		cb = (WORD ? ph.pl[j0] : ph.left[j0]) & ~15u;
-		const unsigned ce = WORD ? ph.pl[jl] + 2 * ph.fwp : ph.left[jl] + ph.fw;
+		const unsigned ce = WORD ? ph.pl[jl] + 2 * N : ph.left[jl] + ph.fw;
 		nc = (ce - cb + 15) & ~15u;
 	} else {
 		cb = j0;
@@ -691,22 +740,23 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 
 	// Vertical pass over column pairs: groups of threads split the rows.
 	const unsigned pairs = nc / 2;
-	const unsigned groups = pairs >= kThreads ? 1 : kThreads / pairs;
-	const unsigned grp = t / (pairs >= kThreads ? kThreads : pairs);
+	const unsigned pstride = pairs >= kThreads ? kThreads : pairs;
+	const unsigned groups = kThreads / pstride;
+	const unsigned grp = t / pstride;
 	const unsigned pm = a.lds_mid_cols;
 	if (grp < groups) {
-		for (unsigned q = t - grp * (pairs >= kThreads ? kThreads : pairs); q < pairs; q += kThreads) {
+		for (unsigned q = t - grp * pstride; q < pairs; q += kThreads) {
 			for (unsigned i = grp; i < nrows; i += groups) {
 				const unsigned row = i0 + i;
-				const unsigned top = pv.left[row] - r0;
+				const unsigned top = vtop[i] - r0;
+				const uint32_t *cv = vcoef + i * vtaps;
 				T v0, v1;
 				if constexpr (WORD) {
-					const uint32_t *xw = reinterpret_cast<const uint32_t *>(xin);
-					const int16_t *cv = pv.ci + static_cast<size_t>(row) * pv.fw;
+					const uint32_t *xw = reinterpret_cast<const uint32_t *>(xin) + top * (pin / 2) + q;
 					int32_t a0 = 0, a1 = 0;
-					for (int k = 0; k < pv.fw; ++k) {
-						const uint32_t w = xw[(top + k) * (pin / 2) + q];
-						const int32_t c = cv[k];
+					for (unsigned k = 0; k < vtaps; ++k) {
+						const uint32_t w = xw[k * (pin / 2)];
+						const int32_t c = static_cast<int32_t>(cv[k]);
 						a0 += c * static_cast<int32_t>(static_cast<int16_t>(w & 0xFFFFu));
 						a1 += c * (static_cast<int32_t>(w) >> 16);
 					}
@@ -717,12 +767,13 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 					v0 = pack_word(a0, a.pixel_max);
 					v1 = pack_word(a1, a.pixel_max);
 				} else {
-					const float *cv = pv.cf + static_cast<size_t>(row) * pv.fw;
+					const float *xr = xin + top * pin + 2 * q;
 					float a0 = 0.0f, a1 = 0.0f;
-					for (int k = 0; k < pv.fw; ++k) {
-						const float2 m = *reinterpret_cast<const float2 *>(xin + (top + k) * pin + 2 * q);
-						a0 += cv[k] * m.x;
-						a1 += cv[k] * m.y;
+					for (unsigned k = 0; k < vtaps; ++k) {
+						const float2 m = *reinterpret_cast<const float2 *>(xr + k * pin);
+						const float c = __uint_as_float(cv[k]);
+						a0 += c * m.x;
+						a1 += c * m.y;
 					}
 					if (a.passes == 2) {
 						*reinterpret_cast<float2 *>(reinterpret_cast<float *>(mid_base) + i * pm + 2 * q) = make_float2(a0, a1);
@@ -744,8 +795,8 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 	// Horizontal pass from the intermediate rows.
 	const PassArg &ph = a.p2;
 	const unsigned opairs = (ncols + 1) / 2;
-	const unsigned ogroups = opairs >= kThreads ? 1 : kThreads / opairs;
 	const unsigned ostride = opairs >= kThreads ? kThreads : opairs;
+	const unsigned ogroups = kThreads / ostride;
 	const unsigned og = t / ostride;
 	if (og >= ogroups)
 		return;
@@ -759,7 +810,7 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 			const unsigned row = i0 + i;
 			if constexpr (WORD) {
 				int32_t a0, a1;
-				h.apply_word(reinterpret_cast<const uint32_t *>(mid_base) + i * (pm / 2), ph.fwp, a0, a1);
+				h.apply_word(reinterpret_cast<const uint32_t *>(mid_base) + i * (pm / 2), a0, a1);
 				store_pair<true>(row_ptr(dst, a.dst.stride, row), j, pack_word(a0, a.pixel_max), pack_word(a1, a.pixel_max), two, a.out, row);
 			} else {
 				float v0, v1;
@@ -1033,9 +1084,10 @@ int launch_resize_t(const ResizeArg &a, void *stream)
 
 size_t resize_lds_bytes(const ResizeArg &a)
 {
+	if (a.fwh)
+		return tiled_lds_bytes(a, nullptr, nullptr);
 	const size_t elem = a.word ? 2 : 4;
-	const size_t in = (elem * a.lds_in_rows * a.lds_in_cols + 15) & ~static_cast<size_t>(15);
-	return in + elem * a.lds_mid_rows * a.lds_mid_cols;
+	return elem * (static_cast<size_t>(a.lds_in_rows) * a.lds_in_cols + static_cast<size_t>(a.lds_mid_rows) * a.lds_mid_cols);
 }
 
 int launch_resize(const ResizeArg &a, void *stream)
