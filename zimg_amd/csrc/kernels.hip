@@ -124,13 +124,24 @@ __device__ __forceinline__ unsigned quantize(float x, const OutXform &o, unsigne
 	return static_cast<unsigned>(static_cast<int>(__builtin_rintf(x)));
 }
 
-// pack_pixel_u16 (resize_impl.cpp:36-42)
+__device__ __forceinline__ int32_t med3_i32(int32_t x, int32_t lo, int32_t hi)
+{
+	int32_t r;
+	asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(lo), "v"(hi));
+	return r;
+}
+
+// pack_pixel_u16 (resize_impl.cpp:36-42): ((acc + 2^13) >> 14) + 32768 clamped
+// to [0, pixel_max]. Adding 32768 << 14 before the shift is the same integer.
 __device__ __forceinline__ uint16_t pack_word(int32_t acc, int32_t pixel_max)
 {
-	int32_t x = ((acc + (1 << 13)) >> 14) + 32768;
-	x = x > pixel_max ? pixel_max : x;
-	x = x < 0 ? 0 : x;
-	return static_cast<uint16_t>(x);
+	return static_cast<uint16_t>(med3_i32((acc + ((32768 << 14) + (1 << 13))) >> 14, 0, pixel_max));
+}
+
+// Two 16-bit values (low halves of lo, hi) in one dword.
+__device__ __forceinline__ uint32_t pack16x2(uint32_t lo, uint32_t hi)
+{
+	return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
 }
 
 template <bool WORD>
@@ -384,10 +395,11 @@ __device__ __forceinline__ int32_t dot2(uint32_t x, int32_t c, int32_t acc)
 	return __builtin_amdgcn_sdot2(__builtin_bit_cast(v2i16, x), __builtin_bit_cast(v2i16, c), acc, false);
 }
 
-// pack_word, returned as the biased int16 bits the next pass consumes.
+// pack_word minus 32768: the biased int16 the next pass consumes (upper
+// half of the result unspecified).
 __device__ __forceinline__ uint32_t pack_biased(int32_t acc, int32_t pixel_max)
 {
-	return static_cast<uint32_t>(pack_word(acc, pixel_max)) ^ 0x8000u;
+	return static_cast<uint32_t>(med3_i32((acc + (1 << 13)) >> 14, -32768, pixel_max - 32768));
 }
 
 // Shift every 16-bit half of w left by s (s < 16), then bias to int16.
@@ -554,13 +566,22 @@ struct HRegs {
 // LDS layout of the tiled kernel: per-row vertical table of the tile (start
 // row, then the row's taps; copied once so the row loops read LDS, not HBM),
 // the staged source window, then the intermediate.
-__host__ __device__ inline unsigned tiled_vtab_words(const ResizeArg &a)
+__host__ __device__ inline unsigned tiled_vtaps(const ResizeArg &a)
 {
 	if (a.h_first && a.passes == 1)
 		return 0;
 	const PassArg &pv = a.h_first ? a.p2 : a.p1;
-	const unsigned taps = a.word && a.h_first ? pv.fwp : pv.fw;
-	return a.tile_h * (1 + taps);
+	return a.word && a.h_first ? pv.fwp : pv.fw;
+}
+
+// Start rows occupy the first round4(tile_h) words, then one row of taps per
+// output row at a stride of round4(taps) words (16-byte aligned rows).
+__host__ __device__ inline unsigned tiled_vtab_words(const ResizeArg &a)
+{
+	const unsigned taps = tiled_vtaps(a);
+	if (!taps)
+		return 0;
+	return ((a.tile_h + 3) & ~3u) + a.tile_h * ((taps + 3) & ~3u);
 }
 
 __host__ __device__ inline size_t tiled_lds_bytes(const ResizeArg &a, size_t *in_off, size_t *mid_off)
@@ -598,24 +619,24 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 	// Vertical table rows of this tile: vtab[i] = first source row,
 	// vtab[nrows + i * taps + k] = tap k (int16 pairs for WORD after a
 	// horizontal pass, else int16 or float bits).
-	unsigned vtaps = 0;
-	if (!(a.h_first && a.passes == 1)) {
+	const unsigned vtaps = tiled_vtaps(a);
+	const unsigned vstride = (vtaps + 3) & ~3u;
+	const uint32_t *vtop = vtab;
+	const uint32_t *vcoef = vtab + ((a.tile_h + 3) & ~3u);
+	if (vtaps) {
 		const PassArg &pv = a.h_first ? a.p2 : a.p1;
 		const bool pairs = WORD && a.h_first;
-		vtaps = pairs ? pv.fwp : pv.fw;
-		for (unsigned e = t; e < nrows * (1 + vtaps); e += kThreads) {
-			uint32_t v;
-			if (e < nrows) {
-				v = pairs ? pv.pl[i0 + e] : pv.left[i0 + e];
-			} else {
-				const size_t k = static_cast<size_t>(i0) * vtaps + (e - nrows);
-				v = pairs ? static_cast<uint32_t>(pv.pc[k]) : WORD ? static_cast<uint32_t>(static_cast<int32_t>(pv.ci[k])) : __float_as_uint(pv.cf[k]);
-			}
-			vtab[e] = v;
+		for (unsigned e = t; e < nrows; e += kThreads)
+			vtab[e] = pairs ? pv.pl[i0 + e] : pv.left[i0 + e];
+		for (unsigned e = t; e < nrows * vstride; e += kThreads) {
+			const unsigned i = e / vstride, k = e - i * vstride;
+			const size_t g = static_cast<size_t>(i0 + i) * vtaps + k;
+			uint32_t v = 0;
+			if (k < vtaps)
+				v = pairs ? static_cast<uint32_t>(pv.pc[g]) : WORD ? static_cast<uint32_t>(static_cast<int32_t>(pv.ci[g])) : __float_as_uint(pv.cf[g]);
+			vtab[((a.tile_h + 3) & ~3u) + e] = v;
 		}
 	}
-	const uint32_t *vtop = vtab;
-	const uint32_t *vcoef = vtab + nrows;
 
 	if (a.h_first) {
 		// Tile width 2 * kThreads: thread t owns output columns j0 + 2t, +1.
@@ -656,23 +677,45 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 					int32_t e0, e1, o0, o1;
 					h.apply_word(xw + r * pw, e0, e1);
 					h.apply_word(xw + (r + 1) * pw, o0, o1);
-					const uint32_t m0 = pack_biased(e0, a.pixel_max) | (pack_biased(o0, a.pixel_max) << 16);
-					const uint32_t m1 = pack_biased(e1, a.pixel_max) | (pack_biased(o1, a.pixel_max) << 16);
+					const uint32_t m0 = pack16x2(pack_biased(e0, a.pixel_max), pack_biased(o0, a.pixel_max));
+					const uint32_t m1 = pack16x2(pack_biased(e1, a.pixel_max), pack_biased(o1, a.pixel_max));
 					*reinterpret_cast<uint2 *>(mid + (r / 2) * pm + jc) = make_uint2(m0, m1);
 				}
 				__syncthreads();
-				for (unsigned i = 0; i < nrows; ++i) {
+				// Vertical pass: four columns per thread, two groups of rows.
+				const unsigned j4 = 4 * (t % (kThreads / 2));
+				const unsigned g = t / (kThreads / 2);
+				const unsigned n4 = j4 < ncols ? min(4u, ncols - j4) : 0;
+				const bool wide = n4 == 4 && a.out.op == 0 && a.out.type == 1;
+				for (unsigned i = g; i < nrows; i += 2) {
 					const unsigned row = i0 + i;
-					const uint32_t *m = mid + ((vtop[i] - r0) / 2) * pm + jc;
-					const uint32_t *cv = vcoef + i * vtaps;
-					int32_t a0 = 0, a1 = 0;
-					for (unsigned k = 0; k < vtaps; ++k) {
-						const uint2 mm = *reinterpret_cast<const uint2 *>(m + k * pm);
-						a0 = dot2(mm.x, static_cast<int32_t>(cv[k]), a0);
-						a1 = dot2(mm.y, static_cast<int32_t>(cv[k]), a1);
+					const uint32_t *m = mid + ((vtop[i] - r0) / 2) * pm + j4;
+					const uint32_t *cv = vcoef + i * vstride;
+					int32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+					for (unsigned k = 0; k < vtaps; k += 4) {
+						const uint4 c4 = *reinterpret_cast<const uint4 *>(cv + k);
+						const uint32_t cc[4] = { c4.x, c4.y, c4.z, c4.w };
+#pragma unroll
+						for (unsigned u = 0; u < 4; ++u) {
+							if (k + u < vtaps) {
+								const uint4 mm = *reinterpret_cast<const uint4 *>(m + (k + u) * pm);
+								a0 = dot2(mm.x, static_cast<int32_t>(cc[u]), a0);
+								a1 = dot2(mm.y, static_cast<int32_t>(cc[u]), a1);
+								a2 = dot2(mm.z, static_cast<int32_t>(cc[u]), a2);
+								a3 = dot2(mm.w, static_cast<int32_t>(cc[u]), a3);
+							}
+						}
 					}
-					if (active)
-						store_pair<true>(row_ptr(dst, a.dst.stride, row), j, pack_word(a0, a.pixel_max), pack_word(a1, a.pixel_max), two, a.out, row);
+					char *drow = row_ptr(dst, a.dst.stride, row);
+					const uint16_t v0 = pack_word(a0, a.pixel_max), v1 = pack_word(a1, a.pixel_max);
+					const uint16_t v2 = pack_word(a2, a.pixel_max), v3 = pack_word(a3, a.pixel_max);
+					if (wide) {
+						*reinterpret_cast<uint2 *>(drow + 2 * static_cast<size_t>(j0 + j4)) = make_uint2(pack16x2(v0, v1), pack16x2(v2, v3));
+					} else if (n4) {
+						store_pair<true>(drow, j0 + j4, v0, v1, n4 > 1, a.out, row);
+						if (n4 > 2)
+							store_pair<true>(drow, j0 + j4 + 2, v2, v3, n4 > 3, a.out, row);
+					}
 				}
 			} else {
 				for (unsigned r = 0; r < span; ++r) {
@@ -692,19 +735,31 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 					*reinterpret_cast<float2 *>(mid + r * pm + jc) = make_float2(v0, v1);
 				}
 				__syncthreads();
-				for (unsigned i = 0; i < nrows; ++i) {
+				const unsigned j4 = 4 * (t % (kThreads / 2));
+				const unsigned g = t / (kThreads / 2);
+				const unsigned n4 = j4 < ncols ? min(4u, ncols - j4) : 0;
+				const bool wide = n4 == 4 && a.out.op == 0 && a.out.type == 3;
+				for (unsigned i = g; i < nrows; i += 2) {
 					const unsigned row = i0 + i;
-					const float *m = mid + (vtop[i] - r0) * pm + jc;
-					const uint32_t *cv = vcoef + i * vtaps;
-					float a0 = 0.0f, a1 = 0.0f;
+					const float *m = mid + (vtop[i] - r0) * pm + j4;
+					const uint32_t *cv = vcoef + i * vstride;
+					float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
 					for (unsigned k = 0; k < vtaps; ++k) {
-						const float2 mm = *reinterpret_cast<const float2 *>(m + k * pm);
+						const float4 mm = *reinterpret_cast<const float4 *>(m + k * pm);
 						const float c = __uint_as_float(cv[k]);
 						a0 += c * mm.x;
 						a1 += c * mm.y;
+						a2 += c * mm.z;
+						a3 += c * mm.w;
 					}
-					if (active)
-						store_pair<false>(row_ptr(dst, a.dst.stride, row), j, a0, a1, two, a.out, row);
+					char *drow = row_ptr(dst, a.dst.stride, row);
+					if (wide) {
+						*reinterpret_cast<float4 *>(drow + 4 * static_cast<size_t>(j0 + j4)) = make_float4(a0, a1, a2, a3);
+					} else if (n4) {
+						store_pair<false>(drow, j0 + j4, a0, a1, n4 > 1, a.out, row);
+						if (n4 > 2)
+							store_pair<false>(drow, j0 + j4 + 2, a2, a3, n4 > 3, a.out, row);
+					}
 				}
 			} else {
 				for (unsigned r = 0; r < span; ++r) {
@@ -749,7 +804,7 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 			for (unsigned i = grp; i < nrows; i += groups) {
 				const unsigned row = i0 + i;
 				const unsigned top = vtop[i] - r0;
-				const uint32_t *cv = vcoef + i * vtaps;
+				const uint32_t *cv = vcoef + i * vstride;
 				T v0, v1;
 				if constexpr (WORD) {
 					const uint32_t *xw = reinterpret_cast<const uint32_t *>(xin) + top * (pin / 2) + q;
@@ -761,7 +816,7 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 						a1 += c * (static_cast<int32_t>(w) >> 16);
 					}
 					if (a.passes == 2) {
-						reinterpret_cast<uint32_t *>(mid_base)[i * (pm / 2) + q] = pack_biased(a0, a.pixel_max) | (pack_biased(a1, a.pixel_max) << 16);
+						reinterpret_cast<uint32_t *>(mid_base)[i * (pm / 2) + q] = pack16x2(pack_biased(a0, a.pixel_max), pack_biased(a1, a.pixel_max));
 						continue;
 					}
 					v0 = pack_word(a0, a.pixel_max);
