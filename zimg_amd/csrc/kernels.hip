@@ -910,60 +910,98 @@ __global__ void __launch_bounds__(kThreads) k_point(PointArg a)
 
 // ColorspaceConversionImpl::process (colorspace.cpp:45-77): up to six in-place
 // operations on three float channels, fused with the depth conversions before
-// and after it.
-__device__ __forceinline__ void apply_op(const CspOp &op, float &a, float &b, float &c)
+// and after it. One operation runs over NP pixels of three channels; the
+// operation (and a transfer function's selection) is decided once for all.
+template <int NP>
+__device__ __forceinline__ void apply_op(const CspOp &op, float *a, float *b, float *c)
 {
 	switch (op.kind) {
-	case CSP_MATRIX: {
-		const float x = op.m[0] * a + op.m[1] * b + op.m[2] * c;
-		const float y = op.m[3] * a + op.m[4] * b + op.m[5] * c;
-		const float z = op.m[6] * a + op.m[7] * b + op.m[8] * c;
-		a = x; b = y; c = z;
+	case CSP_MATRIX:
+#pragma unroll
+		for (int p = 0; p < NP; ++p) {
+			const float x = op.m[0] * a[p] + op.m[1] * b[p] + op.m[2] * c[p];
+			const float y = op.m[3] * a[p] + op.m[4] * b[p] + op.m[5] * c[p];
+			const float z = op.m[6] * a[p] + op.m[7] * b[p] + op.m[8] * c[p];
+			a[p] = x; b[p] = y; c[p] = z;
+		}
+		break;
+	case CSP_GAMMA: {
+		float x[3 * NP];
+#pragma unroll
+		for (int p = 0; p < NP; ++p) {
+			x[p] = a[p] * op.pre;
+			x[NP + p] = b[p] * op.pre;
+			x[2 * NP + p] = c[p] * op.pre;
+		}
+		zm::transfer_n<3 * NP>(op.func, x);
+#pragma unroll
+		for (int p = 0; p < NP; ++p) {
+			a[p] = op.post * x[p];
+			b[p] = op.post * x[NP + p];
+			c[p] = op.post * x[2 * NP + p];
+		}
 		break;
 	}
-	case CSP_GAMMA:
-		a = op.post * zm::transfer(op.func, a * op.pre);
-		b = op.post * zm::transfer(op.func, b * op.pre);
-		c = op.post * zm::transfer(op.func, c * op.pre);
+	case CSP_B67:
+#pragma unroll
+		for (int p = 0; p < NP; ++p) {
+			const float gamma = 1.2f;
+			const float r = a[p] * op.pre, g = b[p] * op.pre, bl = c[p] * op.pre;
+			const float yd = zm::maxf(op.kr * r + op.kg * g + op.kb * bl, zm::FLT_MIN_);
+			const float ys_inv = zm::powf_(yd, (1.0f - gamma) / gamma);
+			a[p] = zm::arib_b67_oetf(r * ys_inv);
+			b[p] = zm::arib_b67_oetf(g * ys_inv);
+			c[p] = zm::arib_b67_oetf(bl * ys_inv);
+		}
 		break;
-	case CSP_B67: {
-		const float gamma = 1.2f;
-		const float r = a * op.pre, g = b * op.pre, bl = c * op.pre;
-		const float yd = zm::maxf(op.kr * r + op.kg * g + op.kb * bl, zm::FLT_MIN_);
-		const float ys_inv = zm::powf_(yd, (1.0f - gamma) / gamma);
-		a = zm::arib_b67_oetf(r * ys_inv);
-		b = zm::arib_b67_oetf(g * ys_inv);
-		c = zm::arib_b67_oetf(bl * ys_inv);
+	case CSP_B67_INV:
+#pragma unroll
+		for (int p = 0; p < NP; ++p) {
+			const float gamma = 1.2f;
+			float r = zm::arib_b67_inverse_oetf(a[p]), g = zm::arib_b67_inverse_oetf(b[p]), bl = zm::arib_b67_inverse_oetf(c[p]);
+			float ys = zm::maxf(op.kr * r + op.kg * g + op.kb * bl, zm::FLT_MIN_);
+			ys = zm::powf_(ys, gamma - 1.0f);
+			r = r * ys; g = g * ys; bl = bl * ys;
+			a[p] = r * op.post; b[p] = g * op.post; c[p] = bl * op.post;
+		}
 		break;
-	}
-	case CSP_B67_INV: {
-		const float gamma = 1.2f;
-		float r = zm::arib_b67_inverse_oetf(a), g = zm::arib_b67_inverse_oetf(b), bl = zm::arib_b67_inverse_oetf(c);
-		float ys = zm::maxf(op.kr * r + op.kg * g + op.kb * bl, zm::FLT_MIN_);
-		ys = zm::powf_(ys, gamma - 1.0f);
-		r = r * ys; g = g * ys; bl = bl * ys;
-		a = r * op.post; b = g * op.post; c = bl * op.post;
-		break;
-	}
 	case CSP_CL_TO_RGB: {
-		const float y = a, u = b, v = c;
-		const float bmy = u < 0 ? u * 2.0f * op.nb : u * 2.0f * op.pb;
-		const float rmy = v < 0 ? v * 2.0f * op.nr : v * 2.0f * op.pr;
-		const float bl = zm::transfer(op.func, bmy + y);
-		const float r = zm::transfer(op.func, rmy + y);
-		const float yl = zm::transfer(op.func, y);
-		const float g = (yl - op.kr * r - op.kb * bl) / op.kg;
-		a = r * op.post; b = g * op.post; c = bl * op.post;
+		float x[3 * NP];
+#pragma unroll
+		for (int p = 0; p < NP; ++p) {
+			const float y = a[p], u = b[p], v = c[p];
+			const float bmy = u < 0 ? u * 2.0f * op.nb : u * 2.0f * op.pb;
+			const float rmy = v < 0 ? v * 2.0f * op.nr : v * 2.0f * op.pr;
+			x[p] = bmy + y;
+			x[NP + p] = rmy + y;
+			x[2 * NP + p] = y;
+		}
+		zm::transfer_n<3 * NP>(op.func, x);
+#pragma unroll
+		for (int p = 0; p < NP; ++p) {
+			const float bl = x[p], r = x[NP + p], yl = x[2 * NP + p];
+			const float g = (yl - op.kr * r - op.kb * bl) / op.kg;
+			a[p] = r * op.post; b[p] = g * op.post; c[p] = bl * op.post;
+		}
 		break;
 	}
 	case CSP_CL_TO_YUV: {
-		float r = a * op.pre, g = b * op.pre, bl = c * op.pre;
-		const float y = zm::transfer(op.func, op.kr * r + op.kg * g + op.kb * bl);
-		bl = zm::transfer(op.func, bl);
-		r = zm::transfer(op.func, r);
-		const float u = (bl - y < 0.0f) ? (bl - y) / (2.0f * op.nb) : (bl - y) / (2.0f * op.pb);
-		const float v = (r - y < 0.0f) ? (r - y) / (2.0f * op.nr) : (r - y) / (2.0f * op.pr);
-		a = y; b = u; c = v;
+		float x[3 * NP];
+#pragma unroll
+		for (int p = 0; p < NP; ++p) {
+			const float r = a[p] * op.pre, g = b[p] * op.pre, bl = c[p] * op.pre;
+			x[p] = op.kr * r + op.kg * g + op.kb * bl;
+			x[NP + p] = bl;
+			x[2 * NP + p] = r;
+		}
+		zm::transfer_n<3 * NP>(op.func, x);
+#pragma unroll
+		for (int p = 0; p < NP; ++p) {
+			const float y = x[p], bl = x[NP + p], r = x[2 * NP + p];
+			const float u = (bl - y < 0.0f) ? (bl - y) / (2.0f * op.nb) : (bl - y) / (2.0f * op.pb);
+			const float v = (r - y < 0.0f) ? (r - y) / (2.0f * op.nr) : (r - y) / (2.0f * op.pr);
+			a[p] = y; b[p] = u; c[p] = v;
+		}
 		break;
 	}
 	default:
@@ -971,31 +1009,44 @@ __device__ __forceinline__ void apply_op(const CspOp &op, float &a, float &b, fl
 	}
 }
 
+// Pixels per thread of the colourspace kernel (consecutive blocks of kThreads columns).
+constexpr int kCspPix = 4;
+
 __global__ void __launch_bounds__(kThreads) k_colorspace(CspArg a)
 {
 	const unsigned f = blockIdx.z;
 	const unsigned i = blockIdx.y;
-	const unsigned j = blockIdx.x * kThreads + threadIdx.x;
-	if (j >= a.width || i >= a.height)
+	const unsigned jb = blockIdx.x * (kThreads * kCspPix) + threadIdx.x;
+	if (i >= a.height || jb >= a.width)
 		return;
-	float v[3];
+	float v[3][kCspPix];
 #pragma unroll
 	for (int p = 0; p < 3; ++p) {
 		const char *srow = row_ptr(frame_ptr(a.src[p], f), a.src[p].stride, i);
-		float x = load_as_float(srow, j, a.in[p].type);
-		if (a.in[p].op == 2)
-			x = x * a.in[p].scale + a.in[p].offset;
-		v[p] = x;
+#pragma unroll
+		for (int k = 0; k < kCspPix; ++k) {
+			const unsigned j = min(jb + k * kThreads, a.width - 1);
+			float x = load_as_float(srow, j, a.in[p].type);
+			if (a.in[p].op == 2)
+				x = x * a.in[p].scale + a.in[p].offset;
+			v[p][k] = x;
+		}
 	}
 	for (int k = 0; k < a.prog.n; ++k)
-		apply_op(a.prog.op[k], v[0], v[1], v[2]);
+		apply_op<kCspPix>(a.prog.op[k], v[0], v[1], v[2]);
 #pragma unroll
 	for (int p = 0; p < 3; ++p) {
 		char *drow = row_ptr(frame_ptr_w(a.dst[p], f), a.dst[p].stride, i);
-		if (a.out[p].op == 1)
-			store_uint(drow, j, a.out[p].type, quantize(v[p], a.out[p], i, j));
-		else
-			store_float(drow, j, a.out[p].type, v[p]);
+#pragma unroll
+		for (int k = 0; k < kCspPix; ++k) {
+			const unsigned j = jb + k * kThreads;
+			if (j >= a.width)
+				break;
+			if (a.out[p].op == 1)
+				store_uint(drow, j, a.out[p].type, quantize(v[p][k], a.out[p], i, j));
+			else
+				store_float(drow, j, a.out[p].type, v[p][k]);
+		}
 	}
 }
 
@@ -1159,7 +1210,7 @@ int launch_point(const PointArg &a, void *stream)
 
 int launch_colorspace(const CspArg &a, void *stream)
 {
-	dim3 grid((a.width + kThreads - 1) / kThreads, a.height, a.frames);
+	dim3 grid((a.width + kThreads * kCspPix - 1) / (kThreads * kCspPix), a.height, a.frames);
 	hipLaunchKernelGGL(k_colorspace, grid, dim3(kThreads), 0, static_cast<hipStream_t>(stream), a);
 	return static_cast<int>(hipGetLastError());
 }

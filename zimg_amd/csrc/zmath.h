@@ -323,69 +323,114 @@ ZHD float st2084_inverse_eotf(float x)
 ZHD float copysign_(float mag, float sgn) { return bitsf((fbits(mag) & 0x7fffffffu) | (fbits(sgn) & 0x80000000u)); }
 ZHD float fabs_(float x) { return bitsf(fbits(x) & 0x7fffffffu); }
 
+// One function per TransferFunc id (gamma.cpp:63-338 via select_transfer_function).
+ZHD float t_log100_oetf(float x) { return x <= 0.01f ? 0.0f : 1.0f + log10f_(x) / 2.0f; }
+ZHD float t_log100_inv_oetf(float x) { return x <= 0.0f ? 0.01f : powf_(10.0f, 2 * (x - 1.0f)); }
+ZHD float t_log316_oetf(float x) { return x <= 0.00316227766f ? 0.0f : 1.0f + log10f_(x) / 2.5f; }
+ZHD float t_log316_inv_oetf(float x) { return x <= 0.0f ? 0.00316227766f : powf_(10.0f, 2.5f * (x - 1.0f)); }
+ZHD float t_470m_oetf(float x) { return x < 0.0f ? 0.0f : powf_(x, 2.2f); }
+ZHD float t_470m_inv_oetf(float x) { return x < 0.0f ? 0.0f : powf_(x, 1.0f / 2.2f); }
+ZHD float t_470bg_oetf(float x) { return x < 0.0f ? 0.0f : powf_(x, 2.8f); }
+ZHD float t_470bg_inv_oetf(float x) { return x < 0.0f ? 0.0f : powf_(x, 1.0f / 2.8f); }
+ZHD float t_240m_oetf(float x)
+{
+	x = maxf(x, 0.0f);
+	return x < 4.0f * SMPTE_240M_BETA ? x / 4.0f : powf_((x + (SMPTE_240M_ALPHA - 1.0f)) / SMPTE_240M_ALPHA, 1.0f / 0.45f);
+}
+ZHD float t_240m_inv_oetf(float x)
+{
+	x = maxf(x, 0.0f);
+	return x < SMPTE_240M_BETA ? x * 4.0f : SMPTE_240M_ALPHA * powf_(x, 0.45f) - (SMPTE_240M_ALPHA - 1.0f);
+}
+ZHD float t_xvycc_oetf(float x) { return copysign_(rec709_oetf(fabs_(x)), x); }
+ZHD float t_xvycc_inv_oetf(float x) { return copysign_(rec709_inverse_oetf(fabs_(x)), x); }
+ZHD float t_1361_oetf(float x) { return x >= 0.0f ? rec709_oetf(x) : -rec709_oetf(-x * 4.0f) / 4.0f; }
+ZHD float t_1361_inv_oetf(float x) { return x >= 0.0f ? rec709_inverse_oetf(x) : -rec709_inverse_oetf(-x * 4.0f) / 4.0f; }
+ZHD float t_1361_eotf(float x)
+{
+	if (x < 0.0f || x > 1.0f) return x >= 0.0f ? rec709_inverse_oetf(x) : -rec709_inverse_oetf(-x * 4.0f) / 4.0f;
+	return rec1886_eotf(x);
+}
+ZHD float t_1361_inv_eotf(float x)
+{
+	if (x < 0.0f || x > 1.0f) return x >= 0.0f ? rec709_oetf(x) : -rec709_oetf(-x * 4.0f) / 4.0f;
+	return rec1886_inverse_eotf(x);
+}
+ZHD float t_xvycc_eotf(float x)
+{
+	if (x < 0.0f || x > 1.0f) return copysign_(rec709_inverse_oetf(fabs_(x)), x);
+	return rec1886_eotf(x);
+}
+ZHD float t_xvycc_inv_eotf(float x)
+{
+	if (x < 0.0f || x > 1.0f) return copysign_(rec709_oetf(fabs_(x)), x);
+	return rec1886_inverse_eotf(x);
+}
+ZHD float t_srgb_eotf(float x)
+{
+	x = maxf(x, 0.0f);
+	return x < 12.92f * SRGB_BETA ? x / 12.92f : powf_((x + (SRGB_ALPHA - 1.0f)) / SRGB_ALPHA, 2.4f);
+}
+ZHD float t_srgb_inv_eotf(float x)
+{
+	x = maxf(x, 0.0f);
+	return x < SRGB_BETA ? x * 12.92f : SRGB_ALPHA * powf_(x, 1.0f / 2.4f) - (SRGB_ALPHA - 1.0f);
+}
+ZHD float t_428_eotf(float x) { return x < 0.0f ? 0.0f : 52.37f / 48.0f * powf_(x, 2.6f); }
+ZHD float t_428_inv_eotf(float x) { return x < 0.0f ? 0.0f : powf_(x * (48.0f / 52.37f), 1.0f / 2.6f); }
+ZHD float t_b67_eotf(float x)
+{
+	float y = arib_b67_inverse_oetf(x);
+	return y < 0.0f ? y : powf_(y, 1.2f);
+}
+ZHD float t_b67_inv_eotf(float x)
+{
+	float y = x < 0.0f ? x : powf_(x, 1.0f / 1.2f);
+	return arib_b67_oetf(y);
+}
+ZHD float t_2084_oetf(float x) { return st2084_inverse_eotf(rec1886_eotf(rec709_oetf(x * ST2084_OOTF_SCALE)) / 100.0f); }
+ZHD float t_2084_inv_oetf(float x) { return rec709_inverse_oetf(rec1886_inverse_eotf(st2084_eotf(x) * 100.0f)) / ST2084_OOTF_SCALE; }
+
+#define ZAMD_TRANSFERS(X) \
+	X(F_709_OETF, rec709_oetf) X(F_709_INV_OETF, rec709_inverse_oetf) \
+	X(F_LOG100_OETF, t_log100_oetf) X(F_LOG100_INV_OETF, t_log100_inv_oetf) \
+	X(F_LOG316_OETF, t_log316_oetf) X(F_LOG316_INV_OETF, t_log316_inv_oetf) \
+	X(F_470M_OETF, t_470m_oetf) X(F_470M_INV_OETF, t_470m_inv_oetf) \
+	X(F_470BG_OETF, t_470bg_oetf) X(F_470BG_INV_OETF, t_470bg_inv_oetf) \
+	X(F_240M_OETF, t_240m_oetf) X(F_240M_INV_OETF, t_240m_inv_oetf) \
+	X(F_XVYCC_OETF, t_xvycc_oetf) X(F_XVYCC_INV_OETF, t_xvycc_inv_oetf) \
+	X(F_1361_OETF, t_1361_oetf) X(F_1361_INV_OETF, t_1361_inv_oetf) \
+	X(F_B67_OETF, arib_b67_oetf) X(F_B67_INV_OETF, arib_b67_inverse_oetf) \
+	X(F_1886_EOTF, rec1886_eotf) X(F_1886_INV_EOTF, rec1886_inverse_eotf) \
+	X(F_1361_EOTF, t_1361_eotf) X(F_1361_INV_EOTF, t_1361_inv_eotf) \
+	X(F_XVYCC_EOTF, t_xvycc_eotf) X(F_XVYCC_INV_EOTF, t_xvycc_inv_eotf) \
+	X(F_SRGB_EOTF, t_srgb_eotf) X(F_SRGB_INV_EOTF, t_srgb_inv_eotf) \
+	X(F_2084_EOTF, st2084_eotf) X(F_2084_INV_EOTF, st2084_inverse_eotf) \
+	X(F_428_EOTF, t_428_eotf) X(F_428_INV_EOTF, t_428_inv_eotf) \
+	X(F_B67_EOTF, t_b67_eotf) X(F_B67_INV_EOTF, t_b67_inv_eotf) \
+	X(F_2084_OETF, t_2084_oetf) X(F_2084_INV_OETF, t_2084_inv_oetf)
+
 ZHD float transfer(int f, float x)
 {
 	switch (f) {
-	case F_709_OETF: return rec709_oetf(x);
-	case F_709_INV_OETF: return rec709_inverse_oetf(x);
-	case F_LOG100_OETF: return x <= 0.01f ? 0.0f : 1.0f + log10f_(x) / 2.0f;
-	case F_LOG100_INV_OETF: return x <= 0.0f ? 0.01f : powf_(10.0f, 2 * (x - 1.0f));
-	case F_LOG316_OETF: return x <= 0.00316227766f ? 0.0f : 1.0f + log10f_(x) / 2.5f;
-	case F_LOG316_INV_OETF: return x <= 0.0f ? 0.00316227766f : powf_(10.0f, 2.5f * (x - 1.0f));
-	case F_470M_OETF: return x < 0.0f ? 0.0f : powf_(x, 2.2f);
-	case F_470M_INV_OETF: return x < 0.0f ? 0.0f : powf_(x, 1.0f / 2.2f);
-	case F_470BG_OETF: return x < 0.0f ? 0.0f : powf_(x, 2.8f);
-	case F_470BG_INV_OETF: return x < 0.0f ? 0.0f : powf_(x, 1.0f / 2.8f);
-	case F_240M_OETF:
-		x = maxf(x, 0.0f);
-		return x < 4.0f * SMPTE_240M_BETA ? x / 4.0f : powf_((x + (SMPTE_240M_ALPHA - 1.0f)) / SMPTE_240M_ALPHA, 1.0f / 0.45f);
-	case F_240M_INV_OETF:
-		x = maxf(x, 0.0f);
-		return x < SMPTE_240M_BETA ? x * 4.0f : SMPTE_240M_ALPHA * powf_(x, 0.45f) - (SMPTE_240M_ALPHA - 1.0f);
-	case F_XVYCC_OETF: return copysign_(rec709_oetf(fabs_(x)), x);
-	case F_XVYCC_INV_OETF: return copysign_(rec709_inverse_oetf(fabs_(x)), x);
-	case F_1361_OETF: return x >= 0.0f ? rec709_oetf(x) : -rec709_oetf(-x * 4.0f) / 4.0f;
-	case F_1361_INV_OETF: return x >= 0.0f ? rec709_inverse_oetf(x) : -rec709_inverse_oetf(-x * 4.0f) / 4.0f;
-	case F_B67_OETF: return arib_b67_oetf(x);
-	case F_B67_INV_OETF: return arib_b67_inverse_oetf(x);
-	case F_1886_EOTF: return rec1886_eotf(x);
-	case F_1886_INV_EOTF: return rec1886_inverse_eotf(x);
-	case F_1361_EOTF:
-		if (x < 0.0f || x > 1.0f) return x >= 0.0f ? rec709_inverse_oetf(x) : -rec709_inverse_oetf(-x * 4.0f) / 4.0f;
-		return rec1886_eotf(x);
-	case F_1361_INV_EOTF:
-		if (x < 0.0f || x > 1.0f) return x >= 0.0f ? rec709_oetf(x) : -rec709_oetf(-x * 4.0f) / 4.0f;
-		return rec1886_inverse_eotf(x);
-	case F_XVYCC_EOTF:
-		if (x < 0.0f || x > 1.0f) return copysign_(rec709_inverse_oetf(fabs_(x)), x);
-		return rec1886_eotf(x);
-	case F_XVYCC_INV_EOTF:
-		if (x < 0.0f || x > 1.0f) return copysign_(rec709_oetf(fabs_(x)), x);
-		return rec1886_inverse_eotf(x);
-	case F_SRGB_EOTF:
-		x = maxf(x, 0.0f);
-		return x < 12.92f * SRGB_BETA ? x / 12.92f : powf_((x + (SRGB_ALPHA - 1.0f)) / SRGB_ALPHA, 2.4f);
-	case F_SRGB_INV_EOTF:
-		x = maxf(x, 0.0f);
-		return x < SRGB_BETA ? x * 12.92f : SRGB_ALPHA * powf_(x, 1.0f / 2.4f) - (SRGB_ALPHA - 1.0f);
-	case F_2084_EOTF: return st2084_eotf(x);
-	case F_2084_INV_EOTF: return st2084_inverse_eotf(x);
-	case F_428_EOTF: return x < 0.0f ? 0.0f : 52.37f / 48.0f * powf_(x, 2.6f);
-	case F_428_INV_EOTF: return x < 0.0f ? 0.0f : powf_(x * (48.0f / 52.37f), 1.0f / 2.6f);
-	case F_B67_EOTF: {
-		float y = arib_b67_inverse_oetf(x);
-		return y < 0.0f ? y : powf_(y, 1.2f);
-	}
-	case F_B67_INV_EOTF: {
-		float y = x < 0.0f ? x : powf_(x, 1.0f / 1.2f);
-		return arib_b67_oetf(y);
-	}
-	case F_2084_OETF:
-		return st2084_inverse_eotf(rec1886_eotf(rec709_oetf(x * ST2084_OOTF_SCALE)) / 100.0f);
-	case F_2084_INV_OETF:
-		return rec709_inverse_oetf(rec1886_inverse_eotf(st2084_eotf(x) * 100.0f)) / ST2084_OOTF_SCALE;
+#define ZAMD_CASE(id, fn) case id: return fn(x);
+	ZAMD_TRANSFERS(ZAMD_CASE)
+#undef ZAMD_CASE
 	default:
 		return bitsf(0x7fc00000u);
+	}
+}
+
+// transfer() over n values with the function selected once.
+template <int N>
+ZHD void transfer_n(int f, float *x)
+{
+	switch (f) {
+#define ZAMD_CASE(id, fn) case id: for (int i = 0; i < N; ++i) x[i] = fn(x[i]); return;
+	ZAMD_TRANSFERS(ZAMD_CASE)
+#undef ZAMD_CASE
+	default:
+		for (int i = 0; i < N; ++i) x[i] = bitsf(0x7fc00000u);
 	}
 }
 
