@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="fhd_709_to_2020", choices=sorted(WORKLOADS))
     ap.add_argument("--secondary", default="uhd_lanczos", help="second workload reported in the same line ('' for none)")
+    ap.add_argument("--precision", default="exact", choices=["exact", "fp16", "bf16"],
+                    help="FLOAT resize arithmetic: exact FP32 order, or matrix cores with FP16/BF16 hi+lo operands")
     return ap.parse_args()
 
 
@@ -156,8 +158,14 @@ def cpu_baseline(name, seconds):
             "sample": f"{n} whole {name} frames through the C oracle (tests/pipelines.py), {dt:.1f} s, 1 thread"}
 
 
-def measure(torch, name, frames, steps, warmup):
+PRECISION = {"exact": 0, "fp16": 1, "bf16": 2}
+
+
+def measure(torch, name, frames, steps, warmup, precision="exact"):
     g, planes, outs, algo_bytes, dtype = build_workload(name, frames)
+    if precision != "exact":
+        g.set_resize_precision(PRECISION[precision])
+        dtype += f" (resize on MFMA, {precision} hi+lo, FP32 accumulate)"
     launch, keep = device_batch(torch, g, planes, outs, frames)
     elapsed = time_steps(torch, launch, steps, warmup)
     per_step = elapsed / steps
@@ -180,8 +188,8 @@ def main():
         sys.exit(1)
     primary = args.workload
     secondary = args.secondary if args.secondary and args.secondary != primary else None
-    m = measure(torch, primary, args.frames, args.steps, args.warmup)
-    m2 = measure(torch, secondary, min(args.frames, 32), max(args.steps // 2, 5), args.warmup) if secondary else None
+    m = measure(torch, primary, args.frames, args.steps, args.warmup, args.precision)
+    m2 = measure(torch, secondary, min(args.frames, 32), max(args.steps // 2, 5), args.warmup, args.precision) if secondary else None
     cpu = None if args.no_cpu or primary not in ("fhd_709_to_2020", "uhd_lanczos") else cpu_baseline(primary, args.cpu_seconds)
 
     line = {
@@ -197,7 +205,7 @@ def main():
         "vs_baseline": None,
         "dtype": m["dtype"],
         "data": "synthetic (seeded, zimg_amd/workloads.py), inputs resident in HBM",
-        "config": {"workload": primary, "frames_per_step": args.frames, "graph_launches": m["launches"]},
+        "config": {"workload": primary, "frames_per_step": args.frames, "graph_launches": m["launches"], "precision": args.precision},
         "roofline": {"bound": "hbm", "achieved": round(m["achieved_gbs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(m["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": None,
                      "kernel": "whole graph per launch sequence" if m["launches"] > 1 else m["graph"][0]},

@@ -405,6 +405,23 @@ zimg_error_code_e zimg_filter_graph_process_device_batch(const zimg_filter_graph
                                                          const zimg_image_buffer *dst, const ptrdiff_t dst_frame_stride[4],
                                                          void *tmp, void *stream);
 
+/* device extension: arithmetic of FLOAT resize stages on the device.
+ * ZIMG_RESIZE_PRECISION_EXACT: the reference's FP32 evaluation order, bit for
+ * bit (default). ZIMG_RESIZE_PRECISION_MFMA_FP16 / _BF16: each pass as a
+ * banded contraction on matrix cores with FP32 accumulation, operands split
+ * into a 16-bit high part and a 16-bit remainder (FP16 requires |samples| <
+ * 65504). Integer (WORD) resizes and dithered outputs always use the exact
+ * path. Call before zimg_filter_graph_get_device_tmp_size and processing; not
+ * safe against concurrent processing of the same graph. */
+typedef enum zimg_resize_precision_e {
+	ZIMG_RESIZE_PRECISION_EXACT     = 0,
+	ZIMG_RESIZE_PRECISION_MFMA_FP16 = 1,
+	ZIMG_RESIZE_PRECISION_MFMA_BF16 = 2
+} zimg_resize_precision_e;
+
+ZIMG_VISIBILITY
+zimg_error_code_e zimg_filter_graph_set_resize_precision(zimg_filter_graph *ptr, int mode);
+
 /* device extension: a short text description of the lowered device stages
  * (the analogue of the reference's FilterObserver trace, graphbuilder.h:45-64).
  * Writes at most n bytes including the terminator; returns the full length. */

@@ -421,10 +421,23 @@ static double orc_row_get(const orc_row *r, size_t j)
 	return (j >= r->lo && j < r->hi) ? r->v[j] : 0.0;
 }
 
+int orc_compute_filter64(int type, double param_a, double param_b,
+                         unsigned src_dim, unsigned dst_dim, double shift, double width,
+                         unsigned *filter_width_out,
+                         float *data, int16_t *data_i16, double *data64, unsigned *left_out);
+
 int orc_compute_filter(int type, double param_a, double param_b,
                        unsigned src_dim, unsigned dst_dim, double shift, double width,
                        unsigned *filter_width_out,
                        float *data, int16_t *data_i16, unsigned *left_out)
+{
+	return orc_compute_filter64(type, param_a, param_b, src_dim, dst_dim, shift, width, filter_width_out, data, data_i16, 0, left_out);
+}
+
+int orc_compute_filter64(int type, double param_a, double param_b,
+                         unsigned src_dim, unsigned dst_dim, double shift, double width,
+                         unsigned *filter_width_out,
+                         float *data, int16_t *data_i16, double *data64, unsigned *left_out)
 {
 	/* filter.cpp:249-306 */
 	double scale = (double)dst_dim / width;
@@ -525,6 +538,8 @@ int orc_compute_filter(int type, double param_a, double param_b,
 
 				data[i * width_fw + j] = c_f32;
 				data_i16[i * width_fw + j] = c_i16;
+				if (data64)
+					data64[i * width_fw + j] = coeff;
 			}
 			data_i16[i * width_fw + i16_greatest_idx] = (int16_t)(data_i16[i * width_fw + i16_greatest_idx] + ((1 << 14) - i16_sum));
 			left_out[i] = (unsigned)left;

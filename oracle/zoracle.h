@@ -64,6 +64,13 @@ int orc_compute_filter(int type, double param_a, double param_b,
                        unsigned *filter_width,
                        float *data, int16_t *data_i16, unsigned *left);
 
+/* As orc_compute_filter, also returning the unrounded FP64 entries of the
+ * resampling matrix in the packed layout (data64, may be NULL). */
+int orc_compute_filter64(int type, double param_a, double param_b,
+                         unsigned src_dim, unsigned dst_dim, double shift, double width,
+                         unsigned *filter_width,
+                         float *data, int16_t *data_i16, double *data64, unsigned *left);
+
 /* ---- resize kernels (resize/resize_impl.cpp:31-114), whole planes ---- */
 void orc_resize_h_u16(const int16_t *coeffs, const unsigned *left, unsigned fw, unsigned dst_w,
                       const uint16_t *src, ptrdiff_t src_stride, uint16_t *dst, ptrdiff_t dst_stride,

@@ -91,6 +91,10 @@ def oracle():
         lib.orc_half_to_float.argtypes = [ctypes.c_uint16]
         lib.orc_float_to_half.restype = ctypes.c_uint16
         lib.orc_float_to_half.argtypes = [ctypes.c_float]
+        lib.orc_compute_filter64.restype = ctypes.c_int
+        lib.orc_compute_filter64.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint, ctypes.c_uint,
+                                             ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_uint32), c_float_p,
+                                             c_i16_p, c_double_p, c_u32_p]
         lib.orc_convert_to_half.restype = None
         lib.orc_convert_to_half.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_ssize_t, ctypes.c_void_p,
                                             ctypes.c_ssize_t, ctypes.c_uint, ctypes.c_uint, ctypes.c_float, ctypes.c_float]
@@ -180,6 +184,40 @@ def compute_filter(lib, prefix, ftype, a, b, src, dst, shift, width):
             ptr(data, c_float_p), ptr(data_i16, c_i16_p), ptr(left, c_u32_p))
     assert rc == 0
     return fw, data.reshape(dst, fw), data_i16.reshape(dst, fw), left
+
+
+def oracle_filter64(ftype, a, b, src, dst, shift=0.0, width=None):
+    """The unrounded FP64 resampling matrix (filter.cpp:249-306) as a dense
+    (dst x src) float64 array."""
+    lib = oracle()
+    width = float(src if width is None else width)
+    fw = ctypes.c_uint32(0)
+    assert lib.orc_compute_filter64(ftype, a, b, src, dst, shift, width, ctypes.byref(fw), None, None, None, None) == 0
+    fw = fw.value
+    data = np.zeros(dst * fw, np.float32)
+    data_i16 = np.zeros(dst * fw, np.int16)
+    data64 = np.zeros(dst * fw, np.float64)
+    left = np.zeros(dst, np.uint32)
+    assert lib.orc_compute_filter64(ftype, a, b, src, dst, shift, width, ctypes.byref(ctypes.c_uint32(0)), ptr(data, c_float_p),
+                                    ptr(data_i16, c_i16_p), ptr(data64, c_double_p), ptr(left, c_u32_p)) == 0
+    m = np.zeros((dst, src), np.float64)
+    rows = np.repeat(np.arange(dst), fw)
+    cols = (left[:, None].astype(np.int64) + np.arange(fw)[None, :]).ravel()
+    np.add.at(m, (rows, cols), data64)
+    return m
+
+
+def resize_plane_f64(img, ftype, a, b, dst_w, dst_h):
+    """FP64 oracle of a resize: the FP64 matrices applied in FP64 (centre siting, no shift)."""
+    src_h, src_w = img.shape
+    x = np.asarray(img, np.float64)
+    mh = oracle_filter64(ftype, a, b, src_w, dst_w) if dst_w != src_w else None
+    mv = oracle_filter64(ftype, a, b, src_h, dst_h) if dst_h != src_h else None
+    if mh is not None:
+        x = x @ mh.T
+    if mv is not None:
+        x = mv @ x
+    return x
 
 
 def oracle_filter(ftype, a, b, src, dst, shift=0.0, width=None):

@@ -11,6 +11,7 @@ import subprocess
 import pytest
 
 import zimg_amd as z
+from zimg_amd import workloads as W
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -213,3 +214,15 @@ def test_tiled_resize_configurations_in_bounds():
             s.active_region.width, s.active_region.height = sw * rng.uniform(0.5, 1.0), sh * rng.uniform(0.5, 1.0)
         p = z.builder_params(resample_filter=filt, filter_param_a=rng.choice([2.0, 3.0, 4.0]) if filt == z.RESIZE_LANCZOS else float("nan"))
         z.FilterGraph(s, d, p).describe()
+
+
+def test_resize_precision_setter_validates_and_sizes_scratch():
+    s, d, p = W.fp32_formats(3840, 2160, 1920, 1080, z.RESIZE_LANCZOS, 3.0)
+    g = z.FilterGraph(s, d, p)
+    base = g.device_tmp_size(2)
+    g.set_resize_precision(z.RESIZE_PRECISION_MFMA_FP16)
+    # the two-pass matrix-core form keeps its FP32 intermediate (3840 x 1080) in tmp
+    assert g.device_tmp_size(2) >= base + 2 * 3840 * 1080 * 4
+    with pytest.raises(z.ZimgError) as e:
+        g.set_resize_precision(7)
+    assert e.value.code == z.ERROR_ENUM_OUT_OF_RANGE

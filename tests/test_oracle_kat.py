@@ -344,3 +344,35 @@ def test_libm_close_to_correctly_rounded():
     got = np.array([o.orc_logf(float(v)) for v in xl], np.float32)
     want = np.log(xl.astype(np.float64)).astype(np.float32)
     assert np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32)).max() <= 1
+
+
+def test_fp64_matrix_reproduces_pinned_f32_table():
+    """The FP64 entries the precision-form oracle applies are the ones the
+    f32 table (pinned against the compiled reference) was rounded from: the
+    error-dithered rounding of filter.cpp:86-97 applied to them in numpy
+    gives the table back bit for bit; rows sum to 1 in FP64."""
+    import ctypes as C
+    from oracle_lib import c_double_p, c_float_p, c_i16_p, c_u32_p, oracle_filter64
+    for ftype, a, b, src, dst in [(LANCZOS, 3.0, 0.0, 3840, 1920), (BICUBIC, 0.0, 0.5, 1280, 3840), (SPLINE36, 0.0, 0.0, 333, 1000)]:
+        lib = oracle()
+        fw = C.c_uint32(0)
+        lib.orc_compute_filter64(ftype, a, b, src, dst, 0.0, float(src), C.byref(fw), None, None, None, None)
+        fw = fw.value
+        d32 = np.zeros(dst * fw, np.float32)
+        d16 = np.zeros(dst * fw, np.int16)
+        d64 = np.zeros(dst * fw, np.float64)
+        left = np.zeros(dst, np.uint32)
+        lib.orc_compute_filter64(ftype, a, b, src, dst, 0.0, float(src), C.byref(C.c_uint32(0)), ptr(d32, c_float_p),
+                                 ptr(d16, c_i16_p), ptr(d64, c_double_p), ptr(left, c_u32_p))
+        d64 = d64.reshape(dst, fw)
+        want = np.zeros((dst, fw), np.float32)
+        for i in range(dst):
+            err = 0.0
+            for j in range(fw):
+                e = float(d64[i, j]) - err
+                q = np.float32(e)
+                err = float(q) - e
+                want[i, j] = q
+        assert np.array_equal(want.ravel().view(np.uint32), d32.view(np.uint32))
+        m = oracle_filter64(ftype, a, b, src, dst)
+        assert np.max(np.abs(m.sum(axis=1) - 1.0)) < 1e-12

@@ -66,7 +66,10 @@ EXPORTS = [
     "zimg_image_format_default", "zimg_graph_builder_params_default", "zimg_filter_graph_build",
     "zimg_filter_graph_get_device_tmp_size", "zimg_filter_graph_process_device",
     "zimg_filter_graph_process_device_batch", "zimg_filter_graph_describe",
+    "zimg_filter_graph_set_resize_precision",
 ]
+
+RESIZE_PRECISION_EXACT, RESIZE_PRECISION_MFMA_FP16, RESIZE_PRECISION_MFMA_BF16 = 0, 1, 2
 
 
 class _Region(ctypes.Structure):
@@ -156,6 +159,8 @@ def lib():
                                                              ctypes.POINTER(ctypes.c_ssize_t), vp, vp]
         L.zimg_filter_graph_describe.restype = sz
         L.zimg_filter_graph_describe.argtypes = [vp, ctypes.c_char_p, sz]
+        L.zimg_filter_graph_set_resize_precision.restype = ctypes.c_int
+        L.zimg_filter_graph_set_resize_precision.argtypes = [vp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -252,6 +257,9 @@ class FilterGraph:
         out = ctypes.c_uint()
         check(lib().zimg_filter_graph_get_output_buffering(self._h, ctypes.byref(out)))
         return out.value
+
+    def set_resize_precision(self, mode):
+        check(lib().zimg_filter_graph_set_resize_precision(self._h, mode))
 
     def describe(self):
         n = lib().zimg_filter_graph_describe(self._h, None, 0)

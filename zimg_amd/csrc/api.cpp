@@ -670,6 +670,18 @@ zimg_error_code_e zimg_filter_graph_process_device(const zimg_filter_graph *ptr,
 	return zimg_filter_graph_process_device_batch(ptr, 1, src, nullptr, dst, nullptr, tmp, stream);
 }
 
+zimg_error_code_e zimg_filter_graph_set_resize_precision(zimg_filter_graph *ptr, int mode)
+{
+	try {
+		if (!ptr || !ptr->prog)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null graph");
+		ptr->prog->set_resize_precision(mode);
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
 size_t zimg_filter_graph_describe(const zimg_filter_graph *ptr, char *buf, size_t n)
 {
 	try {

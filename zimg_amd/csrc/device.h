@@ -67,6 +67,11 @@ public:
 	explicit DeviceProgram(const Plan &plan);
 
 	size_t tmp_size(unsigned frames) const;
+	// FLOAT resize arithmetic: 0 the reference's FP32 order (default), 1 matrix
+	// cores with FP16 hi+lo operands, 2 with BF16 hi+lo operands. Set before
+	// sizing tmp and processing; not thread-safe against concurrent processing.
+	void set_resize_precision(int mode);
+	int resize_precision() const { return m_precision; }
 	// Enqueue the program for `frames` frames. Throws Error on invalid buffers or launch failure.
 	void run(const DeviceBuffers &src, const DeviceBuffers &dst, void *tmp, unsigned frames, void *stream) const;
 	std::string describe() const;
@@ -82,8 +87,19 @@ private:
 		void *mem = nullptr;
 		std::map<int, dev::PassArg> pass; // by node id
 	};
+	struct MfmaTables {
+		void *mem = nullptr;
+		std::map<int, dev::MfmaTab> pass; // by node id
+	};
 	mutable std::mutex m_mutex;
 	mutable std::map<int, std::shared_ptr<Tables>> m_tables; // by HIP device
+	mutable std::map<int, std::shared_ptr<MfmaTables>> m_mfma_tables; // by HIP device * 4 + mode
+	int m_precision = 0;
+
+	bool mfma_step(const Step &s) const { return m_precision && s.kind == Step::RESIZE && !s.word && s.ox[0].dither == 0; }
+	size_t mfma_mid_bytes(const Step &s, unsigned frames) const;
+	const MfmaTables &mfma_tables_for_current_device() const;
+	void run_mfma_resize(const Step &s, const dev::PlaneArg &src, const dev::PlaneArg &dst, char *mid, unsigned frames, void *stream) const;
 
 	void lower();
 	void fuse();

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 
@@ -569,7 +570,181 @@ size_t DeviceProgram::tmp_size(unsigned frames) const
 	size_t total = 0;
 	for (auto &sl : m_slots)
 		total += align_up(sl.pitch * sl.height * frames, kPitchAlign);
+	for (auto &s : m_steps)
+		if (!s.dead && mfma_step(s))
+			total += mfma_mid_bytes(s, frames);
 	return total;
+}
+
+void DeviceProgram::set_resize_precision(int mode)
+{
+	if (mode < 0 || mode > 2)
+		fail(ZIMG_ERROR_ENUM_OUT_OF_RANGE, "resize precision mode");
+	m_precision = mode;
+}
+
+// FP32 intermediate of a two-pass FLOAT resize run on matrix cores.
+size_t DeviceProgram::mfma_mid_bytes(const Step &s, unsigned frames) const
+{
+	if (s.passes != 2)
+		return 0;
+	return align_up(align_up(static_cast<size_t>(s.mid_w) * 4, kPitchAlign) * s.mid_h * frames, kPitchAlign);
+}
+
+namespace {
+
+uint16_t to_half_bits(float x) { return __builtin_bit_cast(uint16_t, static_cast<_Float16>(x)); }
+float half_bits_to_float(uint16_t h) { return static_cast<float>(__builtin_bit_cast(_Float16, h)); }
+
+uint16_t to_bf16_bits(float x)
+{
+	uint32_t u;
+	std::memcpy(&u, &x, 4);
+	u += 0x7FFFu + ((u >> 16) & 1u); // round to nearest even
+	return static_cast<uint16_t>(u >> 16);
+}
+float bf16_bits_to_float(uint16_t h)
+{
+	const uint32_t u = static_cast<uint32_t>(h) << 16;
+	float x;
+	std::memcpy(&x, &u, 4);
+	return x;
+}
+
+} // namespace
+
+const DeviceProgram::MfmaTables &DeviceProgram::mfma_tables_for_current_device() const
+{
+	int device = 0;
+	check_hip(hipGetDevice(&device), "hipGetDevice");
+	const int key = device * 4 + m_precision;
+	std::lock_guard<std::mutex> lock(m_mutex);
+	auto it = m_mfma_tables.find(key);
+	if (it != m_mfma_tables.end())
+		return *it->second;
+
+	const bool bf16 = m_precision == 2;
+	struct Host {
+		int id;
+		std::vector<unsigned> off, nks;
+		std::vector<int> kb;
+		std::vector<uint16_t> coef;
+	};
+	std::vector<Host> hosts;
+	for (auto &s : m_steps) {
+		if (s.dead || !mfma_step(s))
+			continue;
+		for (int p = 0; p < s.passes; ++p) {
+			const int id = s.tab[p];
+			if (std::any_of(hosts.begin(), hosts.end(), [&](const Host &h) { return h.id == id; }))
+				continue;
+			const FilterTable &t = *m_plan.nodes[id].table;
+			Host h;
+			h.id = id;
+			const unsigned tiles = (t.filter_rows + 15) / 16;
+			unsigned blocks = 0;
+			for (unsigned tt = 0; tt < tiles; ++tt) {
+				unsigned lo = UINT_MAX, hi = 0;
+				for (unsigned o = 16 * tt; o < std::min(16 * tt + 16, t.filter_rows); ++o) {
+					lo = std::min(lo, t.left[o]);
+					hi = std::max(hi, t.left[o] + t.filter_width);
+				}
+				const int kb = static_cast<int>(lo & ~3u);
+				const unsigned nks = (hi - static_cast<unsigned>(kb) + 31) / 32;
+				h.kb.push_back(kb);
+				h.nks.push_back(nks);
+				h.off.push_back(blocks);
+				for (unsigned st = 0; st < nks; ++st) {
+					uint16_t hiv[64 * 8], lov[64 * 8];
+					for (unsigned lane = 0; lane < 64; ++lane) {
+						for (unsigned i = 0; i < 8; ++i) {
+							const unsigned o = 16 * tt + lane % 16;
+							const long k = kb + 32 * st + 8 * (lane / 16) + i;
+							double c = 0.0;
+							if (o < t.filter_rows && k >= t.left[o] && k < static_cast<long>(t.left[o] + t.filter_width))
+								c = t.data64[static_cast<size_t>(o) * t.filter_width + (k - t.left[o])];
+							const float cf = static_cast<float>(c);
+							const uint16_t hb = bf16 ? to_bf16_bits(cf) : to_half_bits(cf);
+							const double rem = c - static_cast<double>(bf16 ? bf16_bits_to_float(hb) : half_bits_to_float(hb));
+							const float rf = static_cast<float>(rem);
+							hiv[lane * 8 + i] = hb;
+							lov[lane * 8 + i] = bf16 ? to_bf16_bits(rf) : to_half_bits(rf);
+						}
+					}
+					h.coef.insert(h.coef.end(), hiv, hiv + 512);
+					h.coef.insert(h.coef.end(), lov, lov + 512);
+				}
+				blocks += nks;
+			}
+			hosts.push_back(std::move(h));
+		}
+	}
+
+	auto tab = std::make_shared<MfmaTables>();
+	size_t bytes = 0;
+	std::vector<size_t> offs;
+	for (auto &h : hosts) {
+		offs.push_back(bytes);
+		bytes += align_up(h.coef.size() * 2, 256) + align_up(h.off.size() * 4, 256) * 3;
+	}
+	if (bytes) {
+		std::vector<char> host(bytes, 0);
+		for (size_t n = 0; n < hosts.size(); ++n) {
+			const Host &h = hosts[n];
+			size_t o = offs[n];
+			std::memcpy(&host[o], h.coef.data(), h.coef.size() * 2);
+			o += align_up(h.coef.size() * 2, 256);
+			std::memcpy(&host[o], h.off.data(), h.off.size() * 4);
+			std::memcpy(&host[o + align_up(h.off.size() * 4, 256)], h.kb.data(), h.kb.size() * 4);
+			std::memcpy(&host[o + 2 * align_up(h.off.size() * 4, 256)], h.nks.data(), h.nks.size() * 4);
+		}
+		check_hip(hipMalloc(&tab->mem, bytes), "hipMalloc(matrix-core tables)");
+		check_hip(hipMemcpy(tab->mem, host.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(matrix-core tables)");
+		char *base = static_cast<char *>(tab->mem);
+		for (size_t n = 0; n < hosts.size(); ++n) {
+			const Host &h = hosts[n];
+			const size_t o = offs[n] + align_up(h.coef.size() * 2, 256);
+			const size_t a = align_up(h.off.size() * 4, 256);
+			tab->pass[h.id] = dev::MfmaTab{ reinterpret_cast<const uint16_t *>(base + offs[n]), reinterpret_cast<const unsigned *>(base + o),
+			                                reinterpret_cast<const int *>(base + o + a), reinterpret_cast<const unsigned *>(base + o + 2 * a) };
+		}
+	}
+	m_mfma_tables[key] = tab;
+	return *tab;
+}
+
+void DeviceProgram::run_mfma_resize(const Step &s, const dev::PlaneArg &src, const dev::PlaneArg &dst, char *mid, unsigned frames,
+                                    void *stream) const
+{
+	const MfmaTables &mt = mfma_tables_for_current_device();
+	const dev::InXform plain_in{ 3, 0, 0, 1.0f, 0.0f };
+	const dev::OutXform plain_out{ 3, 0, 0, 0, 32, 1.0f, 0.0f };
+	const size_t mid_pitch = align_up(static_cast<size_t>(s.mid_w) * 4, kPitchAlign);
+	const dev::PlaneArg midp{ mid, static_cast<long long>(mid_pitch), static_cast<long long>(mid_pitch * s.mid_h) };
+	unsigned w = s.in_w, h = s.in_h;
+	for (int p = 0; p < s.passes; ++p) {
+		const bool horizontal = (p == 0) == s.h_first;
+		const FilterTable &t = *m_plan.nodes[s.tab[p]].table;
+		dev::MfmaArg a{};
+		a.src = p == 0 ? src : midp;
+		a.dst = p == s.passes - 1 ? dst : midp;
+		a.frames = frames;
+		a.horizontal = horizontal ? 1 : 0;
+		a.bf16 = m_precision == 2 ? 1 : 0;
+		a.src_len = horizontal ? w : h;
+		a.dst_len = t.filter_rows;
+		a.rows = h;
+		a.cols = w;
+		a.tab = mt.pass.at(s.tab[p]);
+		a.in = p == 0 ? s.ix[0] : plain_in;
+		a.out = p == s.passes - 1 ? s.ox[0] : plain_out;
+		a.vec16 = (reinterpret_cast<uintptr_t>(a.src.ptr) % 16 == 0 && a.src.stride % 16 == 0 && a.src.fstride % 16 == 0) ? 1 : 0;
+		check_hip(static_cast<hipError_t>(dev::launch_mfma_resize(a, stream)), "matrix-core resize launch");
+		if (horizontal)
+			w = t.filter_rows;
+		else
+			h = t.filter_rows;
+	}
 }
 
 const DeviceProgram::Tables &DeviceProgram::tables_for_current_device() const
@@ -659,6 +834,13 @@ void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void
 		slot_off[k] = off;
 		off += align_up(m_slots[k].pitch * m_slots[k].height * frames, kPitchAlign);
 	}
+	std::vector<size_t> mfma_off(m_steps.size(), 0);
+	for (size_t k = 0; k < m_steps.size(); ++k) {
+		if (!m_steps[k].dead && mfma_step(m_steps[k])) {
+			mfma_off[k] = off;
+			off += mfma_mid_bytes(m_steps[k], frames);
+		}
+	}
 	if (off && !tmp)
 		fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "device tmp buffer required");
 
@@ -685,8 +867,13 @@ void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void
 		return p;
 	};
 
-	for (const Step &s : m_steps) {
+	for (size_t si = 0; si < m_steps.size(); ++si) {
+		const Step &s = m_steps[si];
 		int rc = 0;
+		if (!s.dead && mfma_step(s)) {
+			run_mfma_resize(s, resolve(s.in[0]), resolve(s.out[0]), static_cast<char *>(tmp) + mfma_off[si], frames, stream);
+			continue;
+		}
 		switch (s.kind) {
 		case Step::RESIZE: {
 			dev::ResizeArg a{};

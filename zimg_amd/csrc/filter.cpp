@@ -200,6 +200,7 @@ FilterTable compute_filter(const ResampleFilter &f, unsigned src_dim, unsigned d
 	t.input_width = src_dim;
 	t.data.resize(fw * dst_dim);
 	t.data_i16.resize(fw * dst_dim);
+	t.data64.resize(fw * dst_dim);
 	t.left.resize(dst_dim);
 
 	for (unsigned i = 0; i < dst_dim; ++i) {
@@ -227,6 +228,7 @@ FilterTable compute_filter(const ResampleFilter &f, unsigned src_dim, unsigned d
 			sum_i16 = static_cast<int16_t>(sum_i16 + q_i16);
 			drow[j] = q_f32;
 			irow[j] = q_i16;
+			t.data64[i * fw + j] = c;
 		}
 		// Integer taps sum to exactly 1.0 in Q1.14: the residue goes to the largest tap.
 		irow[peak_idx] = static_cast<int16_t>(irow[peak_idx] + ((1 << 14) - sum_i16));

@@ -27,6 +27,7 @@ struct FilterTable {
 	unsigned input_width = 0;
 	std::vector<float> data;       // [rows * filter_width]
 	std::vector<int16_t> data_i16; // Q1.14, [rows * filter_width]
+	std::vector<double> data64;    // the unrounded FP64 matrix entries, same layout
 	std::vector<unsigned> left;    // [rows]
 
 	bool sorted() const;

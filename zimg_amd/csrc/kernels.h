@@ -101,11 +101,38 @@ struct EdArg {
 	float scale[4], offset[4];
 };
 
+// Banded-contraction tables of one FLOAT resize pass for the matrix-core
+// precision form: outputs in tiles of 16; tile t reads inputs kb[t] ..
+// kb[t] + 32 nks[t] - 1; its coefficient blocks start at block off[t]; a
+// block is one 32-wide k step: 64 lanes x 8 high 16-bit values, then 64 x 8
+// low values (lane l: output 16 t + l % 16, inputs kb + 32 s + 8 (l / 16) + i).
+struct MfmaTab {
+	const uint16_t *coef;
+	const unsigned *off;
+	const int *kb;
+	const unsigned *nks;
+};
+
+struct MfmaArg {
+	PlaneArg src, dst;
+	unsigned frames;
+	int horizontal;     // 1: along rows (rows x src_len -> rows x dst_len), 0: along columns
+	int bf16;           // 0: FP16 hi+lo, 1: BF16 hi+lo
+	unsigned src_len, dst_len;
+	unsigned rows;      // horizontal pass: rows processed
+	unsigned cols;      // vertical pass: columns processed
+	int vec16;          // 1: source rows are 16-byte aligned (FLOAT rows may be read as float4)
+	MfmaTab tab;
+	InXform in;
+	OutXform out;
+};
+
 // Launchers. Return hipSuccess (0) or the HIP error of the launch.
 int launch_resize(const ResizeArg &a, void *stream);
 int launch_point(const PointArg &a, void *stream);
 int launch_colorspace(const CspArg &a, void *stream);
 int launch_error_diffusion(const EdArg &a, void *stream);
+int launch_mfma_resize(const MfmaArg &a, void *stream);
 
 // LDS bytes the resize kernel needs for a tile configuration (host-side sizing).
 size_t resize_lds_bytes(const ResizeArg &a);
