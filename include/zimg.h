@@ -422,6 +422,38 @@ typedef enum zimg_resize_precision_e {
 ZIMG_VISIBILITY
 zimg_error_code_e zimg_filter_graph_set_resize_precision(zimg_filter_graph *ptr, int mode);
 
+/* device extension: batched jobs. `count` independent graphs, job k running
+ * graphs[k] on src[k] -> dst[k] (device buffers as above, one frame each),
+ * are planned once into one persistent launch over the resize tiles of all
+ * jobs whose stages are all tiled integer resizes (the others run their own
+ * launch sequences first), captured into a HIP graph at creation and
+ * replayed by zimg_device_jobs_run. Buffers are fixed at creation. tmp holds
+ * zimg_device_jobs_get_tmp_size bytes (may be NULL when that is 0).
+ * ZIMG_DEVICE_JOBS_AUTOTUNE times the cost model's best tile shapes for each
+ * distinct stage shape on the given planes at creation (results are cached
+ * per process); ZIMG_DEVICE_JOBS_NO_GRAPH skips the capture (each run
+ * enqueues the launches). `stream` at creation is used for autotuning. */
+typedef struct zimg_device_jobs zimg_device_jobs;
+
+#define ZIMG_DEVICE_JOBS_AUTOTUNE 1u
+#define ZIMG_DEVICE_JOBS_NO_GRAPH 2u
+
+ZIMG_VISIBILITY
+zimg_error_code_e zimg_device_jobs_get_tmp_size(const zimg_filter_graph *const *graphs, unsigned count, size_t *out);
+
+ZIMG_VISIBILITY
+zimg_device_jobs *zimg_device_jobs_create(const zimg_filter_graph *const *graphs, unsigned count, const zimg_image_buffer_const *src,
+                                          const zimg_image_buffer *dst, void *tmp, unsigned flags, void *stream);
+
+ZIMG_VISIBILITY
+zimg_error_code_e zimg_device_jobs_run(zimg_device_jobs *jobs, void *stream);
+
+ZIMG_VISIBILITY
+size_t zimg_device_jobs_describe(const zimg_device_jobs *jobs, char *buf, size_t n);
+
+ZIMG_VISIBILITY
+void zimg_device_jobs_free(zimg_device_jobs *jobs);
+
 /* device extension: a short text description of the lowered device stages
  * (the analogue of the reference's FilterObserver trace, graphbuilder.h:45-64).
  * Writes at most n bytes including the terminator; returns the full length. */

@@ -226,3 +226,16 @@ def test_resize_precision_setter_validates_and_sizes_scratch():
     with pytest.raises(z.ZimgError) as e:
         g.set_resize_precision(7)
     assert e.value.code == z.ERROR_ENUM_OUT_OF_RANGE
+
+
+def test_batch_job_list_fixture_and_jobs_tmp_size():
+    """The seeded 64-job list (committed fixture) regenerates exactly; every job
+    lowers to one tiled integer resize per plane, so the batch needs no tmp."""
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "batch64_jobs.json")))
+    jobs = W.batch_jobs(64, fx["seed"])
+    assert jobs == fx["jobs"]
+    graphs = [z.FilterGraph(*W.batch_job_graph(j)) for j in jobs]
+    for g in graphs:
+        dev = g.describe().split("device:\n")[1].splitlines()
+        assert len(dev) == 3 and all(l.startswith("resize") and "tiled=" in l and "tiled=0" not in l for l in dev), dev
+    assert z.device_jobs_tmp_size(graphs) == 0

@@ -67,7 +67,11 @@ EXPORTS = [
     "zimg_filter_graph_get_device_tmp_size", "zimg_filter_graph_process_device",
     "zimg_filter_graph_process_device_batch", "zimg_filter_graph_describe",
     "zimg_filter_graph_set_resize_precision",
+    "zimg_device_jobs_get_tmp_size", "zimg_device_jobs_create", "zimg_device_jobs_run", "zimg_device_jobs_describe",
+    "zimg_device_jobs_free",
 ]
+
+DEVICE_JOBS_AUTOTUNE, DEVICE_JOBS_NO_GRAPH = 1, 2
 
 RESIZE_PRECISION_EXACT, RESIZE_PRECISION_MFMA_FP16, RESIZE_PRECISION_MFMA_BF16 = 0, 1, 2
 
@@ -161,6 +165,17 @@ def lib():
         L.zimg_filter_graph_describe.argtypes = [vp, ctypes.c_char_p, sz]
         L.zimg_filter_graph_set_resize_precision.restype = ctypes.c_int
         L.zimg_filter_graph_set_resize_precision.argtypes = [vp, ctypes.c_int]
+        L.zimg_device_jobs_get_tmp_size.restype = ctypes.c_int
+        L.zimg_device_jobs_get_tmp_size.argtypes = [ctypes.POINTER(vp), ctypes.c_uint, ctypes.POINTER(sz)]
+        L.zimg_device_jobs_create.restype = vp
+        L.zimg_device_jobs_create.argtypes = [ctypes.POINTER(vp), ctypes.c_uint, ctypes.POINTER(ImageBufferStruct),
+                                              ctypes.POINTER(ImageBufferStruct), vp, ctypes.c_uint, vp]
+        L.zimg_device_jobs_run.restype = ctypes.c_int
+        L.zimg_device_jobs_run.argtypes = [vp, vp]
+        L.zimg_device_jobs_describe.restype = sz
+        L.zimg_device_jobs_describe.argtypes = [vp, ctypes.c_char_p, sz]
+        L.zimg_device_jobs_free.restype = None
+        L.zimg_device_jobs_free.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -285,3 +300,41 @@ class FilterGraph:
         ss = (ctypes.c_ssize_t * 4)(*src_frame_stride)
         ds = (ctypes.c_ssize_t * 4)(*dst_frame_stride)
         check(lib().zimg_filter_graph_process_device_batch(self._h, count, ctypes.byref(src), ss, ctypes.byref(dst), ds, tmp, stream))
+
+
+def device_jobs_tmp_size(graphs):
+    arr = (ctypes.c_void_p * len(graphs))(*[g.handle for g in graphs])
+    out = ctypes.c_size_t()
+    check(lib().zimg_device_jobs_get_tmp_size(arr, len(graphs), ctypes.byref(out)))
+    return out.value
+
+
+class DeviceJobs:
+    """Batched jobs (zimg_device_jobs_*): graphs[k] on srcs[k] -> dsts[k]
+    (image_buffer structs of device pointers), fixed at creation."""
+
+    def __init__(self, graphs, srcs, dsts, tmp=None, flags=0, stream=None):
+        self._graphs = list(graphs)  # keep the graphs alive
+        n = len(self._graphs)
+        arr = (ctypes.c_void_p * n)(*[g.handle for g in self._graphs])
+        self._src = (ImageBufferStruct * n)(*srcs)
+        self._dst = (ImageBufferStruct * n)(*dsts)
+        self._h = lib().zimg_device_jobs_create(arr, n, self._src, self._dst, tmp, flags, stream)
+        if not self._h:
+            code, msg = last_error()
+            raise ZimgError(code, msg)
+
+    def run(self, stream=None):
+        check(lib().zimg_device_jobs_run(self._h, stream))
+
+    def describe(self):
+        n = lib().zimg_device_jobs_describe(self._h, None, 0)
+        buf = ctypes.create_string_buffer(n + 1)
+        lib().zimg_device_jobs_describe(self._h, buf, n + 1)
+        return buf.value.decode()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.zimg_device_jobs_free(h)
+            self._h = None

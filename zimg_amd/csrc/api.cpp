@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "device.h"
+#include "jobs.h"
 #include "graph.h"
 #include "zimg.h"
 
@@ -680,6 +681,70 @@ zimg_error_code_e zimg_filter_graph_set_resize_precision(zimg_filter_graph *ptr,
 	} catch (...) {
 		return handle_current_exception();
 	}
+}
+
+zimg_error_code_e zimg_device_jobs_get_tmp_size(const zimg_filter_graph *const *graphs, unsigned count, size_t *out)
+{
+	try {
+		if (!out || (count && !graphs))
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null argument");
+		std::vector<const DeviceProgram *> progs;
+		for (unsigned k = 0; k < count; ++k)
+			progs.push_back(&program(graphs[k]));
+		*out = zamd::DeviceJobs::tmp_size(progs);
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
+zimg_device_jobs *zimg_device_jobs_create(const zimg_filter_graph *const *graphs, unsigned count, const zimg_image_buffer_const *src,
+                                          const zimg_image_buffer *dst, void *tmp, unsigned flags, void *stream)
+{
+	try {
+		if (count && (!graphs || !src || !dst))
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null argument");
+		std::vector<zamd::DeviceJobs::Job> jobs;
+		for (unsigned k = 0; k < count; ++k) {
+			const DeviceProgram &prog = program(graphs[k]);
+			const Plan &plan = prog.plan();
+			jobs.push_back(zamd::DeviceJobs::Job{ &prog, import_device_buffer(&src[k], nullptr, plan.src_w, true),
+			                                      import_device_buffer(&dst[k], nullptr, plan.dst_w, false) });
+		}
+		return reinterpret_cast<zimg_device_jobs *>(new zamd::DeviceJobs(std::move(jobs), tmp, flags, stream));
+	} catch (...) {
+		handle_current_exception();
+		return nullptr;
+	}
+}
+
+zimg_error_code_e zimg_device_jobs_run(zimg_device_jobs *jobs, void *stream)
+{
+	try {
+		if (!jobs)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null jobs");
+		reinterpret_cast<zamd::DeviceJobs *>(jobs)->run(stream);
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
+size_t zimg_device_jobs_describe(const zimg_device_jobs *jobs, char *buf, size_t n)
+{
+	if (!jobs)
+		return 0;
+	const std::string s = reinterpret_cast<const zamd::DeviceJobs *>(jobs)->describe();
+	if (buf && n) {
+		std::strncpy(buf, s.c_str(), n);
+		buf[n - 1] = '\0';
+	}
+	return s.size();
+}
+
+void zimg_device_jobs_free(zimg_device_jobs *jobs)
+{
+	delete reinterpret_cast<zamd::DeviceJobs *>(jobs);
 }
 
 size_t zimg_filter_graph_describe(const zimg_filter_graph *ptr, char *buf, size_t n)

@@ -21,6 +21,13 @@ struct DeviceBuffers {
 	long long fstride[4] = {};
 };
 
+// One launch configuration of the tiled resize kernel.
+struct TiledConfig {
+	int fwh = 0;
+	unsigned tile_w = 0, tile_h = 0, in_rows = 0, in_cols = 0, mid_rows = 0, mid_cols = 0;
+	double cost = 0;
+};
+
 class DeviceProgram {
 public:
 	struct Buf {
@@ -43,9 +50,10 @@ public:
 		int pixel_max = 65535;
 		unsigned mid_w = 0, mid_h = 0;
 		unsigned tile_w = 0, tile_h = 0, lds_in_rows = 0, lds_in_cols = 0, lds_mid_rows = 0, lds_mid_cols = 0;
-		// fast kernel configuration (fast_fwh == 0: not eligible)
-		int fast_fwh = 0;
-		unsigned fast_tile_w = 0, fast_tile_h = 0, fast_in_rows = 0, fast_in_cols = 0, fast_mid_rows = 0, fast_mid_cols = 0;
+		// tiled kernel configuration (fast.fwh == 0: not eligible) and the
+		// best candidates by the cost model, cheapest first (for the autotuner)
+		TiledConfig fast{};
+		std::vector<TiledConfig> fast_options;
 		// POINT
 		unsigned crop_left = 0, crop_top = 0;
 		int fill = 0;
@@ -78,15 +86,26 @@ public:
 
 	const Plan &plan() const { return m_plan; }
 
+	struct Tables {
+		void *mem = nullptr;
+		std::map<int, dev::PassArg> pass; // by node id
+	};
+
+	// Steps and per-device tables, for launches batched across programs.
+	const std::vector<Step> &steps() const { return m_steps; }
+	const Tables &tables_for_current_device() const;
+	// Kernel arguments of a RESIZE step on given planes. cfg selects a tiled
+	// configuration; nullptr takes the step's chosen one when the step and
+	// the buffers allow it, else the generic kernel.
+	dev::ResizeArg resize_arg(const Step &s, const dev::PlaneArg &src, const dev::PlaneArg &dst, unsigned frames, const Tables &tabs,
+	                          const TiledConfig *cfg = nullptr) const;
+	static bool aligned16(const dev::PlaneArg &p);
+
 private:
 	Plan m_plan;
 	std::vector<Step> m_steps;
 	std::vector<Slot> m_slots;
 
-	struct Tables {
-		void *mem = nullptr;
-		std::map<int, dev::PassArg> pass; // by node id
-	};
 	struct MfmaTables {
 		void *mem = nullptr;
 		std::map<int, dev::MfmaTab> pass; // by node id
@@ -104,8 +123,7 @@ private:
 	void lower();
 	void fuse();
 	void configure_resize(Step &s);
-	void check_tiled(const Step &s, const FilterTable *th_tab, const FilterTable *tv_tab) const;
-	const Tables &tables_for_current_device() const;
+	void check_tiled(const Step &s, const TiledConfig &c, const FilterTable *th_tab, const FilterTable *tv_tab) const;
 };
 
 } // namespace zamd

@@ -362,10 +362,10 @@ unsigned pair_count(const FilterTable &t) { return t.filter_width / 2 + 1; }
 // Every LDS and table index the tiled kernel forms for this configuration,
 // recomputed on the host with the kernel's formulas: the staged windows fit
 // the LDS arrays and every tap window lies inside its staged window.
-void DeviceProgram::check_tiled(const Step &s, const FilterTable *th_tab, const FilterTable *tv_tab) const
+void DeviceProgram::check_tiled(const Step &s, const TiledConfig &c, const FilterTable *th_tab, const FilterTable *tv_tab) const
 {
 	auto bad = [](const char *what) { fail(ZIMG_ERROR_UNKNOWN, std::string("tiled resize configuration: ") + what); };
-	const unsigned tw = s.fast_tile_w, th = s.fast_tile_h;
+	const unsigned tw = c.tile_w, th = c.tile_h;
 	if (s.h_first && tw != dev::kFastTileW)
 		bad("tile width");
 	if (!s.h_first && !th_tab && tw % 16)
@@ -376,17 +376,17 @@ void DeviceProgram::check_tiled(const Step &s, const FilterTable *th_tab, const 
 		unsigned cb = j0, nc = (nc_out + 15) & ~15u;
 		if (th_tab) {
 			cb = (w ? pair_left(*th_tab, j0) : th_tab->left[j0]) & ~15u;
-			nc = ((w ? pair_left(*th_tab, jl) + 2 * static_cast<unsigned>(s.fast_fwh) : th_tab->left[jl] + th_tab->filter_width) - cb + 15) & ~15u;
+			nc = ((w ? pair_left(*th_tab, jl) + 2 * static_cast<unsigned>(c.fwh) : th_tab->left[jl] + th_tab->filter_width) - cb + 15) & ~15u;
 			for (unsigned j = j0; j <= jl; ++j) {
 				const unsigned lo = w ? pair_left(*th_tab, j) : th_tab->left[j];
-				const unsigned width = w ? 2 * static_cast<unsigned>(s.fast_fwh) : th_tab->filter_width;
+				const unsigned width = w ? 2 * static_cast<unsigned>(c.fwh) : th_tab->filter_width;
 				if (lo < cb || lo - cb + width > nc || (w && (lo - cb) % 2))
 					bad("horizontal window");
 			}
-			if (static_cast<int>(w ? pair_count(*th_tab) : th_tab->filter_width) > s.fast_fwh)
+			if (static_cast<int>(w ? pair_count(*th_tab) : th_tab->filter_width) > c.fwh)
 				bad("horizontal register capacity");
 		}
-		if (nc > s.fast_in_cols || (th_tab && !s.h_first && nc > s.fast_mid_cols) || cb % 16 || nc % 16)
+		if (nc > c.in_cols || (th_tab && !s.h_first && nc > c.mid_cols) || cb % 16 || nc % 16)
 			bad("staged columns");
 	}
 	for (unsigned i0 = 0; i0 < s.height; i0 += th) {
@@ -404,10 +404,10 @@ void DeviceProgram::check_tiled(const Step &s, const FilterTable *th_tab, const 
 			if (pairs && (span % 2 || r0 % 2))
 				bad("row pairs");
 		}
-		if (span > s.fast_in_rows || (s.h_first && tv_tab && span > s.fast_mid_rows) || (!s.h_first && th_tab && nr > s.fast_mid_rows))
+		if (span > c.in_rows || (s.h_first && tv_tab && span > c.mid_rows) || (!s.h_first && th_tab && nr > c.mid_rows))
 			bad("staged rows");
 	}
-	if (s.h_first && tv_tab && s.fast_mid_cols < dev::kFastTileW)
+	if (s.h_first && tv_tab && c.mid_cols < dev::kFastTileW)
 		bad("intermediate width");
 }
 
@@ -443,7 +443,6 @@ void DeviceProgram::configure_resize(Step &s)
 		// Per-row vertical taps held in LDS for a tile row (start row + taps).
 		const unsigned vtaps = (s.h_first && !t2) ? 0 : (s.word && s.h_first ? pair_count(*tv_tab) : tv_tab->filter_width);
 		auto ceil_div = [](size_t a, size_t b) { return (a + b - 1) / b; };
-		double best = 0;
 		std::vector<unsigned> widths = s.h_first ? std::vector<unsigned>{ dev::kFastTileW } : std::vector<unsigned>{ 512, 256, 128, 64, 32, 16 };
 		for (unsigned tw : widths) {
 			for (unsigned th : { 64u, 32u, 16u, 8u, 4u, 2u, 1u }) {
@@ -493,20 +492,25 @@ void DeviceProgram::configure_resize(Step &s)
 				}
 				const size_t per_cu = std::min<size_t>(4, (160 * 1024) / std::max<size_t>(bytes, 1));
 				const double cost = static_cast<double>(blocks) * (stage + work) / static_cast<double>(std::max<size_t>(per_cu, 1));
-				if (best == 0 || cost < best) {
-					best = cost;
-					s.fast_fwh = fwh;
-					s.fast_tile_w = tw;
-					s.fast_tile_h = th;
-					s.fast_in_rows = in_rows;
-					s.fast_in_cols = in_cols;
-					s.fast_mid_rows = mid_rows;
-					s.fast_mid_cols = mid_cols;
-				}
+				TiledConfig c;
+				c.fwh = fwh;
+				c.tile_w = tw;
+				c.tile_h = th;
+				c.in_rows = in_rows;
+				c.in_cols = in_cols;
+				c.mid_rows = mid_rows;
+				c.mid_cols = mid_cols;
+				c.cost = cost;
+				s.fast_options.push_back(c);
 			}
 		}
-		if (s.fast_fwh)
-			check_tiled(s, th_tab, tv_tab);
+		std::stable_sort(s.fast_options.begin(), s.fast_options.end(), [](const TiledConfig &x, const TiledConfig &y) { return x.cost < y.cost; });
+		if (s.fast_options.size() > 6)
+			s.fast_options.resize(6);
+		for (const TiledConfig &c : s.fast_options)
+			check_tiled(s, c, th_tab, tv_tab);
+		if (!s.fast_options.empty())
+			s.fast = s.fast_options.front();
 	}
 
 	static const unsigned kWidths[] = { 256, 128, 64, 32, 16, 8, 4, 2, 1 };
@@ -820,6 +824,55 @@ const DeviceProgram::Tables &DeviceProgram::tables_for_current_device() const
 	return *tab;
 }
 
+bool DeviceProgram::aligned16(const dev::PlaneArg &p)
+{
+	return (reinterpret_cast<uintptr_t>(p.ptr) % 16) == 0 && (p.stride % 16) == 0 && (p.fstride % 16) == 0;
+}
+
+dev::ResizeArg DeviceProgram::resize_arg(const Step &s, const dev::PlaneArg &src, const dev::PlaneArg &dst, unsigned frames,
+                                         const Tables &tabs, const TiledConfig *cfg) const
+{
+	dev::ResizeArg a{};
+	a.src = src;
+	a.dst = dst;
+	a.frames = frames;
+	a.src_w = s.in_w;
+	a.src_h = s.in_h;
+	a.mid_w = s.mid_w;
+	a.mid_h = s.mid_h;
+	a.dst_w = s.width;
+	a.dst_h = s.height;
+	a.h_first = s.h_first ? 1 : 0;
+	a.passes = s.passes;
+	a.p1 = tabs.pass.at(s.tab[0]);
+	if (s.passes == 2)
+		a.p2 = tabs.pass.at(s.tab[1]);
+	a.word = s.word ? 1 : 0;
+	a.pixel_max = s.pixel_max;
+	a.in = s.ix[0];
+	a.out = s.ox[0];
+	if (!cfg && s.fast.fwh && aligned16(a.src) && aligned16(a.dst))
+		cfg = &s.fast;
+	if (cfg) {
+		a.fwh = cfg->fwh;
+		a.tile_w = cfg->tile_w;
+		a.tile_h = cfg->tile_h;
+		a.lds_in_rows = cfg->in_rows;
+		a.lds_in_cols = cfg->in_cols;
+		a.lds_mid_rows = cfg->mid_rows;
+		a.lds_mid_cols = cfg->mid_cols;
+	} else {
+		a.fwh = 0;
+		a.tile_w = s.tile_w;
+		a.tile_h = s.tile_h;
+		a.lds_in_rows = s.lds_in_rows;
+		a.lds_in_cols = s.lds_in_cols;
+		a.lds_mid_rows = s.lds_mid_rows;
+		a.lds_mid_cols = s.lds_mid_cols;
+	}
+	return a;
+}
+
 void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void *tmp, unsigned frames, void *stream) const
 {
 	if (!frames)
@@ -876,45 +929,7 @@ void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void
 		}
 		switch (s.kind) {
 		case Step::RESIZE: {
-			dev::ResizeArg a{};
-			a.src = resolve(s.in[0]);
-			a.dst = resolve(s.out[0]);
-			a.frames = frames;
-			a.src_w = s.in_w;
-			a.src_h = s.in_h;
-			a.mid_w = s.mid_w;
-			a.mid_h = s.mid_h;
-			a.dst_w = s.width;
-			a.dst_h = s.height;
-			a.h_first = s.h_first ? 1 : 0;
-			a.passes = s.passes;
-			a.p1 = tabs.pass.at(s.tab[0]);
-			if (s.passes == 2)
-				a.p2 = tabs.pass.at(s.tab[1]);
-			a.word = s.word ? 1 : 0;
-			a.pixel_max = s.pixel_max;
-			a.in = s.ix[0];
-			a.out = s.ox[0];
-			auto aligned = [](const dev::PlaneArg &p) {
-				return (reinterpret_cast<uintptr_t>(p.ptr) % 16) == 0 && (p.stride % 16) == 0 && (p.fstride % 16) == 0;
-			};
-			if (s.fast_fwh && aligned(a.src) && aligned(a.dst)) {
-				a.fwh = s.fast_fwh;
-				a.tile_w = s.fast_tile_w;
-				a.tile_h = s.fast_tile_h;
-				a.lds_in_rows = s.fast_in_rows;
-				a.lds_in_cols = s.fast_in_cols;
-				a.lds_mid_rows = s.fast_mid_rows;
-				a.lds_mid_cols = s.fast_mid_cols;
-			} else {
-				a.fwh = 0;
-				a.tile_w = s.tile_w;
-				a.tile_h = s.tile_h;
-				a.lds_in_rows = s.lds_in_rows;
-				a.lds_in_cols = s.lds_in_cols;
-				a.lds_mid_rows = s.lds_mid_rows;
-				a.lds_mid_cols = s.lds_mid_cols;
-			}
+			const dev::ResizeArg a = resize_arg(s, resolve(s.in[0]), resolve(s.out[0]), frames, tabs);
 			rc = dev::launch_resize(a, stream);
 			break;
 		}
@@ -983,8 +998,8 @@ std::string DeviceProgram::describe() const
 			std::snprintf(buf, sizeof(buf), "%s %ux%u->%ux%u passes=%d %s tile=%ux%u lds=%zu tiled=%d(%ux%u lds=%zu): %s\n", kinds[s.kind],
 			              s.in_w, s.in_h, s.width, s.height, s.passes, s.h_first ? "h_first" : "v_first", s.tile_w, s.tile_h,
 			              (s.word ? 2 : 4) * (static_cast<size_t>(s.lds_in_rows) * s.lds_in_cols + static_cast<size_t>(s.lds_mid_rows) * s.lds_mid_cols),
-			              s.fast_fwh, s.fast_tile_w, s.fast_tile_h,
-			              (s.word ? 2 : 4) * (static_cast<size_t>(s.fast_in_rows) * s.fast_in_cols + static_cast<size_t>(s.fast_mid_rows) * s.fast_mid_cols),
+			              s.fast.fwh, s.fast.tile_w, s.fast.tile_h,
+			              (s.word ? 2 : 4) * (static_cast<size_t>(s.fast.in_rows) * s.fast.in_cols + static_cast<size_t>(s.fast.mid_rows) * s.fast.mid_cols),
 			              s.label.c_str());
 		else
 			std::snprintf(buf, sizeof(buf), "%s %ux%u: %s\n", kinds[s.kind], s.width, s.height, s.label.c_str());

@@ -1,0 +1,234 @@
+// Batched jobs: one persistent tiled-resize launch over many graphs, with a
+// timing-based tile-shape autotuner and HIP graph capture.
+#include "jobs.h"
+
+#include <algorithm>
+#include <cstring>
+
+#include "common.h"
+
+namespace zamd {
+
+namespace {
+
+void check_hip(hipError_t e, const char *what)
+{
+	if (e != hipSuccess)
+		fail(ZIMG_ERROR_UNKNOWN, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+dev::PlaneArg plane_of(const DeviceBuffers &b, int idx) { return dev::PlaneArg{ b.ptr[idx], b.stride[idx], b.fstride[idx] }; }
+
+// Tile shapes measured once per step shape and process.
+std::mutex g_tune_mutex;
+std::map<std::string, TiledConfig> g_tuned;
+
+std::string shape_key(const DeviceProgram &prog, const DeviceProgram::Step &s)
+{
+	char buf[256];
+	const auto &nodes = prog.plan().nodes;
+	const FilterTable &t1 = *nodes[s.tab[0]].table;
+	const FilterTable *t2 = s.passes == 2 ? nodes[s.tab[1]].table.get() : nullptr;
+	std::snprintf(buf, sizeof(buf), "%ux%u>%ux%u p%d h%d w%d fw%u/%u l%u/%u in%d/%d out%d/%d", s.in_w, s.in_h, s.width, s.height, s.passes,
+	              s.h_first ? 1 : 0, s.word ? 1 : 0, t1.filter_width, t2 ? t2->filter_width : 0, t1.left.empty() ? 0 : t1.left.back(),
+	              t2 && !t2->left.empty() ? t2->left.back() : 0, s.ix[0].type, s.ix[0].op, s.ox[0].type, s.ox[0].op);
+	return buf;
+}
+
+} // namespace
+
+bool DeviceJobs::eligible(const DeviceProgram &prog, const DeviceBuffers &src, const DeviceBuffers &dst)
+{
+	for (const auto &s : prog.steps()) {
+		if (s.kind != DeviceProgram::Step::RESIZE || !s.word || !s.fast.fwh || s.fast.fwh > dev::kJobsMaxTaps || prog.resize_precision())
+			return false;
+		if (s.in[0].kind != DeviceProgram::Buf::SRC || s.out[0].kind != DeviceProgram::Buf::DST)
+			return false;
+		if (!src.ptr[s.in[0].idx] || !dst.ptr[s.out[0].idx])
+			return false;
+		if (!DeviceProgram::aligned16(plane_of(src, s.in[0].idx)) || !DeviceProgram::aligned16(plane_of(dst, s.out[0].idx)))
+			return false;
+	}
+	return !prog.steps().empty();
+}
+
+size_t DeviceJobs::tmp_size(const std::vector<const DeviceProgram *> &progs)
+{
+	size_t total = 0;
+	for (const DeviceProgram *p : progs)
+		total += align_up(p->tmp_size(1), 256);
+	return total;
+}
+
+// Times the cheapest candidate shapes of the cost model on the real planes
+// (a few launches each, HIP events on `stream`) and keeps the fastest.
+TiledConfig DeviceJobs::tune(const DeviceProgram &prog, const DeviceProgram::Step &s, const dev::PlaneArg &src, const dev::PlaneArg &dst,
+                             const DeviceProgram::Tables &tabs, hipStream_t stream)
+{
+	const std::string key = shape_key(prog, s);
+	{
+		std::lock_guard<std::mutex> lock(g_tune_mutex);
+		auto it = g_tuned.find(key);
+		if (it != g_tuned.end())
+			return it->second;
+	}
+	hipEvent_t e0, e1;
+	check_hip(hipEventCreate(&e0), "hipEventCreate");
+	check_hip(hipEventCreate(&e1), "hipEventCreate");
+	TiledConfig best = s.fast;
+	float best_ms = 0;
+	const size_t ncand = std::min<size_t>(s.fast_options.size(), 4);
+	for (size_t c = 0; c < ncand; ++c) {
+		// Tiles below 8 rows or 64 columns cost more per tile in a persistent launch than they save here.
+		if (c > 0 && (s.fast_options[c].tile_h < 8 || s.fast_options[c].tile_w < 64))
+			continue;
+		const dev::ResizeArg a = prog.resize_arg(s, src, dst, 1, tabs, &s.fast_options[c]);
+		check_hip(static_cast<hipError_t>(dev::launch_resize(a, stream)), "autotune launch");
+		check_hip(hipEventRecord(e0, stream), "hipEventRecord");
+		for (int r = 0; r < 4; ++r)
+			check_hip(static_cast<hipError_t>(dev::launch_resize(a, stream)), "autotune launch");
+		check_hip(hipEventRecord(e1, stream), "hipEventRecord");
+		check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+		float ms = 0;
+		check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+		if (c == 0 || ms < best_ms) {
+			best_ms = ms;
+			best = s.fast_options[c];
+		}
+	}
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	++m_tuned;
+	std::lock_guard<std::mutex> lock(g_tune_mutex);
+	g_tuned[key] = best;
+	return best;
+}
+
+DeviceJobs::DeviceJobs(std::vector<Job> jobs, void *tmp, unsigned flags, void *stream) :
+	m_jobs(std::move(jobs)),
+	m_tmp(static_cast<char *>(tmp)),
+	m_flags(flags)
+{
+	hipStream_t hs = static_cast<hipStream_t>(stream);
+	std::vector<dev::ResizeArg> descs;
+	std::vector<dev::JobTile> tiles;
+	std::vector<double> tile_cost;
+	size_t off = 0;
+	for (const Job &j : m_jobs) {
+		const DeviceProgram &prog = *j.prog;
+		const DeviceProgram::Tables &tabs = prog.tables_for_current_device(); // uploads tables outside any capture
+		m_tmp_off.push_back(off);
+		const bool pers = eligible(prog, j.src, j.dst);
+		m_persistent.push_back(pers ? 1 : 0);
+		if (!pers) {
+			off += align_up(prog.tmp_size(1), 256);
+			continue;
+		}
+		for (const auto &s : prog.steps()) {
+			const dev::PlaneArg src = plane_of(j.src, s.in[0].idx), dst = plane_of(j.dst, s.out[0].idx);
+			const TiledConfig cfg = (m_flags & AUTOTUNE) ? tune(prog, s, src, dst, tabs, hs) : s.fast;
+			const dev::ResizeArg a = prog.resize_arg(s, src, dst, 1, tabs, &cfg);
+			const unsigned gx = (a.dst_w + a.tile_w - 1) / a.tile_w, gy = (a.dst_h + a.tile_h - 1) / a.tile_h;
+			const double per_tile = cfg.cost / (static_cast<double>(gx) * gy);
+			const unsigned d = static_cast<unsigned>(descs.size());
+			descs.push_back(a);
+			m_lds = std::max(m_lds, dev::resize_lds_bytes(a));
+			for (unsigned by = 0; by < gy; ++by) {
+				for (unsigned bx = 0; bx < gx; ++bx) {
+					tiles.push_back(dev::JobTile{ d, bx, by, 0 });
+					tile_cost.push_back(per_tile);
+				}
+			}
+		}
+	}
+	if (off && !m_tmp)
+		fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "device tmp buffer required");
+
+	// Most expensive tiles first; workgroups take tiles round-robin.
+	std::vector<size_t> order(tiles.size());
+	for (size_t k = 0; k < order.size(); ++k)
+		order[k] = k;
+	std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return tile_cost[x] > tile_cost[y]; });
+	std::vector<dev::JobTile> sorted;
+	sorted.reserve(tiles.size());
+	for (size_t k : order)
+		sorted.push_back(tiles[k]);
+
+	m_ndescs = static_cast<unsigned>(descs.size());
+	m_ntiles = static_cast<unsigned>(sorted.size());
+	if (m_ntiles) {
+		const size_t dbytes = align_up(descs.size() * sizeof(dev::ResizeArg), 256);
+		const size_t bytes = dbytes + sorted.size() * sizeof(dev::JobTile);
+		check_hip(hipMalloc(&m_dev, bytes), "hipMalloc(jobs)");
+		check_hip(hipMemcpy(m_dev, descs.data(), descs.size() * sizeof(dev::ResizeArg), hipMemcpyHostToDevice), "hipMemcpy(jobs)");
+		check_hip(hipMemcpy(static_cast<char *>(m_dev) + dbytes, sorted.data(), sorted.size() * sizeof(dev::JobTile), hipMemcpyHostToDevice),
+		          "hipMemcpy(jobs)");
+		m_descs = static_cast<const dev::ResizeArg *>(m_dev);
+		m_tiles = reinterpret_cast<const dev::JobTile *>(static_cast<char *>(m_dev) + dbytes);
+		int device = 0, cus = 0;
+		check_hip(hipGetDevice(&device), "hipGetDevice");
+		check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
+		const size_t per_cu = std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / std::max<size_t>(m_lds, 1)));
+		m_grid = static_cast<unsigned>(std::min<size_t>(m_ntiles, static_cast<size_t>(cus) * per_cu));
+	}
+
+	if (!(m_flags & NO_GRAPH)) {
+		check_hip(hipStreamCreateWithFlags(&m_capture, hipStreamNonBlocking), "hipStreamCreate");
+		check_hip(hipStreamBeginCapture(m_capture, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+		try {
+			enqueue(m_capture);
+		} catch (...) {
+			hipGraph_t g = nullptr;
+			(void)hipStreamEndCapture(m_capture, &g);
+			if (g)
+				(void)hipGraphDestroy(g);
+			throw;
+		}
+		check_hip(hipStreamEndCapture(m_capture, &m_graph), "hipStreamEndCapture");
+		check_hip(hipGraphInstantiate(&m_exec, m_graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+	}
+}
+
+DeviceJobs::~DeviceJobs()
+{
+	if (m_exec)
+		(void)hipGraphExecDestroy(m_exec);
+	if (m_graph)
+		(void)hipGraphDestroy(m_graph);
+	if (m_capture)
+		(void)hipStreamDestroy(m_capture);
+	if (m_dev)
+		(void)hipFree(m_dev);
+}
+
+void DeviceJobs::enqueue(hipStream_t stream) const
+{
+	for (size_t k = 0; k < m_jobs.size(); ++k) {
+		if (!m_persistent[k])
+			m_jobs[k].prog->run(m_jobs[k].src, m_jobs[k].dst, m_tmp ? m_tmp + m_tmp_off[k] : nullptr, 1, stream);
+	}
+	check_hip(static_cast<hipError_t>(dev::launch_resize_jobs(m_descs, m_tiles, m_ntiles, m_grid, m_lds, stream)), "persistent resize launch");
+}
+
+void DeviceJobs::run(void *stream)
+{
+	if (m_exec)
+		check_hip(hipGraphLaunch(m_exec, static_cast<hipStream_t>(stream)), "hipGraphLaunch");
+	else
+		enqueue(static_cast<hipStream_t>(stream));
+}
+
+std::string DeviceJobs::describe() const
+{
+	size_t pers = 0;
+	for (char c : m_persistent)
+		pers += c;
+	char buf[256];
+	std::snprintf(buf, sizeof(buf), "jobs=%zu persistent=%zu descriptors=%u tiles=%u grid=%u lds=%zu tuned=%u graph=%d\n", m_jobs.size(), pers,
+	              m_ndescs, m_ntiles, m_grid, m_lds, m_tuned, m_exec ? 1 : 0);
+	return buf;
+}
+
+} // namespace zamd

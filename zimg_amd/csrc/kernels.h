@@ -127,12 +127,23 @@ struct MfmaArg {
 	OutXform out;
 };
 
+// One work item of a batched launch: output tile (bx, by) of frame f of
+// resize descriptor desc.
+struct JobTile {
+	unsigned desc, bx, by, f;
+};
+
 // Launchers. Return hipSuccess (0) or the HIP error of the launch.
 int launch_resize(const ResizeArg &a, void *stream);
 int launch_point(const PointArg &a, void *stream);
 int launch_colorspace(const CspArg &a, void *stream);
 int launch_error_diffusion(const EdArg &a, void *stream);
 int launch_mfma_resize(const MfmaArg &a, void *stream);
+// Largest register capacity (tap pairs) the persistent batched kernel handles.
+constexpr int kJobsMaxTaps = 12;
+// Persistent launch over WORD tiled resize descriptors (device arrays):
+// `grid` workgroups walk the tile list with a stride of `grid`.
+int launch_resize_jobs(const ResizeArg *descs, const JobTile *tiles, unsigned ntiles, unsigned grid, size_t lds, void *stream);
 
 // LDS bytes the resize kernel needs for a tile configuration (host-side sizing).
 size_t resize_lds_bytes(const ResizeArg &a);

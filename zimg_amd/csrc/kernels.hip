@@ -596,11 +596,12 @@ __host__ __device__ inline size_t tiled_lds_bytes(const ResizeArg &a, size_t *in
 	return tab + in + elem * a.lds_mid_rows * a.lds_mid_cols;
 }
 
+// One output tile (bx, by) of frame f. Every thread of the block reaches the
+// same barriers; the block must synchronise before reusing `smem`.
 template <bool WORD, int N>
-__global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
+__device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, unsigned by, unsigned f, unsigned char *smem)
 {
 	using T = lds_t<WORD>;
-	extern __shared__ __align__(16) unsigned char smem[];
 	size_t in_off, mid_off;
 	tiled_lds_bytes(a, &in_off, &mid_off);
 	uint32_t *vtab = reinterpret_cast<uint32_t *>(smem);
@@ -608,8 +609,7 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 	unsigned char *mid_base = smem + mid_off;
 
 	const unsigned t = threadIdx.x;
-	const unsigned f = blockIdx.z;
-	const unsigned j0 = blockIdx.x * a.tile_w, i0 = blockIdx.y * a.tile_h;
+	const unsigned j0 = bx * a.tile_w, i0 = by * a.tile_h;
 	const unsigned nrows = min(a.tile_h, a.dst_h - i0);
 	const unsigned ncols = min(a.tile_w, a.dst_w - j0);
 	const char *src = frame_ptr(a.src, f);
@@ -873,6 +873,34 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 				store_pair<false>(row_ptr(dst, a.dst.stride, row), j, v0, v1, two, a.out, row);
 			}
 		}
+	}
+}
+
+template <bool WORD, int N>
+__global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
+{
+	extern __shared__ __align__(16) unsigned char smem[];
+	resize_tile<WORD, N>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+}
+
+// Persistent batched resize: each workgroup walks the (cost-ordered) tile
+// list of many WORD resize descriptors (register capacity up to
+// kJobsMaxTaps pairs), one tile at a time. Descriptors and tiles are
+// read-only and not aliased, so their uniform reads are scalar loads.
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) k_resize_jobs(const ResizeArg *__restrict__ descs, const JobTile *__restrict__ tiles,
+                                                          unsigned ntiles)
+{
+	extern __shared__ __align__(16) unsigned char smem[];
+	for (unsigned it = blockIdx.x; it < ntiles; it += gridDim.x) {
+		const JobTile &jt = tiles[it];
+		const ResizeArg &a = descs[jt.desc];
+		switch (a.fwh) {
+#define ZAMD_JOB(n) case n: resize_tile<true, n>(a, jt.bx, jt.by, jt.f, smem); break;
+		ZAMD_JOB(2) ZAMD_JOB(3) ZAMD_JOB(4) ZAMD_JOB(5) ZAMD_JOB(6) ZAMD_JOB(8) ZAMD_JOB(12)
+#undef ZAMD_JOB
+		default: break;
+		}
+		__syncthreads();
 	}
 }
 
@@ -1199,6 +1227,14 @@ size_t resize_lds_bytes(const ResizeArg &a)
 int launch_resize(const ResizeArg &a, void *stream)
 {
 	return a.word ? launch_resize_t<true>(a, stream) : launch_resize_t<false>(a, stream);
+}
+
+int launch_resize_jobs(const ResizeArg *descs, const JobTile *tiles, unsigned ntiles, unsigned grid, size_t lds, void *stream)
+{
+	if (!ntiles)
+		return 0;
+	hipLaunchKernelGGL(k_resize_jobs, dim3(grid), dim3(kThreads), lds, static_cast<hipStream_t>(stream), descs, tiles, ntiles);
+	return static_cast<int>(hipGetLastError());
 }
 
 int launch_point(const PointArg &a, void *stream)

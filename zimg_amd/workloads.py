@@ -9,9 +9,10 @@ Used by the tests and bench.py; nothing here is on the processing path.
 """
 import numpy as np
 
-from . import (COLOR_RGB, COLOR_YUV, DITHER_ERROR_DIFFUSION, MATRIX_BT709, MATRIX_BT2020_NCL, MATRIX_RGB, PIXEL_FLOAT,
-               PIXEL_WORD, PRIMARIES_BT709, PRIMARIES_BT2020, RANGE_FULL, RANGE_LIMITED, RESIZE_BICUBIC, RESIZE_LANCZOS,
-               TRANSFER_ARIB_B67, TRANSFER_BT709, TRANSFER_ST2084, builder_params, image_format)
+from . import (COLOR_RGB, COLOR_YUV, DITHER_ERROR_DIFFUSION, DITHER_ORDERED, MATRIX_BT709, MATRIX_BT2020_NCL, MATRIX_RGB,
+               PIXEL_BYTE, PIXEL_FLOAT, PIXEL_WORD, PRIMARIES_BT709, PRIMARIES_BT2020, RANGE_FULL, RANGE_LIMITED, RESIZE_BICUBIC,
+               RESIZE_BILINEAR, RESIZE_LANCZOS, RESIZE_SPLINE36, TRANSFER_ARIB_B67, TRANSFER_BT709, TRANSFER_ST2084, builder_params,
+               image_format)
 
 
 def rng(seed):
@@ -111,3 +112,56 @@ def uhd_pq_to_hlg_formats(w=3840, h=2160, peak=203.0):
                        matrix_coefficients=MATRIX_RGB, transfer_characteristics=TRANSFER_ARIB_B67,
                        color_primaries=PRIMARIES_BT2020)
     return src, dst, builder_params(dither_type=DITHER_ERROR_DIFFUSION, nominal_peak_luminance=peak)
+
+
+# Batched uneven jobs: YUV 4:2:0, BYTE 8-bit or WORD 10-bit limited range, each
+# resized to another size of the set with its own filter, then ordered-dithered
+# to 8-bit. Sizes and filters are drawn from a seeded generator.
+BATCH_SIZES = [(1280, 720), (1920, 1080), (2560, 1440), (3840, 2160)]
+BATCH_FILTERS = [("bilinear", RESIZE_BILINEAR), ("bicubic", RESIZE_BICUBIC), ("spline36", RESIZE_SPLINE36), ("lanczos3", RESIZE_LANCZOS)]
+
+
+def batch_jobs(n=64, seed=2024, sizes=BATCH_SIZES):
+    r = rng(seed)
+    jobs = []
+    for k in range(n):
+        si = int(r.integers(len(sizes)))
+        di = int(r.integers(len(sizes) - 1))
+        di = di + 1 if di >= si else di
+        fname, _ = BATCH_FILTERS[int(r.integers(len(BATCH_FILTERS)))]
+        bits = 8 if r.random() < 0.5 else 10
+        jobs.append({"src": list(sizes[si]), "dst": list(sizes[di]), "filter": fname, "bits": bits})
+    return jobs
+
+
+def batch_job_graph(job):
+    """(src format, dst format, params) of one batched job."""
+    common = dict(subsample_w=1, subsample_h=1, color_family=COLOR_YUV, matrix_coefficients=MATRIX_BT709,
+                  transfer_characteristics=TRANSFER_BT709, color_primaries=PRIMARIES_BT709)
+    (sw, sh), (dw, dh) = job["src"], job["dst"]
+    bits = job["bits"]
+    src = image_format(width=sw, height=sh, pixel_type=PIXEL_BYTE if bits == 8 else PIXEL_WORD, depth=bits, **common)
+    dst = image_format(width=dw, height=dh, pixel_type=PIXEL_BYTE, depth=8, **common)
+    filt = dict(BATCH_FILTERS)[job["filter"]]
+    a = 3.0 if filt == RESIZE_LANCZOS else float("nan")
+    b = float("nan")
+    if filt == RESIZE_BICUBIC:
+        a, b = 0.0, 0.5
+    params = builder_params(resample_filter=filt, resample_filter_uv=filt, filter_param_a=a, filter_param_b=b,
+                            filter_param_a_uv=a, filter_param_b_uv=b, dither_type=DITHER_ORDERED)
+    return src, dst, params
+
+
+def batch_job_planes(job, seed):
+    """Seeded Y, U, V planes of a job: smooth gradients plus noise, limited range."""
+    (w, h), bits = job["src"], job["bits"]
+    r = rng(seed)
+    lo, hi = (16, 235) if bits == 8 else (64, 940)
+    clo, chi = (16, 240) if bits == 8 else (64, 960)
+    dt = np.uint8 if bits == 8 else np.uint16
+    yy, xx = np.mgrid[0:h, 0:w]
+    Y = lo + (hi - lo) * (0.5 + 0.4 * np.sin(xx / 53.0) * np.cos(yy / 37.0)) + r.normal(0, 4 if bits == 8 else 16, (h, w))
+    cy, cx = np.mgrid[0:h // 2, 0:w // 2]
+    U = (clo + chi) / 2 + (chi - clo) * 0.3 * np.sin(cx / 29.0 + cy / 41.0) + r.normal(0, 3, (h // 2, w // 2))
+    V = (clo + chi) / 2 + (chi - clo) * 0.3 * np.cos(cx / 31.0 - cy / 23.0) + r.normal(0, 3, (h // 2, w // 2))
+    return [np.clip(np.rint(P), lo if P is Y else clo, hi if P is Y else chi).astype(dt) for P in (Y, U, V)]
