@@ -454,6 +454,52 @@ size_t zimg_device_jobs_describe(const zimg_device_jobs *jobs, char *buf, size_t
 ZIMG_VISIBILITY
 void zimg_device_jobs_free(zimg_device_jobs *jobs);
 
+/* device extension: learned transfer-function surrogate. A 1-32-32-1 SiLU
+ * MLP for one transfer function (transfer: zimg_transfer_characteristics_e;
+ * ST2084, ARIB_B67 and the BT.1886 family; to_linear selects the direction),
+ * trained on the current device by the library's own forward and backward
+ * kernels on samples of the exact function, then validated per input bin
+ * against the FP64 function: bins whose error exceeds `tolerance` (absolute,
+ * or relative when `relative` is nonzero) are evaluated exactly. */
+typedef struct zimg_transfer_surrogate zimg_transfer_surrogate;
+
+ZIMG_VISIBILITY
+zimg_transfer_surrogate *zimg_transfer_surrogate_train(int transfer, int to_linear, double tolerance, int relative, unsigned steps,
+                                                       unsigned seed, void *stream);
+
+/* values[k] <- post * f(values[k] * pre) for n device floats. */
+ZIMG_VISIBILITY
+zimg_error_code_e zimg_transfer_surrogate_apply(const zimg_transfer_surrogate *s, float *values, size_t n, float pre, float post,
+                                                void *stream);
+
+/* The same map with the exact transfer function only. */
+ZIMG_VISIBILITY
+zimg_error_code_e zimg_transfer_exact_apply(int transfer, int to_linear, float *values, size_t n, float pre, float post, void *stream);
+
+/* Fraction of validated bins, largest validated error, final training loss. */
+ZIMG_VISIBILITY
+zimg_error_code_e zimg_transfer_surrogate_info(const zimg_transfer_surrogate *s, double *coverage, double *max_error, double *train_loss);
+
+/* Network parameters (zimg_transfer_surrogate_param_count() floats: w1[32],
+ * b1[32], W2[32][32] row-major by output, b2[32], w3[32], b3). */
+ZIMG_VISIBILITY
+unsigned zimg_transfer_surrogate_param_count(void);
+
+ZIMG_VISIBILITY
+zimg_error_code_e zimg_transfer_surrogate_get_params(const zimg_transfer_surrogate *s, float *params);
+
+/* The training kernels' gradient of mean((v - t)^2) over n host samples
+ * (n a multiple of 64); grad receives param_count() + 1 floats, the last one
+ * the loss. The inference layers' outputs v at inputs u. For verification. */
+ZIMG_VISIBILITY
+zimg_error_code_e zimg_transfer_surrogate_gradients(const float *params, const float *u, const float *t, unsigned n, float *grad);
+
+ZIMG_VISIBILITY
+zimg_error_code_e zimg_transfer_surrogate_eval(const float *params, const float *u, float *v, unsigned n);
+
+ZIMG_VISIBILITY
+void zimg_transfer_surrogate_free(zimg_transfer_surrogate *s);
+
 /* device extension: a short text description of the lowered device stages
  * (the analogue of the reference's FilterObserver trace, graphbuilder.h:45-64).
  * Writes at most n bytes including the terminator; returns the full length. */

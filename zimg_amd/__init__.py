@@ -69,6 +69,9 @@ EXPORTS = [
     "zimg_filter_graph_set_resize_precision",
     "zimg_device_jobs_get_tmp_size", "zimg_device_jobs_create", "zimg_device_jobs_run", "zimg_device_jobs_describe",
     "zimg_device_jobs_free",
+    "zimg_transfer_surrogate_train", "zimg_transfer_surrogate_apply", "zimg_transfer_exact_apply", "zimg_transfer_surrogate_info",
+    "zimg_transfer_surrogate_param_count", "zimg_transfer_surrogate_get_params", "zimg_transfer_surrogate_gradients",
+    "zimg_transfer_surrogate_eval", "zimg_transfer_surrogate_free",
 ]
 
 DEVICE_JOBS_AUTOTUNE, DEVICE_JOBS_NO_GRAPH = 1, 2
@@ -176,6 +179,26 @@ def lib():
         L.zimg_device_jobs_describe.argtypes = [vp, ctypes.c_char_p, sz]
         L.zimg_device_jobs_free.restype = None
         L.zimg_device_jobs_free.argtypes = [vp]
+        fp = ctypes.POINTER(ctypes.c_float)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.zimg_transfer_surrogate_train.restype = vp
+        L.zimg_transfer_surrogate_train.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_uint, ctypes.c_uint, vp]
+        L.zimg_transfer_surrogate_apply.restype = ctypes.c_int
+        L.zimg_transfer_surrogate_apply.argtypes = [vp, vp, sz, ctypes.c_float, ctypes.c_float, vp]
+        L.zimg_transfer_exact_apply.restype = ctypes.c_int
+        L.zimg_transfer_exact_apply.argtypes = [ctypes.c_int, ctypes.c_int, vp, sz, ctypes.c_float, ctypes.c_float, vp]
+        L.zimg_transfer_surrogate_info.restype = ctypes.c_int
+        L.zimg_transfer_surrogate_info.argtypes = [vp, dp, dp, dp]
+        L.zimg_transfer_surrogate_param_count.restype = ctypes.c_uint
+        L.zimg_transfer_surrogate_param_count.argtypes = []
+        L.zimg_transfer_surrogate_get_params.restype = ctypes.c_int
+        L.zimg_transfer_surrogate_get_params.argtypes = [vp, fp]
+        L.zimg_transfer_surrogate_gradients.restype = ctypes.c_int
+        L.zimg_transfer_surrogate_gradients.argtypes = [fp, fp, fp, ctypes.c_uint, fp]
+        L.zimg_transfer_surrogate_eval.restype = ctypes.c_int
+        L.zimg_transfer_surrogate_eval.argtypes = [fp, fp, fp, ctypes.c_uint]
+        L.zimg_transfer_surrogate_free.restype = None
+        L.zimg_transfer_surrogate_free.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -338,3 +361,61 @@ class DeviceJobs:
         if h and _lib is not None:
             _lib.zimg_device_jobs_free(h)
             self._h = None
+
+
+def _fptr(a):
+    import numpy as _np
+    assert a.dtype == _np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class TransferSurrogate:
+    """zimg_transfer_surrogate_*: a trained MLP for one transfer function."""
+
+    def __init__(self, transfer, to_linear=True, tolerance=1e-4, relative=False, steps=4000, seed=1, stream=None):
+        self._h = lib().zimg_transfer_surrogate_train(transfer, int(to_linear), tolerance, int(relative), steps, seed, stream)
+        if not self._h:
+            code, msg = last_error()
+            raise ZimgError(code, msg)
+
+    def apply(self, ptr, n, pre=1.0, post=1.0, stream=None):
+        check(lib().zimg_transfer_surrogate_apply(self._h, ptr, n, pre, post, stream))
+
+    def info(self):
+        c, e, l = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(lib().zimg_transfer_surrogate_info(self._h, ctypes.byref(c), ctypes.byref(e), ctypes.byref(l)))
+        return {"coverage": c.value, "max_validated_error": e.value, "train_loss": l.value}
+
+    def params(self):
+        import numpy as _np
+        out = _np.zeros(surrogate_param_count(), _np.float32)
+        check(lib().zimg_transfer_surrogate_get_params(self._h, _fptr(out)))
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.zimg_transfer_surrogate_free(h)
+            self._h = None
+
+
+def transfer_exact_apply(transfer, to_linear, ptr, n, pre=1.0, post=1.0, stream=None):
+    check(lib().zimg_transfer_exact_apply(transfer, int(to_linear), ptr, n, pre, post, stream))
+
+
+def surrogate_param_count():
+    return lib().zimg_transfer_surrogate_param_count()
+
+
+def surrogate_gradients(params, u, t):
+    import numpy as _np
+    grad = _np.zeros(surrogate_param_count() + 1, _np.float32)
+    check(lib().zimg_transfer_surrogate_gradients(_fptr(params), _fptr(u), _fptr(t), len(u), _fptr(grad)))
+    return grad
+
+
+def surrogate_eval(params, u):
+    import numpy as _np
+    v = _np.zeros(len(u), _np.float32)
+    check(lib().zimg_transfer_surrogate_eval(_fptr(params), _fptr(u), _fptr(v), len(u)))
+    return v

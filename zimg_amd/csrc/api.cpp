@@ -17,6 +17,7 @@
 
 #include "device.h"
 #include "jobs.h"
+#include "learned.h"
 #include "graph.h"
 #include "zimg.h"
 
@@ -745,6 +746,110 @@ size_t zimg_device_jobs_describe(const zimg_device_jobs *jobs, char *buf, size_t
 void zimg_device_jobs_free(zimg_device_jobs *jobs)
 {
 	delete reinterpret_cast<zamd::DeviceJobs *>(jobs);
+}
+
+zimg_transfer_surrogate *zimg_transfer_surrogate_train(int transfer, int to_linear, double tolerance, int relative, unsigned steps,
+                                                       unsigned seed, void *stream)
+{
+	try {
+		zamd::learned::Config cfg;
+		cfg.transfer = transfer;
+		cfg.to_linear = to_linear ? 1 : 0;
+		cfg.tolerance = tolerance;
+		cfg.relative = relative ? 1 : 0;
+		cfg.steps = steps;
+		cfg.seed = seed;
+		return reinterpret_cast<zimg_transfer_surrogate *>(new zamd::learned::Surrogate(cfg, stream));
+	} catch (...) {
+		handle_current_exception();
+		return nullptr;
+	}
+}
+
+zimg_error_code_e zimg_transfer_surrogate_apply(const zimg_transfer_surrogate *s, float *values, size_t n, float pre, float post,
+                                                void *stream)
+{
+	try {
+		if (!s || (n && !values))
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null argument");
+		reinterpret_cast<const zamd::learned::Surrogate *>(s)->apply(values, n, pre, post, stream);
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
+zimg_error_code_e zimg_transfer_exact_apply(int transfer, int to_linear, float *values, size_t n, float pre, float post, void *stream)
+{
+	try {
+		if (n && !values)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null argument");
+		zamd::learned::apply_exact(zamd::learned::function_for(transfer, to_linear), values, n, pre, post, stream);
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
+zimg_error_code_e zimg_transfer_surrogate_info(const zimg_transfer_surrogate *s, double *coverage, double *max_error, double *train_loss)
+{
+	try {
+		if (!s)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null surrogate");
+		const auto *m = reinterpret_cast<const zamd::learned::Surrogate *>(s);
+		if (coverage) *coverage = m->coverage();
+		if (max_error) *max_error = m->max_validated_error();
+		if (train_loss) *train_loss = m->train_loss();
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
+unsigned zimg_transfer_surrogate_param_count(void)
+{
+	return static_cast<unsigned>(zamd::learned::param_count());
+}
+
+zimg_error_code_e zimg_transfer_surrogate_get_params(const zimg_transfer_surrogate *s, float *params)
+{
+	try {
+		if (!s || !params)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null argument");
+		reinterpret_cast<const zamd::learned::Surrogate *>(s)->get_params(params);
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
+zimg_error_code_e zimg_transfer_surrogate_gradients(const float *params, const float *u, const float *t, unsigned n, float *grad)
+{
+	try {
+		if (!params || !u || !t || !grad)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null argument");
+		zamd::learned::gradients(params, u, t, n, grad, nullptr);
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
+zimg_error_code_e zimg_transfer_surrogate_eval(const float *params, const float *u, float *v, unsigned n)
+{
+	try {
+		if (!params || !u || !v)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null argument");
+		zamd::learned::eval(params, u, v, n, nullptr);
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
+void zimg_transfer_surrogate_free(zimg_transfer_surrogate *s)
+{
+	delete reinterpret_cast<zamd::learned::Surrogate *>(s);
 }
 
 size_t zimg_filter_graph_describe(const zimg_filter_graph *ptr, char *buf, size_t n)
