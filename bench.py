@@ -75,6 +75,10 @@ def device_batch(torch, graph, planes, out_shapes, frames):
     def launch():
         graph.process_device_batch(frames, src, src_fs, dst, dst_fs, tmp.data_ptr(), stream)
 
+    def profiled():
+        return graph.process_device_profiled(frames, src, src_fs, dst, dst_fs, tmp.data_ptr(), stream)
+
+    launch.profiled = profiled
     return launch, keep
 
 
@@ -215,8 +219,41 @@ def measure(torch, name, frames, steps, warmup, precision="exact"):
     fps = frames * steps / elapsed
     achieved = algo_bytes * frames / per_step / 1e9
     launches = [l for l in g.describe().split("device:\n")[1].splitlines() if l]
+    # Per-stage durations from HIP events recorded on the launch stream around
+    # each device stage, averaged over `steps` further steps.
+    acc = None
+    for _ in range(steps):
+        st = launch.profiled()
+        acc = [(a[0] + b[0], b[1]) for a, b in zip(acc, st)] if acc else list(st)
+    stages = [{"stage": launches[k] if k < len(launches) else str(k), "ms": ms / steps, "bytes_per_frame": by,
+               "gbs": by * frames / (ms / steps * 1e-3) / 1e9 if ms > 0 else 0.0} for k, (ms, by) in enumerate(acc)]
     return {"fps": fps, "ms_per_step": per_step * 1e3, "achieved_gbs": achieved, "algo_bytes_per_frame": algo_bytes,
-            "launches": len(launches), "dtype": dtype, "graph": launches}
+            "launches": len(launches), "dtype": dtype, "graph": launches, "stages": stages}
+
+
+def traffic_of(kernel_key):
+    """HBM bytes per launch of a kernel, from a committed rocprofv3 --pmc summary
+    (profiles/<key>_traffic.txt, a line 'traffic_bytes_per_launch: N'), if present."""
+    path = os.path.join(ROOT, "profiles", f"{kernel_key}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def roofline(m, workload, frames):
+    """Dominant device stage: algorithmic bytes per launch / its average launch
+    duration (HIP events on the launch stream); whole-graph figure alongside."""
+    dom = max(m["stages"], key=lambda st: st["ms"])
+    achieved = dom["gbs"]
+    traffic = traffic_of(f"{workload}_dominant")
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": traffic, "kernel": dom["stage"], "kernel_ms_per_launch": round(dom["ms"], 4),
+            "algorithmic_bytes_per_launch": dom["bytes_per_frame"] * frames,
+            "whole_graph": {"achieved": round(m["achieved_gbs"], 1), "frac": round(m["achieved_gbs"] / PEAK_HBM_GBS, 4)},
+            "stages": [{"ms": round(st["ms"], 4), "gbs": round(st["gbs"], 1), "stage": st["stage"][:60]} for st in m["stages"]]}
 
 
 def main():
@@ -264,9 +301,7 @@ def main():
         "dtype": m["dtype"],
         "data": "synthetic (seeded, zimg_amd/workloads.py), inputs resident in HBM",
         "config": {"workload": primary, "frames_per_step": args.frames, "graph_launches": m["launches"], "precision": args.precision},
-        "roofline": {"bound": "hbm", "achieved": round(m["achieved_gbs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(m["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": None,
-                     "kernel": "whole graph per launch sequence" if m["launches"] > 1 else m["graph"][0]},
+        "roofline": roofline(m, primary, args.frames),
         "cpu_baseline": cpu,
         "secondary": None if not m2 else {
             "workload": secondary, "fps": round(m2["fps"], 1), "achieved_gbs": round(m2["achieved_gbs"], 1),

@@ -422,6 +422,18 @@ typedef enum zimg_resize_precision_e {
 ZIMG_VISIBILITY
 zimg_error_code_e zimg_filter_graph_set_resize_precision(zimg_filter_graph *ptr, int mode);
 
+/* device extension: zimg_filter_graph_process_device_batch with HIP events
+ * recorded around each device stage on `stream`; waits for the stream, then
+ * writes up to max_stages stage durations (ms) and algorithmic bytes per
+ * frame (each stage's input planes read once, output planes written once),
+ * and the number of stages to *stages. */
+ZIMG_VISIBILITY
+zimg_error_code_e zimg_filter_graph_process_device_profiled(const zimg_filter_graph *ptr, unsigned count,
+                                                            const zimg_image_buffer_const *src, const ptrdiff_t src_frame_stride[4],
+                                                            const zimg_image_buffer *dst, const ptrdiff_t dst_frame_stride[4],
+                                                            void *tmp, void *stream, unsigned max_stages, float *stage_ms,
+                                                            double *stage_bytes, unsigned *stages);
+
 /* device extension: batched jobs. `count` independent graphs, job k running
  * graphs[k] on src[k] -> dst[k] (device buffers as above, one frame each),
  * are planned once into one persistent launch over the resize tiles of all

@@ -66,7 +66,7 @@ EXPORTS = [
     "zimg_image_format_default", "zimg_graph_builder_params_default", "zimg_filter_graph_build",
     "zimg_filter_graph_get_device_tmp_size", "zimg_filter_graph_process_device",
     "zimg_filter_graph_process_device_batch", "zimg_filter_graph_describe",
-    "zimg_filter_graph_set_resize_precision",
+    "zimg_filter_graph_set_resize_precision", "zimg_filter_graph_process_device_profiled",
     "zimg_device_jobs_get_tmp_size", "zimg_device_jobs_create", "zimg_device_jobs_run", "zimg_device_jobs_describe",
     "zimg_device_jobs_free",
     "zimg_transfer_surrogate_train", "zimg_transfer_surrogate_apply", "zimg_transfer_exact_apply", "zimg_transfer_surrogate_info",
@@ -166,6 +166,12 @@ def lib():
                                                              ctypes.POINTER(ctypes.c_ssize_t), vp, vp]
         L.zimg_filter_graph_describe.restype = sz
         L.zimg_filter_graph_describe.argtypes = [vp, ctypes.c_char_p, sz]
+        L.zimg_filter_graph_process_device_profiled.restype = ctypes.c_int
+        L.zimg_filter_graph_process_device_profiled.argtypes = [vp, ctypes.c_uint, ctypes.POINTER(ImageBufferStruct),
+                                                                ctypes.POINTER(ctypes.c_ssize_t), ctypes.POINTER(ImageBufferStruct),
+                                                                ctypes.POINTER(ctypes.c_ssize_t), vp, vp, ctypes.c_uint,
+                                                                ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double),
+                                                                ctypes.POINTER(ctypes.c_uint)]
         L.zimg_filter_graph_set_resize_precision.restype = ctypes.c_int
         L.zimg_filter_graph_set_resize_precision.argtypes = [vp, ctypes.c_int]
         L.zimg_device_jobs_get_tmp_size.restype = ctypes.c_int
@@ -295,6 +301,17 @@ class FilterGraph:
         out = ctypes.c_uint()
         check(lib().zimg_filter_graph_get_output_buffering(self._h, ctypes.byref(out)))
         return out.value
+
+    def process_device_profiled(self, count, src, src_fs, dst, dst_fs, tmp=None, stream=None):
+        """process_device_batch with per-stage HIP events: [(ms, bytes per frame)] per device stage."""
+        sfs = (ctypes.c_ssize_t * 4)(*src_fs)
+        dfs = (ctypes.c_ssize_t * 4)(*dst_fs)
+        ms = (ctypes.c_float * 64)()
+        by = (ctypes.c_double * 64)()
+        n = ctypes.c_uint()
+        check(lib().zimg_filter_graph_process_device_profiled(self._h, count, ctypes.byref(src), sfs, ctypes.byref(dst), dfs, tmp, stream,
+                                                              64, ms, by, ctypes.byref(n)))
+        return [(ms[k], by[k]) for k in range(min(n.value, 64))]
 
     def set_resize_precision(self, mode):
         check(lib().zimg_filter_graph_set_resize_precision(self._h, mode))

@@ -672,6 +672,46 @@ zimg_error_code_e zimg_filter_graph_process_device(const zimg_filter_graph *ptr,
 	return zimg_filter_graph_process_device_batch(ptr, 1, src, nullptr, dst, nullptr, tmp, stream);
 }
 
+zimg_error_code_e zimg_filter_graph_process_device_profiled(const zimg_filter_graph *ptr, unsigned count,
+                                                            const zimg_image_buffer_const *src, const ptrdiff_t src_frame_stride[4],
+                                                            const zimg_image_buffer *dst, const ptrdiff_t dst_frame_stride[4],
+                                                            void *tmp, void *stream, unsigned max_stages, float *stage_ms,
+                                                            double *stage_bytes, unsigned *stages)
+{
+	std::vector<hipEvent_t> ev;
+	try {
+		const DeviceProgram &prog = program(ptr);
+		const Plan &plan = prog.plan();
+		DeviceBuffers ds = import_device_buffer(src, src_frame_stride, plan.src_w, true);
+		DeviceBuffers dd = import_device_buffer(dst, dst_frame_stride, plan.dst_w, false);
+		const size_t n = prog.steps().size();
+		ev.resize(n + 1, nullptr);
+		for (auto &e : ev)
+			if (hipEventCreate(&e) != hipSuccess)
+				fail(ZIMG_ERROR_UNKNOWN, "hipEventCreate");
+		std::vector<void *> evp(ev.begin(), ev.end());
+		prog.run(ds, dd, tmp, count, stream, evp.data());
+		if (hipEventSynchronize(ev[n]) != hipSuccess)
+			fail(ZIMG_ERROR_UNKNOWN, "hipEventSynchronize");
+		const std::vector<double> bytes = prog.step_bytes();
+		for (size_t k = 0; k < n && k < max_stages; ++k) {
+			float ms = 0;
+			(void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+			if (stage_ms) stage_ms[k] = ms;
+			if (stage_bytes) stage_bytes[k] = bytes[k];
+		}
+		if (stages) *stages = static_cast<unsigned>(n);
+		for (auto &e : ev)
+			(void)hipEventDestroy(e);
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		for (auto &e : ev)
+			if (e)
+				(void)hipEventDestroy(e);
+		return handle_current_exception();
+	}
+}
+
 zimg_error_code_e zimg_filter_graph_set_resize_precision(zimg_filter_graph *ptr, int mode)
 {
 	try {

@@ -81,7 +81,12 @@ public:
 	void set_resize_precision(int mode);
 	int resize_precision() const { return m_precision; }
 	// Enqueue the program for `frames` frames. Throws Error on invalid buffers or launch failure.
-	void run(const DeviceBuffers &src, const DeviceBuffers &dst, void *tmp, unsigned frames, void *stream) const;
+	// events (optional, steps().size() + 1 hipEvent_t): recorded before each step and after the last.
+	void run(const DeviceBuffers &src, const DeviceBuffers &dst, void *tmp, unsigned frames, void *stream,
+	         void *const *events = nullptr) const;
+	// Algorithmic bytes of one frame through each step: its input planes read
+	// once and its output planes written once.
+	std::vector<double> step_bytes() const;
 	std::string describe() const;
 
 	const Plan &plan() const { return m_plan; }

@@ -873,7 +873,35 @@ dev::ResizeArg DeviceProgram::resize_arg(const Step &s, const dev::PlaneArg &src
 	return a;
 }
 
-void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void *tmp, unsigned frames, void *stream) const
+std::vector<double> DeviceProgram::step_bytes() const
+{
+	auto elem = [](int type) { return type == 0 ? 1.0 : type == 3 ? 4.0 : 2.0; };
+	std::vector<double> out;
+	for (const Step &s : m_steps) {
+		double b = 0;
+		const double px = static_cast<double>(s.width) * s.height;
+		switch (s.kind) {
+		case Step::RESIZE:
+			b = static_cast<double>(s.in_w) * s.in_h * elem(s.ix[0].type) + px * elem(s.ox[0].type);
+			break;
+		case Step::POINT:
+			b = (s.fill ? 0.0 : px * elem(s.ix[0].type)) + px * elem(s.ox[0].type);
+			break;
+		case Step::CSP:
+			for (int k = 0; k < 3; ++k)
+				b += px * (elem(s.ix[k].type) + elem(s.ox[k].type));
+			break;
+		case Step::ED:
+			b = s.ed_planes * px * (elem(s.ix[0].type) + elem(s.ox[0].type));
+			break;
+		}
+		out.push_back(b);
+	}
+	return out;
+}
+
+void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void *tmp, unsigned frames, void *stream,
+                        void *const *events) const
 {
 	if (!frames)
 		return;
@@ -923,6 +951,8 @@ void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void
 	for (size_t si = 0; si < m_steps.size(); ++si) {
 		const Step &s = m_steps[si];
 		int rc = 0;
+		if (events)
+			check_hip(hipEventRecord(static_cast<hipEvent_t>(events[si]), static_cast<hipStream_t>(stream)), "hipEventRecord");
 		if (!s.dead && mfma_step(s)) {
 			run_mfma_resize(s, resolve(s.in[0]), resolve(s.out[0]), static_cast<char *>(tmp) + mfma_off[si], frames, stream);
 			continue;
@@ -986,6 +1016,8 @@ void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void
 		if (rc != 0)
 			fail(ZIMG_ERROR_UNKNOWN, std::string("kernel launch failed: ") + s.label + ": " + hipGetErrorString(static_cast<hipError_t>(rc)));
 	}
+	if (events)
+		check_hip(hipEventRecord(static_cast<hipEvent_t>(events[m_steps.size()]), static_cast<hipStream_t>(stream)), "hipEventRecord");
 }
 
 std::string DeviceProgram::describe() const
