@@ -99,6 +99,7 @@ struct EdArg {
 	InXform in;        // load only (op must be 0)
 	OutXform out;      // op 1, dither ignored
 	float scale[4], offset[4];
+	unsigned delay, ring; // set by the launcher: strip delay (steps) and LDS ring length
 };
 
 // Banded-contraction tables of one FLOAT resize pass for the matrix-core

@@ -1079,96 +1079,144 @@ __global__ void __launch_bounds__(kThreads) k_colorspace(CspArg a)
 }
 
 // Floyd-Steinberg error diffusion (dither.cpp:57-89, 286-349) as a row-skewed
-// wavefront. Lane l of wave w owns row band_base + 64 w + l and handles column
-// j = T - 2 l - w D at step T, so the three errors it needs from the row above
-// were produced by lane l-1 one, two and three steps earlier and arrive by a
-// one-lane shuffle. Across waves the boundary row goes through an LDS ring;
-// waves are offset by D = 2*63 + 2 + kEdChunk columns and synchronise every
-// kEdChunk steps, so every ring entry is written a full chunk before it is
-// read. Across bands the last row is kept in an LDS row buffer. The float
-// operation order equals the sequential reference, hence bit-exact output.
+// wavefront. The plane is cut into strips of 64 rows; strip s belongs to wave
+// s mod kEdWaves and starts at step s D. Lane l of the strip owns row
+// 64 s + l and handles column j = T - s D - 2 l - 2 at step T (every lane,
+// lane 0 included, spends two steps left of column 0 taking in the row
+// above's first errors), so the three errors
+// it needs from the row above were produced by lane l - 1 one, two and three
+// steps earlier and arrive one lane over (DPP wave shift). Between strips the
+// boundary row goes through a per-wave LDS ring. D is at least
+// 2 * 63 + 2 + kEdChunk (a ring entry is written a chunk before it is read)
+// and at least (W + 129) / kEdWaves (a wave finishes a strip before its next
+// one starts), so strips run back to back without a pipeline drain. The float
+// operation order is the sequential reference's, hence bit-exact output.
+// Per 16-step chunk each lane reads its 16 inputs and writes its 16 outputs
+// with paired accesses, and the ring is read and written once.
 constexpr int kEdWaves = 16;
 constexpr int kEdChunk = 16;
-constexpr int kEdDelay = 2 * 63 + 2 + kEdChunk;
-constexpr int kEdRing = 256;
+constexpr int kEdMinDelay = 2 * 63 + 2 + kEdChunk;
 
-__device__ __forceinline__ float ed_load(const char *srow, int j, int W, int type, bool row_ok)
+struct EdGeom {
+	int strip;   // strip this wave works on in the chunk, -1 none
+	int jb;      // this lane's column at the chunk's first step
+	int row;
+	bool row_ok;
+};
+
+__device__ __forceinline__ EdGeom ed_geom(int T0, int w, int lane, int D, int nstrips, int W, int H)
 {
-	return (row_ok && j >= 0 && j < W) ? load_as_float(srow, static_cast<unsigned>(j), type) : 0.0f;
+	EdGeom g{ -1, 0, 0, false };
+	if (T0 < w * D)
+		return g;
+	const int s = w + kEdWaves * ((T0 - w * D) / (kEdWaves * D));
+	if (s >= nstrips || T0 - s * D > W + 2 * 63 + 2)
+		return g;
+	g.strip = s;
+	g.jb = T0 - s * D - 2 * lane - 2;
+	g.row = 64 * s + lane;
+	g.row_ok = g.row < H;
+	return g;
+}
+
+// 16 consecutive inputs of one row from column jb (zero outside [0, W)).
+__device__ __forceinline__ void ed_load16(const char *srow, int jb, int W, int type, bool ok, float *x)
+{
+	if (ok && jb >= 0 && jb + kEdChunk <= W && type == 3 && (reinterpret_cast<uintptr_t>(srow + 4 * jb) & 7) == 0) {
+		const float2 *p = reinterpret_cast<const float2 *>(srow + 4 * static_cast<size_t>(jb));
+#pragma unroll
+		for (int k = 0; k < kEdChunk / 2; ++k) {
+			const float2 v = p[k];
+			x[2 * k] = v.x;
+			x[2 * k + 1] = v.y;
+		}
+		return;
+	}
+	if (ok && jb >= 0 && jb + kEdChunk <= W && type == 1 && (reinterpret_cast<uintptr_t>(srow + 2 * jb) & 3) == 0) {
+		const uint32_t *p = reinterpret_cast<const uint32_t *>(srow + 2 * static_cast<size_t>(jb));
+#pragma unroll
+		for (int k = 0; k < kEdChunk / 2; ++k) {
+			const uint32_t v = p[k];
+			x[2 * k] = static_cast<float>(v & 0xFFFFu);
+			x[2 * k + 1] = static_cast<float>(v >> 16);
+		}
+		return;
+	}
+#pragma unroll
+	for (int k = 0; k < kEdChunk; ++k) {
+		const int j = jb + k;
+		x[k] = (ok && j >= 0 && j < W) ? load_as_float(srow, static_cast<unsigned>(j), type) : 0.0f;
+	}
 }
 
 __global__ void __launch_bounds__(kEdWaves * 64) k_error_diffusion(EdArg a)
 {
 	extern __shared__ __align__(16) float ed_smem[];
-	float *ring = ed_smem;                      // [kEdWaves][kEdRing]
-	float *prev_row = ring + kEdWaves * kEdRing; // [width + 2], padded row of the previous band's last row
+	const int R = static_cast<int>(a.ring);
+	float *ring = ed_smem; // [kEdWaves][R]
 
-	// blockIdx.x: frame, blockIdx.y: plane of this launch.
 	const unsigned f = blockIdx.x;
 	const unsigned pl = blockIdx.y;
 	const PlaneArg &sp = a.src[pl];
 	const PlaneArg &dp = a.dst[pl];
 	const int lane = threadIdx.x & 63;
 	const int w = threadIdx.x >> 6;
-	const int W = static_cast<int>(a.width);
+	const int W = static_cast<int>(a.width), H = static_cast<int>(a.height);
+	const int D = static_cast<int>(a.delay);
 	const char *src = frame_ptr(sp, f);
 	char *dst = frame_ptr_w(dp, f);
 	const float maxval = static_cast<float>(1UL << a.out.bits) - 1;
 	const float scale = a.scale[pl], offset = a.offset[pl];
-
-	for (int t = threadIdx.x; t < W + 2; t += kEdWaves * 64)
-		prev_row[t] = 0.0f;
-	__syncthreads();
-
-	const int band_rows = kEdWaves * 64;
-	const int steps = W + 2 * 63 + (kEdWaves - 1) * kEdDelay + 1;
+	const int nstrips = (H + 63) / 64;
+	const int steps = (nstrips - 1) * D + W + 2 * 63 + 2;
 	const int nchunks = (steps + kEdChunk - 1) / kEdChunk;
+	float *my_ring = ring + w * R;
+	const float *up_ring = ring + ((w + kEdWaves - 1) % kEdWaves) * R;
 
-	for (int base = 0; base < static_cast<int>(a.height); base += band_rows) {
-		const int r = base + w * 64 + lane;
-		const bool row_ok = r < static_cast<int>(a.height);
-		const char *srow = row_ptr(src, sp.stride, row_ok ? r : 0);
-		char *drow = row_ptr(dst, dp.stride, row_ok ? r : 0);
-		float e_left = 0.0f, t_m1 = 0.0f, t_0 = 0.0f, t_p1 = 0.0f, prod = 0.0f;
+	float e_left = 0.0f, t_m1 = 0.0f, t_0 = 0.0f, t_p1 = 0.0f, prod = 0.0f;
+	int cur = -1;
+	float xin[kEdChunk], xnext[kEdChunk];
+	EdGeom gn = ed_geom(0, w, lane, D, nstrips, W, H);
+	ed_load16(row_ptr(src, sp.stride, gn.row_ok ? gn.row : 0), gn.jb, W, a.in.type, gn.strip >= 0 && gn.row_ok, xnext);
 
-		// Inputs of a chunk are fetched one chunk ahead, so the HBM latency of
-		// this lane's row is off the step-to-step critical path.
-		const int jbase = -2 * lane - w * kEdDelay;
-		float xin[kEdChunk], xnext[kEdChunk];
+	for (int ch = 0; ch < nchunks; ++ch) {
+		const int T0 = ch * kEdChunk;
+		const EdGeom g = gn;
 #pragma unroll
-		for (int s = 0; s < kEdChunk; ++s)
-			xnext[s] = ed_load(srow, jbase + s, W, a.in.type, row_ok);
-
-		for (int ch = 0; ch < nchunks; ++ch) {
+		for (int k = 0; k < kEdChunk; ++k)
+			xin[k] = xnext[k];
+		// Inputs of the next chunk are fetched now, off the step-to-step path.
+		gn = ed_geom(T0 + kEdChunk, w, lane, D, nstrips, W, H);
+		ed_load16(row_ptr(src, sp.stride, gn.row_ok ? gn.row : 0), gn.jb, W, a.in.type, gn.strip >= 0 && gn.row_ok, xnext);
+		if (g.strip >= 0) {
+			if (g.strip != cur) {
+				cur = g.strip;
+				e_left = t_m1 = t_0 = t_p1 = prod = 0.0f;
+			}
+			// Row above lane 0: the previous strip's last row, columns jb0 + 1 .. jb0 + 16.
+			const int jb0 = T0 - g.strip * D - 2;
+			float rv = 0.0f;
+			if (lane < kEdChunk && g.strip > 0) {
+				const int c = jb0 + 1 + lane;
+				if (c >= 0 && c < W)
+					rv = up_ring[c & (R - 1)];
+			}
+			float eout[kEdChunk];
+			uint32_t q[kEdChunk];
 #pragma unroll
-			for (int s = 0; s < kEdChunk; ++s)
-				xin[s] = xnext[s];
-#pragma unroll
-			for (int s = 0; s < kEdChunk; ++s)
-				xnext[s] = ed_load(srow, jbase + (ch + 1) * kEdChunk + s, W, a.in.type, row_ok);
-#pragma unroll
-			for (int s = 0; s < kEdChunk; ++s) {
-				const int T = ch * kEdChunk + s;
-				const int j = T - 2 * lane - w * kEdDelay;
-				const float up = __shfl_up(prod, 1);
-				float recv;
-				if (lane == 0) {
-					const int jn = j + 1;
-					if (jn < 0 || jn >= W)
-						recv = 0.0f;
-					else if (w == 0)
-						recv = prev_row[jn + 1];
-					else
-						recv = ring[(w - 1) * kEdRing + (jn & (kEdRing - 1))];
-				} else {
-					recv = up;
-				}
+			for (int t = 0; t < kEdChunk; ++t) {
+				const int j = g.jb + t;
+				// wave_shr:1 -- lane l takes lane l - 1's error; lane 0 keeps `old`, the ring value.
+				const int old = __builtin_amdgcn_readlane(__float_as_int(rv), t);
+				const float recv = __int_as_float(__builtin_amdgcn_update_dpp(old, __float_as_int(prod), 0x138, 0xF, 0xF, false));
 				t_m1 = t_0;
 				t_0 = t_p1;
 				t_p1 = recv;
 				prod = 0.0f;
-				if (row_ok && j >= 0 && j < W) {
-					float x = xin[s] * scale + offset;
+				q[t] = 0;
+				eout[t] = 0.0f;
+				if (g.row_ok && j >= 0 && j < W) {
+					float x = xin[t] * scale + offset;
 					float err = 0;
 					err += e_left * (7.0f / 16.0f);
 					err += t_p1 * (3.0f / 16.0f);
@@ -1176,23 +1224,47 @@ __global__ void __launch_bounds__(kEdWaves * 64) k_error_diffusion(EdArg a)
 					err += t_m1 * (1.0f / 16.0f);
 					x += err;
 					x = x < 0.0f ? 0.0f : (maxval < x ? maxval : x);
-					const float q = __builtin_rintf(x);
-					store_uint(drow, static_cast<unsigned>(j), a.out.type, static_cast<unsigned>(static_cast<int>(q)));
-					const float e = x - q;
+					const float qf = __builtin_rintf(x);
+					q[t] = static_cast<uint32_t>(static_cast<int>(qf));
+					const float e = x - qf;
 					e_left = e;
 					prod = e;
-					if (lane == 63) {
-						if (w < kEdWaves - 1)
-							ring[w * kEdRing + (j & (kEdRing - 1))] = e;
+					eout[t] = e;
+				}
+			}
+			// Outputs of the chunk, in column pairs (jb is even).
+			if (g.row_ok) {
+				char *drow = row_ptr(dst, dp.stride, g.row);
+#pragma unroll
+				for (int t = 0; t < kEdChunk; t += 2) {
+					const int j = g.jb + t;
+					const bool v0 = j >= 0 && j < W, v1 = j + 1 >= 0 && j + 1 < W;
+					const uintptr_t at = reinterpret_cast<uintptr_t>(drow) + (a.out.type == 0 ? 1 : 2) * static_cast<uintptr_t>(j);
+					if (v0 && v1 && (at & (a.out.type == 0 ? 1 : 3)) == 0) {
+						if (a.out.type == 0)
+							*reinterpret_cast<uint16_t *>(drow + j) = static_cast<uint16_t>(q[t] | (q[t + 1] << 8));
 						else
-							prev_row[j + 1] = e;
+							*reinterpret_cast<uint32_t *>(drow + 2 * static_cast<size_t>(j)) = q[t] | (q[t + 1] << 16);
+					} else if (v0 && v1) {
+						store_uint(drow, static_cast<unsigned>(j), a.out.type, q[t]);
+						store_uint(drow, static_cast<unsigned>(j + 1), a.out.type, q[t + 1]);
+					} else if (v0) {
+						store_uint(drow, static_cast<unsigned>(j), a.out.type, q[t]);
+					} else if (v1) {
+						store_uint(drow, static_cast<unsigned>(j + 1), a.out.type, q[t + 1]);
 					}
 				}
 			}
-			__syncthreads();
+			// The strip's last row feeds the next strip through this wave's ring.
+			if (lane == 63) {
+#pragma unroll
+				for (int t = 0; t < kEdChunk; ++t) {
+					const int j = g.jb + t;
+					if (j >= 0 && j < W)
+						my_ring[j & (R - 1)] = eout[t];
+				}
+			}
 		}
-		// A band whose last rows are past the image leaves zeros for nobody; the
-		// next band (if any) always starts with a complete previous row.
 		__syncthreads();
 	}
 }
@@ -1253,8 +1325,20 @@ int launch_colorspace(const CspArg &a, void *stream)
 
 int launch_error_diffusion(const EdArg &a, void *stream)
 {
-	const size_t lds = sizeof(float) * (static_cast<size_t>(kEdWaves) * kEdRing + a.width + 2);
-	hipLaunchKernelGGL(k_error_diffusion, dim3(a.frames, a.planes), dim3(kEdWaves * 64), lds, static_cast<hipStream_t>(stream), a);
+	// Strip delay: a chunk of slack past the two-column skew of 64 rows, and
+	// long enough that a wave finishes a strip before its next one starts.
+	const unsigned need = (a.width + 2 * 63 + 3 + kEdWaves - 1) / kEdWaves;
+	const unsigned delay = std::max<unsigned>(kEdMinDelay, (need + kEdChunk - 1) / kEdChunk * kEdChunk);
+	unsigned ring = 256;
+	while (ring < delay)
+		ring *= 2;
+	if (ring > 1024)
+		return static_cast<int>(hipErrorInvalidValue); // widths above 16257 samples
+	EdArg b = a;
+	b.delay = delay;
+	b.ring = ring;
+	const size_t lds = sizeof(float) * static_cast<size_t>(kEdWaves) * ring;
+	hipLaunchKernelGGL(k_error_diffusion, dim3(b.frames, b.planes), dim3(kEdWaves * 64), lds, static_cast<hipStream_t>(stream), b);
 	return static_cast<int>(hipGetLastError());
 }
 
