@@ -1042,6 +1042,7 @@ constexpr int kCspPix = 4;
 
 __global__ void __launch_bounds__(kThreads) k_colorspace(CspArg a)
 {
+	zm::lds_tables_init();
 	const unsigned f = blockIdx.z;
 	const unsigned i = blockIdx.y;
 	const unsigned jb = blockIdx.x * (kThreads * kCspPix) + threadIdx.x;

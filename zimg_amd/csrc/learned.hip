@@ -286,6 +286,7 @@ __global__ void __launch_bounds__(256) k_mlp_adam(float *__restrict__ params, fl
 // to n values in place: x -> post * f(x * pre).
 __global__ void __launch_bounds__(64 * kWaves) k_surrogate_apply(float *__restrict__ x, unsigned n, learned::DeviceModel m)
 {
+	zm::lds_tables_init();
 	__shared__ float p[NPARAM];
 	__shared__ WaveLds wl[kWaves];
 	const unsigned tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -332,6 +333,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_surrogate_apply(float *__restri
 // The exact transfer over n values in place (the method without the surrogate).
 __global__ void __launch_bounds__(256) k_transfer_apply(float *__restrict__ x, unsigned n, int func, float pre, float post)
 {
+	zm::lds_tables_init();
 	for (unsigned s = blockIdx.x * 256 + threadIdx.x; s < n; s += gridDim.x * 256)
 		x[s] = post * zm::transfer(func, x[s] * pre);
 }

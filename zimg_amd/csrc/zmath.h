@@ -73,6 +73,40 @@ ZCONST static const double kLnC[16] = {
 	0x1.526e57720db08p-3, 0x1.bc2860d22477p-3, 0x1.1058bc8a07ee1p-2, 0x1.4043057b6ee09p-2,
 };
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// Device code reads per-workgroup LDS copies of the tables (the lookups are
+// per lane, so the copies turn vector memory loads into LDS reads). Every
+// kernel that evaluates these functions calls lds_tables_init() first.
+static __shared__ uint64_t s_exp2_tab[32];
+static __shared__ double s_inv_c[16], s_log2_c[16], s_ln_c[16];
+#define ZM_EXP2TAB(i) s_exp2_tab[i]
+#define ZM_INVC(i) s_inv_c[i]
+#define ZM_LOG2C(i) s_log2_c[i]
+#define ZM_LNC(i) s_ln_c[i]
+
+__device__ __forceinline__ void lds_tables_init()
+{
+	for (unsigned k = threadIdx.x; k < 32; k += blockDim.x) {
+		s_exp2_tab[k] = kExp2Tab[k];
+		if (k < 16) {
+			s_inv_c[k] = kInvC[k];
+			s_log2_c[k] = kLog2C[k];
+			s_ln_c[k] = kLnC[k];
+		}
+	}
+	__syncthreads();
+}
+#else
+#define ZM_EXP2TAB(i) kExp2Tab[i]
+#define ZM_INVC(i) kInvC[i]
+#define ZM_LOG2C(i) kLog2C[i]
+#define ZM_LNC(i) kLnC[i]
+#if defined(__HIPCC__)
+// Host pass of a HIP translation unit: kernels are parsed, never run here.
+__device__ inline void lds_tables_init() {}
+#endif
+#endif
+
 ZHD float exp2_from_log(double xd, uint32_t sign_bias)
 {
 	// 2^xd = 2^(k/32) * 2^r, |r| <= 1/64; cubic for 2^r.
@@ -81,7 +115,7 @@ ZHD float exp2_from_log(double xd, uint32_t sign_bias)
 	uint64_t ki = dbits(kd);
 	kd -= shift;
 	double r = xd - kd;
-	uint64_t t = kExp2Tab[ki % 32] + ((ki + sign_bias) << 47);
+	uint64_t t = ZM_EXP2TAB(ki % 32) + ((ki + sign_bias) << 47);
 	double s = bitsd(t);
 	double z = 0x1.c6af84b912394p-5 * r + 0x1.ebfce50fac4f3p-3;
 	double r2 = r * r;
@@ -98,8 +132,8 @@ ZHD double log2_reduced(uint32_t ix)
 	uint32_t top = tmp & 0xff800000u;
 	int k = static_cast<int32_t>(top) >> 23;
 	double z = static_cast<double>(bitsf(ix - top));
-	double r = z * kInvC[i] - 1.0;
-	double y0 = kLog2C[i] + static_cast<double>(k);
+	double r = z * ZM_INVC(i) - 1.0;
+	double y0 = ZM_LOG2C(i) + static_cast<double>(k);
 	double r2 = r * r;
 	double y = 0x1.27616c9496e0bp-2 * r + -0x1.71969a075c67ap-2;
 	double p = 0x1.ec70a6ca7baddp-2 * r + -0x1.7154748bef6c8p-1;
@@ -175,7 +209,7 @@ ZHD float expf_(float x)
 	uint64_t ki = dbits(kd);
 	kd -= shift;
 	double r = z - kd;
-	uint64_t t = kExp2Tab[ki % 32] + (ki << 47);
+	uint64_t t = ZM_EXP2TAB(ki % 32) + (ki << 47);
 	double s = bitsd(t);
 	const double c0 = 0x1.c6af84b912394p-5 / 32 / 32 / 32, c1 = 0x1.ebfce50fac4f3p-3 / 32 / 32, c2 = 0x1.62e42ff0c52d6p-1 / 32;
 	z = c0 * r + c1;
@@ -200,8 +234,8 @@ ZHD float logf_(float x)
 	int i = static_cast<int>((tmp >> 19) % 16);
 	int k = static_cast<int32_t>(tmp) >> 23;
 	double z = static_cast<double>(bitsf(ix - (tmp & (0x1ffu << 23))));
-	double r = z * kInvC[i] - 1.0;
-	double y0 = kLnC[i] + static_cast<double>(k) * 0x1.62e42fefa39efp-1;
+	double r = z * ZM_INVC(i) - 1.0;
+	double y0 = ZM_LNC(i) + static_cast<double>(k) * 0x1.62e42fefa39efp-1;
 	double r2 = r * r;
 	double y = 0x1.5575b0be00b6ap-2 * r + -0x1.ffffef20a4123p-2;
 	y = -0x1.00ea348b88334p-2 * r2 + y;
