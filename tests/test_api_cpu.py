@@ -217,12 +217,25 @@ def test_tiled_resize_configurations_in_bounds():
 
 
 def test_resize_precision_setter_validates_and_sizes_scratch():
+    # one launch with the intermediate in LDS: no tmp
     s, d, p = W.fp32_formats(3840, 2160, 1920, 1080, z.RESIZE_LANCZOS, 3.0)
     g = z.FilterGraph(s, d, p)
     base = g.device_tmp_size(2)
     g.set_resize_precision(z.RESIZE_PRECISION_MFMA_FP16)
-    # the two-pass matrix-core form keeps its FP32 intermediate (3840 x 1080) in tmp
-    assert g.device_tmp_size(2) >= base + 2 * 3840 * 1080 * 4
+    assert g.device_tmp_size(2) == base
+    assert "matrix cores fp16: one launch, 64x64 tiles" in g.describe()
+    s, d, p = W.fp32_formats(1280, 720, 3840, 2160, z.RESIZE_BICUBIC, 0.0, 0.5)
+    g = z.FilterGraph(s, d, p)
+    g.set_resize_precision(z.RESIZE_PRECISION_MFMA_BF16)
+    assert "matrix cores bf16: one launch, 64x128 tiles" in g.describe()
+    # a 120-tap horizontal pass does not fit the one-launch form: two launches
+    # through an FP32 intermediate (dst_w x src_h = 100 x 64) in tmp
+    s, d, p = W.fp32_formats(2000, 64, 100, 500, z.RESIZE_LANCZOS, 3.0)
+    g = z.FilterGraph(s, d, p)
+    base = g.device_tmp_size(2)
+    g.set_resize_precision(z.RESIZE_PRECISION_MFMA_FP16)
+    assert g.device_tmp_size(2) >= base + 2 * 100 * 64 * 4
+    assert "2 launch(es) through an FP32 intermediate" in g.describe()
     with pytest.raises(z.ZimgError) as e:
         g.set_resize_precision(7)
     assert e.value.code == z.ERROR_ENUM_OUT_OF_RANGE

@@ -24,6 +24,9 @@ CASES = [
     (1280, 720, 3840, 2160, BICUBIC, 0.0, 0.5),
     (333, 217, 1001, 77, LANCZOS, 4.0, 0.0),
     (97, 400, 31, 999, BICUBIC, 0.0, 0.5),
+    # two launches through an HBM intermediate (120-tap horizontal pass), and one pass only
+    (2000, 64, 100, 500, LANCZOS, 3.0, 0.0),
+    (500, 300, 700, 300, LANCZOS, 3.0, 0.0),
 ]
 
 BOUND = {z.RESIZE_PRECISION_MFMA_FP16: 4e-6, z.RESIZE_PRECISION_MFMA_BF16: 6e-5}

@@ -121,6 +121,13 @@ private:
 	int m_precision = 0;
 
 	bool mfma_step(const Step &s) const { return m_precision && s.kind == Step::RESIZE && !s.word && s.ox[0].dither == 0; }
+	// Launch shape of the one-launch two-pass matrix-core resize (ok == false:
+	// the step runs as two launches through an HBM intermediate).
+	struct FusedPlan {
+		bool ok = false;
+		unsigned tile_h = 0, slab_stride = 0, mid_stride = 0, coef_blocks = 0, lds = 0;
+	};
+	FusedPlan mfma_fused_plan(const Step &s) const;
 	size_t mfma_mid_bytes(const Step &s, unsigned frames) const;
 	const MfmaTables &mfma_tables_for_current_device() const;
 	void run_mfma_resize(const Step &s, const dev::PlaneArg &src, const dev::PlaneArg &dst, char *mid, unsigned frames, void *stream) const;

@@ -587,10 +587,107 @@ void DeviceProgram::set_resize_precision(int mode)
 	m_precision = mode;
 }
 
-// FP32 intermediate of a two-pass FLOAT resize run on matrix cores.
+namespace {
+
+// Input window of each 16-output tile of a pass on matrix cores: inputs
+// [kb, kend) hold every nonzero coefficient (both multiples of 8), read in
+// nks steps of 32 from kb.
+struct Band {
+	std::vector<int> kb, kend;
+	std::vector<unsigned> nks;
+};
+
+Band band_of(const FilterTable &t)
+{
+	Band b;
+	const unsigned tiles = (t.filter_rows + 15) / 16;
+	for (unsigned tt = 0; tt < tiles; ++tt) {
+		unsigned lo = UINT_MAX, hi = 0;
+		for (unsigned o = 16 * tt; o < std::min(16 * tt + 16, t.filter_rows); ++o) {
+			lo = std::min(lo, t.left[o]);
+			hi = std::max(hi, t.left[o] + t.filter_width);
+		}
+		const unsigned kb = lo & ~7u;
+		b.kb.push_back(static_cast<int>(kb));
+		b.kend.push_back(static_cast<int>((hi + 7) & ~7u));
+		b.nks.push_back((hi - kb + 31) / 32);
+	}
+	return b;
+}
+
+// Widest input window over runs of `group` consecutive tiles.
+unsigned group_window(const Band &b, unsigned group)
+{
+	unsigned widest = 0;
+	for (size_t t0 = 0; t0 < b.kb.size(); t0 += group) {
+		int lo = INT_MAX, hi = 0;
+		for (size_t t = t0; t < std::min(b.kb.size(), t0 + group); ++t) {
+			lo = std::min(lo, b.kb[t]);
+			hi = std::max(hi, b.kend[t]);
+		}
+		widest = std::max(widest, static_cast<unsigned>(hi - lo));
+	}
+	return widest;
+}
+
+// Smallest LDS row stride (16-bit units) of at least n that is an odd multiple
+// of 8: rows then start in distinct 16-byte bank groups.
+unsigned odd_stride(unsigned n)
+{
+	unsigned v = (n + 7) & ~7u;
+	return (v % 16 == 0) ? v + 8 : v;
+}
+
+} // namespace
+
+DeviceProgram::FusedPlan DeviceProgram::mfma_fused_plan(const Step &s) const
+{
+	FusedPlan p;
+	if (s.passes != 2)
+		return p;
+	const FilterTable &th = *m_plan.nodes[s.tab[s.h_first ? 0 : 1]].table;
+	const FilterTable &tv = *m_plan.nodes[s.tab[s.h_first ? 1 : 0]].table;
+	if (th.filter_rows != s.width || tv.filter_rows != s.height)
+		return p;
+	const Band bh = band_of(th), bv = band_of(tv);
+	const auto max_nks = [](const Band &b) { return *std::max_element(b.nks.begin(), b.nks.end()); };
+	if (max_nks(bh) > dev::kFusedMaxSteps || max_nks(bv) > dev::kFusedMaxSteps)
+		return p;
+	const unsigned wc = group_window(bh, dev::kFusedWaves);
+	if (wc > dev::kFusedMaxWindow)
+		return p;
+	// Tallest tile whose LDS leaves room for two workgroups per CU.
+	for (unsigned tile_h : { 128u, 64u, 32u, 16u }) {
+		const unsigned group = tile_h / 16;
+		const unsigned wr = (group_window(bv, group) + 15) & ~15u;
+		unsigned blocks = 0; // vertical coefficient blocks of one workgroup
+		for (size_t t0 = 0; t0 < bv.kb.size(); t0 += group) {
+			unsigned n = 0;
+			for (size_t t = t0; t < std::min(bv.kb.size(), t0 + group); ++t)
+				n += bv.nks[t];
+			blocks = std::max(blocks, n);
+		}
+		if (wr / 16 > dev::kFusedMaxGroups || blocks > dev::kFusedMaxCoefBlocks)
+			continue;
+		const unsigned sc = odd_stride(wc), sr = odd_stride(wr);
+		const unsigned lds = 2 * (4 * 16 * sc + 2 * 16 * dev::kFusedWaves * sr) + 2048 * blocks;
+		if (lds <= 80 * 1024) {
+			p.ok = true;
+			p.tile_h = tile_h;
+			p.slab_stride = sc;
+			p.mid_stride = sr;
+			p.coef_blocks = blocks;
+			p.lds = lds;
+			return p;
+		}
+	}
+	return p;
+}
+
+// FP32 intermediate of a two-pass FLOAT resize run on matrix cores as two launches.
 size_t DeviceProgram::mfma_mid_bytes(const Step &s, unsigned frames) const
 {
-	if (s.passes != 2)
+	if (s.passes != 2 || mfma_fused_plan(s).ok)
 		return 0;
 	return align_up(align_up(static_cast<size_t>(s.mid_w) * 4, kPitchAlign) * s.mid_h * frames, kPitchAlign);
 }
@@ -630,8 +727,8 @@ const DeviceProgram::MfmaTables &DeviceProgram::mfma_tables_for_current_device()
 	const bool bf16 = m_precision == 2;
 	struct Host {
 		int id;
-		std::vector<unsigned> off, nks;
-		std::vector<int> kb;
+		std::vector<unsigned> off;
+		Band band;
 		std::vector<uint16_t> coef;
 	};
 	std::vector<Host> hosts;
@@ -645,18 +742,12 @@ const DeviceProgram::MfmaTables &DeviceProgram::mfma_tables_for_current_device()
 			const FilterTable &t = *m_plan.nodes[id].table;
 			Host h;
 			h.id = id;
-			const unsigned tiles = (t.filter_rows + 15) / 16;
+			h.band = band_of(t);
+			const unsigned tiles = static_cast<unsigned>(h.band.kb.size());
 			unsigned blocks = 0;
 			for (unsigned tt = 0; tt < tiles; ++tt) {
-				unsigned lo = UINT_MAX, hi = 0;
-				for (unsigned o = 16 * tt; o < std::min(16 * tt + 16, t.filter_rows); ++o) {
-					lo = std::min(lo, t.left[o]);
-					hi = std::max(hi, t.left[o] + t.filter_width);
-				}
-				const int kb = static_cast<int>(lo & ~3u);
-				const unsigned nks = (hi - static_cast<unsigned>(kb) + 31) / 32;
-				h.kb.push_back(kb);
-				h.nks.push_back(nks);
+				const int kb = h.band.kb[tt];
+				const unsigned nks = h.band.nks[tt];
 				h.off.push_back(blocks);
 				for (unsigned st = 0; st < nks; ++st) {
 					uint16_t hiv[64 * 8], lov[64 * 8];
@@ -689,7 +780,7 @@ const DeviceProgram::MfmaTables &DeviceProgram::mfma_tables_for_current_device()
 	std::vector<size_t> offs;
 	for (auto &h : hosts) {
 		offs.push_back(bytes);
-		bytes += align_up(h.coef.size() * 2, 256) + align_up(h.off.size() * 4, 256) * 3;
+		bytes += align_up(h.coef.size() * 2, 256) + align_up(h.off.size() * sizeof(dev::MfmaTile), 256);
 	}
 	if (bytes) {
 		std::vector<char> host(bytes, 0);
@@ -698,9 +789,10 @@ const DeviceProgram::MfmaTables &DeviceProgram::mfma_tables_for_current_device()
 			size_t o = offs[n];
 			std::memcpy(&host[o], h.coef.data(), h.coef.size() * 2);
 			o += align_up(h.coef.size() * 2, 256);
-			std::memcpy(&host[o], h.off.data(), h.off.size() * 4);
-			std::memcpy(&host[o + align_up(h.off.size() * 4, 256)], h.kb.data(), h.kb.size() * 4);
-			std::memcpy(&host[o + 2 * align_up(h.off.size() * 4, 256)], h.nks.data(), h.nks.size() * 4);
+			for (size_t t = 0; t < h.off.size(); ++t) {
+				const dev::MfmaTile rec{ h.band.kb[t], h.band.kend[t], h.band.nks[t], h.off[t] };
+				std::memcpy(&host[o + t * sizeof(rec)], &rec, sizeof(rec));
+			}
 		}
 		check_hip(hipMalloc(&tab->mem, bytes), "hipMalloc(matrix-core tables)");
 		check_hip(hipMemcpy(tab->mem, host.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(matrix-core tables)");
@@ -708,9 +800,7 @@ const DeviceProgram::MfmaTables &DeviceProgram::mfma_tables_for_current_device()
 		for (size_t n = 0; n < hosts.size(); ++n) {
 			const Host &h = hosts[n];
 			const size_t o = offs[n] + align_up(h.coef.size() * 2, 256);
-			const size_t a = align_up(h.off.size() * 4, 256);
-			tab->pass[h.id] = dev::MfmaTab{ reinterpret_cast<const uint16_t *>(base + offs[n]), reinterpret_cast<const unsigned *>(base + o),
-			                                reinterpret_cast<const int *>(base + o + a), reinterpret_cast<const unsigned *>(base + o + 2 * a) };
+			tab->pass[h.id] = dev::MfmaTab{ reinterpret_cast<const uint16_t *>(base + offs[n]), reinterpret_cast<const dev::MfmaTile *>(base + o) };
 		}
 	}
 	m_mfma_tables[key] = tab;
@@ -721,6 +811,31 @@ void DeviceProgram::run_mfma_resize(const Step &s, const dev::PlaneArg &src, con
                                     void *stream) const
 {
 	const MfmaTables &mt = mfma_tables_for_current_device();
+	const FusedPlan fp = mfma_fused_plan(s);
+	if (fp.ok) {
+		dev::MfmaFusedArg a{};
+		a.src = src;
+		a.dst = dst;
+		a.frames = frames;
+		a.bf16 = m_precision == 2 ? 1 : 0;
+		a.src_w = s.in_w;
+		a.src_h = s.in_h;
+		a.dst_w = s.width;
+		a.dst_h = s.height;
+		a.tile_h = fp.tile_h;
+		a.slab_stride = fp.slab_stride;
+		a.mid_stride = fp.mid_stride;
+		a.coef_blocks = fp.coef_blocks;
+		a.lds_bytes = fp.lds;
+		a.vec_in = (s.ix[0].type == 3 && s.ix[0].op == 0 && aligned16(src)) ? 1 : 0;
+		a.vec_out = (s.ox[0].type == 3 && s.ox[0].op == 0 && aligned16(dst)) ? 1 : 0;
+		a.th = mt.pass.at(s.tab[s.h_first ? 0 : 1]);
+		a.tv = mt.pass.at(s.tab[s.h_first ? 1 : 0]);
+		a.in = s.ix[0];
+		a.out = s.ox[0];
+		check_hip(static_cast<hipError_t>(dev::launch_mfma_fused(a, stream)), "matrix-core fused resize launch");
+		return;
+	}
 	const dev::InXform plain_in{ 3, 0, 0, 1.0f, 0.0f };
 	const dev::OutXform plain_out{ 3, 0, 0, 0, 32, 1.0f, 0.0f };
 	const size_t mid_pitch = align_up(static_cast<size_t>(s.mid_w) * 4, kPitchAlign);
@@ -1036,6 +1151,16 @@ std::string DeviceProgram::describe() const
 		else
 			std::snprintf(buf, sizeof(buf), "%s %ux%u: %s\n", kinds[s.kind], s.width, s.height, s.label.c_str());
 		r += buf;
+		if (!s.dead && mfma_step(s)) {
+			const FusedPlan fp = mfma_fused_plan(s);
+			if (fp.ok)
+				std::snprintf(buf, sizeof(buf), "  matrix cores %s: one launch, %ux%u tiles, lds=%u\n", m_precision == 2 ? "bf16" : "fp16",
+				              16 * dev::kFusedWaves, fp.tile_h, fp.lds);
+			else
+				std::snprintf(buf, sizeof(buf), "  matrix cores %s: %d launch(es) through an FP32 intermediate\n", m_precision == 2 ? "bf16" : "fp16",
+				              s.passes);
+			r += buf;
+		}
 	}
 	return r;
 }

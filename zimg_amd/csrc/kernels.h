@@ -103,15 +103,19 @@ struct EdArg {
 };
 
 // Banded-contraction tables of one FLOAT resize pass for the matrix-core
-// precision form: outputs in tiles of 16; tile t reads inputs kb[t] ..
-// kb[t] + 32 nks[t] - 1; its coefficient blocks start at block off[t]; a
-// block is one 32-wide k step: 64 lanes x 8 high 16-bit values, then 64 x 8
-// low values (lane l: output 16 t + l % 16, inputs kb + 32 s + 8 (l / 16) + i).
+// precision form: outputs in tiles of 16; tile t reads inputs kb .. kb +
+// 32 nks - 1 (kb a multiple of 8); its coefficients are nonzero only below
+// kend (a multiple of 8); its coefficient blocks start at block off; a block
+// is one 32-wide k step: 64 lanes x 8 high 16-bit values, then 64 x 8 low
+// values (lane l: output 16 t + l % 16, inputs kb + 32 s + 8 (l / 16) + i).
+// Blocks of consecutive tiles are consecutive.
+struct alignas(16) MfmaTile {
+	int kb, kend;
+	unsigned nks, off;
+};
 struct MfmaTab {
 	const uint16_t *coef;
-	const unsigned *off;
-	const int *kb;
-	const unsigned *nks;
+	const MfmaTile *tile;
 };
 
 struct MfmaArg {
@@ -128,6 +132,35 @@ struct MfmaArg {
 	OutXform out;
 };
 
+// Both passes of a two-pass FLOAT resize in one launch, horizontal first.
+// A workgroup owns kFusedWaves x 16 output columns and tile_h output rows;
+// it streams its source window through LDS in 16-row slabs (hi/lo 16-bit
+// planes, row stride slab_stride halves), keeps the intermediate columns in
+// LDS (hi/lo planes, column stride mid_stride halves) next to its vertical
+// coefficient blocks, and writes the output.
+constexpr unsigned kFusedWaves = 4;
+constexpr unsigned kFusedMaxSteps = 4;   // k steps per tile and pass
+constexpr unsigned kFusedMaxWindow = 256; // source columns per workgroup
+constexpr unsigned kFusedMaxRowTiles = 8; // 16-row output tiles per workgroup (tile_h <= 128)
+constexpr unsigned kFusedMaxGroups = 12;  // 16-row intermediate groups per workgroup
+constexpr unsigned kFusedMaxCoefBlocks = 8; // vertical coefficient blocks per workgroup
+struct MfmaFusedArg {
+	PlaneArg src, dst;
+	unsigned frames;
+	int bf16;
+	unsigned src_w, src_h, dst_w, dst_h;
+	unsigned tile_h;       // output rows per workgroup, a multiple of 16
+	unsigned slab_stride;  // halves, >= the widest source window
+	unsigned mid_stride;   // halves, >= 16 x the most intermediate row groups
+	unsigned coef_blocks;  // LDS room for the vertical coefficient blocks of one workgroup
+	unsigned lds_bytes;
+	int vec_in;            // FLOAT source without conversion, 16-byte aligned rows
+	int vec_out;           // FLOAT output stored as is, 16-byte aligned rows
+	MfmaTab th, tv;        // horizontal (src_w -> dst_w) and vertical (src_h -> dst_h) tables
+	InXform in;
+	OutXform out;
+};
+
 // One work item of a batched launch: output tile (bx, by) of frame f of
 // resize descriptor desc.
 struct JobTile {
@@ -140,6 +173,7 @@ int launch_point(const PointArg &a, void *stream);
 int launch_colorspace(const CspArg &a, void *stream);
 int launch_error_diffusion(const EdArg &a, void *stream);
 int launch_mfma_resize(const MfmaArg &a, void *stream);
+int launch_mfma_fused(const MfmaFusedArg &a, void *stream);
 // Largest register capacity (tap pairs) the persistent batched kernel handles.
 constexpr int kJobsMaxTaps = 12;
 // Persistent launch over WORD tiled resize descriptors (device arrays):
