@@ -413,7 +413,14 @@ __device__ __forceinline__ uint32_t shift_bias2(uint32_t w, unsigned s)
 // pitch `pin` samples. c0 and nc are multiples of 16, pin of 16. Rows at or
 // past `rows_valid` and columns at or past `src_w` are stored as zero
 // samples; every tap that could reach them has a zero coefficient.
-template <bool WORD>
+//
+// All 16-byte loads of a batch are issued before any is used, and none is
+// conditional: a thread past the end of the tile repeats the last chunk
+// (same value to the same place), and a chunk at or past the edge of the
+// plane is first read from a clamped address inside the row (the aligned
+// 16 bytes at the row's start or at its last full chunk, in the same page as
+// valid data) and then rebuilt sample by sample.
+template <bool WORD, int U>
 __device__ __forceinline__ void stage_tile(lds_t<WORD> *xin, unsigned pin, const char *src, long long stride, unsigned r0,
                                            unsigned nr, unsigned rows_valid, unsigned c0, unsigned nc, unsigned src_w,
                                            const InXform &in)
@@ -422,36 +429,43 @@ __device__ __forceinline__ void stage_tile(lds_t<WORD> *xin, unsigned pin, const
 	const unsigned ce = 16 / esz; // samples per 16-byte chunk
 	const unsigned cpr = nc / ce; // chunks per row
 	const unsigned total = nr * cpr;
-	constexpr int U = 4;
+	const unsigned last_full = src_w >= ce ? (src_w - ce) / ce * ce : 0; // last chunk start inside the row
 	for (unsigned base = 0; base < total; base += U * kThreads) {
 		uint4 raw[U];
 		unsigned rr[U], cc[U];
 		bool full[U];
 #pragma unroll
 		for (int u = 0; u < U; ++u) {
-			const unsigned e = base + u * kThreads + threadIdx.x;
+			const unsigned e = min(base + u * kThreads + threadIdx.x, total - 1);
 			rr[u] = e / cpr;
 			cc[u] = c0 + (e - rr[u] * cpr) * ce;
-			full[u] = e < total && r0 + rr[u] < rows_valid && cc[u] + ce <= src_w;
-			raw[u] = full[u] ? *reinterpret_cast<const uint4 *>(row_ptr(src, stride, r0 + rr[u]) + static_cast<size_t>(cc[u]) * esz)
-			                 : make_uint4(0, 0, 0, 0);
+			const unsigned r = min(r0 + rr[u], rows_valid - 1);
+			full[u] = r0 + rr[u] < rows_valid && cc[u] + ce <= src_w;
+			const unsigned cl = min(cc[u], last_full);
+			raw[u] = *reinterpret_cast<const uint4 *>(row_ptr(src, stride, r) + static_cast<size_t>(cl) * esz);
 		}
 #pragma unroll
 		for (int u = 0; u < U; ++u) {
-			const unsigned e = base + u * kThreads + threadIdx.x;
-			if (e >= total)
-				continue;
 			const unsigned r = r0 + rr[u];
-			if (!full[u] && r < rows_valid) {
-				// Chunk straddling the right edge: per-sample loads inside the row.
-				const char *srow = row_ptr(src, stride, r);
+			if (!full[u]) {
+				// Chunk below the plane (zero) or straddling its right edge:
+				// per-sample loads inside the row.
 				uint32_t w[4] = { 0, 0, 0, 0 };
-				for (unsigned k = 0; k < ce; ++k) {
-					const unsigned c = cc[u] + k;
-					if (c >= src_w)
-						break;
-					const uint32_t v = load_as_uint(srow, c, in.type);
-					w[(k * esz) / 4] |= esz == 4 ? v : v << (8 * ((k * esz) % 4));
+				if (r < rows_valid) {
+					const char *srow = row_ptr(src, stride, r);
+#pragma unroll
+					for (unsigned k = 0; k < 16; ++k) {
+						const unsigned c = cc[u] + k;
+						if (k < ce && c < src_w) {
+							const uint32_t v = load_as_uint(srow, c, in.type);
+							if (esz == 4)
+								w[k & 3] = v;
+							else if (esz == 2)
+								w[(k >> 1) & 3] |= v << (16 * (k & 1));
+							else
+								w[k >> 2] |= v << (8 * (k & 3));
+						}
+					}
 				}
 				raw[u] = make_uint4(w[0], w[1], w[2], w[3]);
 			}
@@ -482,19 +496,24 @@ __device__ __forceinline__ void stage_tile(lds_t<WORD> *xin, unsigned pin, const
 					}
 					*reinterpret_cast<float4 *>(d) = v;
 				} else {
-					for (unsigned k = 0; k < ce; k += 4) {
-						float f[4];
+					// 16 BYTE or 8 WORD / HALF samples of the chunk, constant word indices.
+					auto conv = [&](uint32_t bits) {
+						float x = in.type == 2 ? half_to_float(static_cast<uint16_t>(bits)) : static_cast<float>(bits);
+						return in.op == 2 ? x * in.scale + in.offset : x;
+					};
+					if (esz == 2) {
 #pragma unroll
-						for (unsigned q = 0; q < 4; ++q) {
-							const unsigned kk = k + q;
-							const uint32_t word = wv[(kk * esz) / 4];
-							const uint32_t bits = esz == 1 ? (word >> (8 * (kk % 4))) & 0xFFu : (word >> (16 * (kk % 2))) & 0xFFFFu;
-							float x = in.type == 2 ? half_to_float(static_cast<uint16_t>(bits)) : static_cast<float>(bits);
-							if (in.op == 2)
-								x = x * in.scale + in.offset;
-							f[q] = x;
+						for (unsigned k = 0; k < 8; k += 4) {
+							const uint32_t lo = wv[k / 2], hi = wv[k / 2 + 1];
+							*reinterpret_cast<float4 *>(d + k) = make_float4(conv(lo & 0xFFFFu), conv(lo >> 16), conv(hi & 0xFFFFu), conv(hi >> 16));
 						}
-						*reinterpret_cast<float4 *>(d + k) = make_float4(f[0], f[1], f[2], f[3]);
+					} else {
+#pragma unroll
+						for (unsigned k = 0; k < 16; k += 4) {
+							const uint32_t wd = wv[k / 4];
+							*reinterpret_cast<float4 *>(d + k) =
+								make_float4(conv(wd & 0xFFu), conv((wd >> 8) & 0xFFu), conv((wd >> 16) & 0xFFu), conv(wd >> 24));
+						}
 					}
 				}
 			}
@@ -516,21 +535,27 @@ struct HRegs {
 	// j, j1: the two columns (equal for a lone last column); cbase: first staged sample.
 	__device__ __forceinline__ void load(const PassArg &p, unsigned j, unsigned j1, unsigned cbase)
 	{
+		// Loads are unconditional (taps past the row re-read its last tap),
+		// so they issue together with the tile's other loads.
 		if constexpr (WORD) {
 			l0 = static_cast<int>(p.pl[j] - cbase) / 2;
 			l1 = static_cast<int>(p.pl[j1] - cbase) / 2;
 #pragma unroll
 			for (int k = 0; k < N; ++k) {
-				c0[k] = k < p.fwp ? p.pc[static_cast<size_t>(j) * p.fwp + k] : 0;
-				c1[k] = k < p.fwp ? p.pc[static_cast<size_t>(j1) * p.fwp + k] : 0;
+				const int kk = min(k, p.fwp - 1);
+				const int32_t v0 = p.pc[static_cast<size_t>(j) * p.fwp + kk], v1 = p.pc[static_cast<size_t>(j1) * p.fwp + kk];
+				c0[k] = k < p.fwp ? v0 : 0;
+				c1[k] = k < p.fwp ? v1 : 0;
 			}
 		} else {
 			l0 = static_cast<int>(p.left[j] - cbase);
 			l1 = static_cast<int>(p.left[j1] - cbase);
 #pragma unroll
 			for (int k = 0; k < N; ++k) {
-				c0[k] = k < p.fw ? p.cf[static_cast<size_t>(j) * p.fw + k] : 0.0f;
-				c1[k] = k < p.fw ? p.cf[static_cast<size_t>(j1) * p.fw + k] : 0.0f;
+				const int kk = min(k, p.fw - 1);
+				const float v0 = p.cf[static_cast<size_t>(j) * p.fw + kk], v1 = p.cf[static_cast<size_t>(j1) * p.fw + kk];
+				c0[k] = k < p.fw ? v0 : 0.0f;
+				c1[k] = k < p.fw ? v1 : 0.0f;
 			}
 		}
 	}
@@ -596,9 +621,12 @@ __host__ __device__ inline size_t tiled_lds_bytes(const ResizeArg &a, size_t *in
 	return tab + in + elem * a.lds_mid_rows * a.lds_mid_cols;
 }
 
-// One output tile (bx, by) of frame f. Every thread of the block reaches the
-// same barriers; the block must synchronise before reusing `smem`.
-template <bool WORD, int N>
+// One output tile (bx, by) of frame f; HF must equal a.h_first. Every thread
+// of the block reaches the same barriers; the block must synchronise before
+// reusing `smem`. Horizontal-first tiles stage fewer rows (4 chunks per
+// thread in flight, fewer registers, more resident blocks); vertical-first
+// tiles stage up to 8 chunks per thread at once.
+template <bool WORD, int N, bool HF>
 __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, unsigned by, unsigned f, unsigned char *smem)
 {
 	using T = lds_t<WORD>;
@@ -623,22 +651,47 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 	const unsigned vstride = (vtaps + 3) & ~3u;
 	const uint32_t *vtop = vtab;
 	const uint32_t *vcoef = vtab + ((a.tile_h + 3) & ~3u);
-	if (vtaps) {
+	// The tile's rows of the vertical table are contiguous in HBM (nrows
+	// start rows, then nrows x vtaps taps): the first kVtRegs x kThreads
+	// entries are loaded into registers now, beside the other loads, and
+	// stored to LDS (taps at a padded row stride) after the staging.
+	constexpr int kVtRegs = 4;
+	const unsigned vtot = vtaps ? nrows * (vtaps + 1) : 0;
+	uint32_t vval[kVtRegs];
+	unsigned vpos[kVtRegs];
+	auto vt_entry = [&](unsigned e, uint32_t &val, unsigned &pos) {
 		const PassArg &pv = a.h_first ? a.p2 : a.p1;
 		const bool pairs = WORD && a.h_first;
-		for (unsigned e = t; e < nrows; e += kThreads)
-			vtab[e] = pairs ? pv.pl[i0 + e] : pv.left[i0 + e];
-		for (unsigned e = t; e < nrows * vstride; e += kThreads) {
-			const unsigned i = e / vstride, k = e - i * vstride;
-			const size_t g = static_cast<size_t>(i0 + i) * vtaps + k;
-			uint32_t v = 0;
-			if (k < vtaps)
-				v = pairs ? static_cast<uint32_t>(pv.pc[g]) : WORD ? static_cast<uint32_t>(static_cast<int32_t>(pv.ci[g])) : __float_as_uint(pv.cf[g]);
-			vtab[((a.tile_h + 3) & ~3u) + e] = v;
+		if (e < nrows) {
+			val = pairs ? pv.pl[i0 + e] : pv.left[i0 + e];
+			pos = e;
+		} else {
+			const unsigned c = e - nrows, i = c / vtaps, k = c - i * vtaps;
+			const size_t g = static_cast<size_t>(i0) * vtaps + c;
+			val = pairs ? static_cast<uint32_t>(pv.pc[g]) : WORD ? static_cast<uint32_t>(static_cast<int32_t>(pv.ci[g])) : __float_as_uint(pv.cf[g]);
+			pos = ((a.tile_h + 3) & ~3u) + i * vstride + k;
 		}
+	};
+	if (vtot) {
+#pragma unroll
+		for (int u = 0; u < kVtRegs; ++u)
+			vt_entry(min(t + u * kThreads, vtot - 1), vval[u], vpos[u]);
 	}
+	auto vt_store = [&]() {
+		if (!vtot)
+			return;
+#pragma unroll
+		for (int u = 0; u < kVtRegs; ++u)
+			vtab[vpos[u]] = vval[u];
+		for (unsigned e = t + kVtRegs * kThreads; e < vtot; e += kThreads) {
+			uint32_t v;
+			unsigned p;
+			vt_entry(e, v, p);
+			vtab[p] = v;
+		}
+	};
 
-	if (a.h_first) {
+	if constexpr (HF) {
 		// Tile width 2 * kThreads: thread t owns output columns j0 + 2t, +1.
 		const PassArg &ph = a.p1;
 		const unsigned jc = 2 * t;
@@ -661,11 +714,11 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 			r1 = i0 + nrows;
 		}
 		const unsigned span = r1 - r0;
-		stage_tile<WORD>(xin, pin, src, a.src.stride, r0, span, a.src_h, cb, nc, a.src_w, a.in);
-		__syncthreads();
-
 		HRegs<WORD, N> h;
 		h.load(ph, j, j1, cb);
+		stage_tile<WORD, 4>(xin, pin, src, a.src.stride, r0, span, a.src_h, cb, nc, a.src_w, a.in);
+		vt_store();
+		__syncthreads();
 		if constexpr (WORD) {
 			const uint32_t *xw = reinterpret_cast<const uint32_t *>(xin);
 			const unsigned pw = pin / 2;
@@ -790,7 +843,21 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 		cb = j0;
 		nc = (ncols + 15) & ~15u;
 	}
-	stage_tile<WORD>(xin, pin, src, a.src.stride, r0, span, a.src_h, cb, nc, a.src_w, a.in);
+	// Horizontal taps of this thread's output column pair (tile_w <= 2 x
+	// kThreads: at most one pair per thread), loaded beside the staging.
+	const unsigned opairs = (ncols + 1) / 2;
+	const unsigned ostride = opairs >= kThreads ? kThreads : opairs;
+	const unsigned ogroups = kThreads / ostride;
+	const unsigned og = t / ostride;
+	const unsigned oq = t - og * ostride;
+	const unsigned ojc = 2 * oq;
+	const bool otwo = ojc + 1 < ncols;
+	const unsigned oj = j0 + ojc;
+	HRegs<WORD, N> h;
+	if (a.passes == 2)
+		h.load(a.p2, oj, otwo ? oj + 1 : oj, cb);
+	stage_tile<WORD, 8>(xin, pin, src, a.src.stride, r0, span, a.src_h, cb, nc, a.src_w, a.in);
+	vt_store();
 	__syncthreads();
 
 	// Vertical pass over column pairs: groups of threads split the rows.
@@ -848,39 +915,27 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 	__syncthreads();
 
 	// Horizontal pass from the intermediate rows.
-	const PassArg &ph = a.p2;
-	const unsigned opairs = (ncols + 1) / 2;
-	const unsigned ostride = opairs >= kThreads ? kThreads : opairs;
-	const unsigned ogroups = kThreads / ostride;
-	const unsigned og = t / ostride;
 	if (og >= ogroups)
 		return;
-	for (unsigned q = t - og * ostride; q < opairs; q += kThreads) {
-		const unsigned jc = 2 * q;
-		const bool two = jc + 1 < ncols;
-		const unsigned j = j0 + jc;
-		HRegs<WORD, N> h;
-		h.load(ph, j, two ? j + 1 : j, cb);
-		for (unsigned i = og; i < nrows; i += ogroups) {
-			const unsigned row = i0 + i;
-			if constexpr (WORD) {
-				int32_t a0, a1;
-				h.apply_word(reinterpret_cast<const uint32_t *>(mid_base) + i * (pm / 2), a0, a1);
-				store_pair<true>(row_ptr(dst, a.dst.stride, row), j, pack_word(a0, a.pixel_max), pack_word(a1, a.pixel_max), two, a.out, row);
-			} else {
-				float v0, v1;
-				h.apply_float(reinterpret_cast<const float *>(mid_base) + i * pm, ph.fw, v0, v1);
-				store_pair<false>(row_ptr(dst, a.dst.stride, row), j, v0, v1, two, a.out, row);
-			}
+	for (unsigned i = og; i < nrows; i += ogroups) {
+		const unsigned row = i0 + i;
+		if constexpr (WORD) {
+			int32_t a0, a1;
+			h.apply_word(reinterpret_cast<const uint32_t *>(mid_base) + i * (pm / 2), a0, a1);
+			store_pair<true>(row_ptr(dst, a.dst.stride, row), oj, pack_word(a0, a.pixel_max), pack_word(a1, a.pixel_max), otwo, a.out, row);
+		} else {
+			float v0, v1;
+			h.apply_float(reinterpret_cast<const float *>(mid_base) + i * pm, a.p2.fw, v0, v1);
+			store_pair<false>(row_ptr(dst, a.dst.stride, row), oj, v0, v1, otwo, a.out, row);
 		}
 	}
 }
 
-template <bool WORD, int N>
+template <bool WORD, int N, bool HF>
 __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 {
 	extern __shared__ __align__(16) unsigned char smem[];
-	resize_tile<WORD, N>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+	resize_tile<WORD, N, HF>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
 
 // Persistent batched resize: each workgroup walks the (cost-ordered) tile
@@ -895,7 +950,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 		const JobTile &jt = tiles[it];
 		const ResizeArg &a = descs[jt.desc];
 		switch (a.fwh) {
-#define ZAMD_JOB(n) case n: resize_tile<true, n>(a, jt.bx, jt.by, jt.f, smem); break;
+#define ZAMD_JOB(n) case n: if (a.h_first) resize_tile<true, n, true>(a, jt.bx, jt.by, jt.f, smem); else resize_tile<true, n, false>(a, jt.bx, jt.by, jt.f, smem); break;
 		ZAMD_JOB(2) ZAMD_JOB(3) ZAMD_JOB(4) ZAMD_JOB(5) ZAMD_JOB(6) ZAMD_JOB(8) ZAMD_JOB(12)
 #undef ZAMD_JOB
 		default: break;
@@ -1278,7 +1333,11 @@ int launch_resize_t(const ResizeArg &a, void *stream)
 	hipStream_t s = static_cast<hipStream_t>(stream);
 	switch (a.fwh) {
 	case 0: hipLaunchKernelGGL(k_resize<WORD>, grid, dim3(kThreads), lds, s, a); break;
-#define ZAMD_TILED(n) case n: hipLaunchKernelGGL((k_resize_tiled<WORD, n>), grid, dim3(kThreads), lds, s, a); break;
+#define ZAMD_TILED(n) \
+	case n: \
+		if (a.h_first) hipLaunchKernelGGL((k_resize_tiled<WORD, n, true>), grid, dim3(kThreads), lds, s, a); \
+		else hipLaunchKernelGGL((k_resize_tiled<WORD, n, false>), grid, dim3(kThreads), lds, s, a); \
+		break;
 	ZAMD_TILED(2) ZAMD_TILED(3) ZAMD_TILED(4) ZAMD_TILED(5) ZAMD_TILED(6)
 	ZAMD_TILED(8) ZAMD_TILED(12) ZAMD_TILED(16) ZAMD_TILED(24) ZAMD_TILED(32)
 #undef ZAMD_TILED
