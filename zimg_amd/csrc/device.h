@@ -73,6 +73,9 @@ public:
 	};
 
 	explicit DeviceProgram(const Plan &plan);
+	~DeviceProgram();
+	DeviceProgram(const DeviceProgram &) = delete;
+	DeviceProgram &operator=(const DeviceProgram &) = delete;
 
 	size_t tmp_size(unsigned frames) const;
 	// FLOAT resize arithmetic: 0 the reference's FP32 order (default), 1 matrix

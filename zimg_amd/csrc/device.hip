@@ -58,6 +58,16 @@ DeviceProgram::DeviceProgram(const Plan &plan) : m_plan(plan)
 	m_slots = std::move(live);
 }
 
+DeviceProgram::~DeviceProgram()
+{
+	for (auto &kv : m_tables)
+		if (kv.second && kv.second->mem)
+			(void)hipFree(kv.second->mem);
+	for (auto &kv : m_mfma_tables)
+		if (kv.second && kv.second->mem)
+			(void)hipFree(kv.second->mem);
+}
+
 void DeviceProgram::lower()
 {
 	const auto &nodes = m_plan.nodes;
