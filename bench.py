@@ -243,12 +243,12 @@ def traffic_of(kernel_key):
     return None
 
 
-def roofline(m, workload, frames):
+def roofline(m, workload, frames, precision="exact"):
     """Dominant device stage: algorithmic bytes per launch / its average launch
     duration (HIP events on the launch stream); whole-graph figure alongside."""
     dom = max(m["stages"], key=lambda st: st["ms"])
     achieved = dom["gbs"]
-    traffic = traffic_of(f"{workload}_dominant")
+    traffic = traffic_of(f"{workload}_dominant" if precision == "exact" else f"{workload}_{precision}_dominant")
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
             "traffic": traffic, "kernel": dom["stage"], "kernel_ms_per_launch": round(dom["ms"], 4),
             "algorithmic_bytes_per_launch": dom["bytes_per_frame"] * frames,
@@ -301,7 +301,7 @@ def main():
         "dtype": m["dtype"],
         "data": "synthetic (seeded, zimg_amd/workloads.py), inputs resident in HBM",
         "config": {"workload": primary, "frames_per_step": args.frames, "graph_launches": m["launches"], "precision": args.precision},
-        "roofline": roofline(m, primary, args.frames),
+        "roofline": roofline(m, primary, args.frames, args.precision),
         "cpu_baseline": cpu,
         "secondary": None if not m2 else {
             "workload": secondary, "fps": round(m2["fps"], 1), "achieved_gbs": round(m2["achieved_gbs"], 1),
