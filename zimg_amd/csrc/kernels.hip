@@ -745,17 +745,37 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 					const uint32_t *m = mid + ((vtop[i] - r0) / 2) * pm + j4;
 					const uint32_t *cv = vcoef + i * vstride;
 					int32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-					for (unsigned k = 0; k < vtaps; k += 4) {
-						const uint4 c4 = *reinterpret_cast<const uint4 *>(cv + k);
-						const uint32_t cc[4] = { c4.x, c4.y, c4.z, c4.w };
+					if (vtaps <= static_cast<unsigned>(N)) {
+						// N tap pairs unrolled, all LDS reads issued first; pairs past
+						// vtaps get a zero tap over the last row (adds exactly 0).
+						uint4 mm[N];
+						int32_t cc[N];
 #pragma unroll
-						for (unsigned u = 0; u < 4; ++u) {
-							if (k + u < vtaps) {
-								const uint4 mm = *reinterpret_cast<const uint4 *>(m + (k + u) * pm);
-								a0 = dot2(mm.x, static_cast<int32_t>(cc[u]), a0);
-								a1 = dot2(mm.y, static_cast<int32_t>(cc[u]), a1);
-								a2 = dot2(mm.z, static_cast<int32_t>(cc[u]), a2);
-								a3 = dot2(mm.w, static_cast<int32_t>(cc[u]), a3);
+						for (int k = 0; k < N; ++k) {
+							const unsigned kk = min(static_cast<unsigned>(k), vtaps - 1);
+							mm[k] = *reinterpret_cast<const uint4 *>(m + kk * pm);
+							cc[k] = static_cast<unsigned>(k) < vtaps ? static_cast<int32_t>(cv[kk]) : 0;
+						}
+#pragma unroll
+						for (int k = 0; k < N; ++k) {
+							a0 = dot2(mm[k].x, cc[k], a0);
+							a1 = dot2(mm[k].y, cc[k], a1);
+							a2 = dot2(mm[k].z, cc[k], a2);
+							a3 = dot2(mm[k].w, cc[k], a3);
+						}
+					} else {
+						for (unsigned k = 0; k < vtaps; k += 4) {
+							const uint4 c4 = *reinterpret_cast<const uint4 *>(cv + k);
+							const uint32_t cc[4] = { c4.x, c4.y, c4.z, c4.w };
+#pragma unroll
+							for (unsigned u = 0; u < 4; ++u) {
+								if (k + u < vtaps) {
+									const uint4 mm = *reinterpret_cast<const uint4 *>(m + (k + u) * pm);
+									a0 = dot2(mm.x, static_cast<int32_t>(cc[u]), a0);
+									a1 = dot2(mm.y, static_cast<int32_t>(cc[u]), a1);
+									a2 = dot2(mm.z, static_cast<int32_t>(cc[u]), a2);
+									a3 = dot2(mm.w, static_cast<int32_t>(cc[u]), a3);
+								}
 							}
 						}
 					}
@@ -797,13 +817,37 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 					const float *m = mid + (vtop[i] - r0) * pm + j4;
 					const uint32_t *cv = vcoef + i * vstride;
 					float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
-					for (unsigned k = 0; k < vtaps; ++k) {
-						const float4 mm = *reinterpret_cast<const float4 *>(m + k * pm);
-						const float c = __uint_as_float(cv[k]);
-						a0 += c * mm.x;
-						a1 += c * mm.y;
-						a2 += c * mm.z;
-						a3 += c * mm.w;
+					if (vtaps <= static_cast<unsigned>(N)) {
+						// N taps unrolled, all LDS reads issued first; taps past vtaps
+						// leave the sums as they are (select), so every kept addition
+						// is the reference's, in its order.
+						float4 mm[N];
+						float cc[N];
+#pragma unroll
+						for (int k = 0; k < N; ++k) {
+							const unsigned kk = min(static_cast<unsigned>(k), vtaps - 1);
+							mm[k] = *reinterpret_cast<const float4 *>(m + kk * pm);
+							cc[k] = __uint_as_float(cv[kk]);
+						}
+#pragma unroll
+						for (int k = 0; k < N; ++k) {
+							const bool on = static_cast<unsigned>(k) < vtaps;
+							const float s0 = a0 + cc[k] * mm[k].x, s1 = a1 + cc[k] * mm[k].y;
+							const float s2 = a2 + cc[k] * mm[k].z, s3 = a3 + cc[k] * mm[k].w;
+							a0 = on ? s0 : a0;
+							a1 = on ? s1 : a1;
+							a2 = on ? s2 : a2;
+							a3 = on ? s3 : a3;
+						}
+					} else {
+						for (unsigned k = 0; k < vtaps; ++k) {
+							const float4 mm = *reinterpret_cast<const float4 *>(m + k * pm);
+							const float c = __uint_as_float(cv[k]);
+							a0 += c * mm.x;
+							a1 += c * mm.y;
+							a2 += c * mm.z;
+							a3 += c * mm.w;
+						}
 					}
 					char *drow = row_ptr(dst, a.dst.stride, row);
 					if (wide) {
@@ -876,11 +920,28 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 				if constexpr (WORD) {
 					const uint32_t *xw = reinterpret_cast<const uint32_t *>(xin) + top * (pin / 2) + q;
 					int32_t a0 = 0, a1 = 0;
-					for (unsigned k = 0; k < vtaps; ++k) {
-						const uint32_t w = xw[k * (pin / 2)];
-						const int32_t c = static_cast<int32_t>(cv[k]);
-						a0 += c * static_cast<int32_t>(static_cast<int16_t>(w & 0xFFFFu));
-						a1 += c * (static_cast<int32_t>(w) >> 16);
+					if (vtaps <= static_cast<unsigned>(N)) {
+						// Unrolled over N taps (zero taps past vtaps add exactly 0).
+						uint32_t wv[N];
+						int32_t cc[N];
+#pragma unroll
+						for (int k = 0; k < N; ++k) {
+							const unsigned kk = min(static_cast<unsigned>(k), vtaps - 1);
+							wv[k] = xw[kk * (pin / 2)];
+							cc[k] = static_cast<unsigned>(k) < vtaps ? static_cast<int32_t>(cv[kk]) : 0;
+						}
+#pragma unroll
+						for (int k = 0; k < N; ++k) {
+							a0 += cc[k] * static_cast<int32_t>(static_cast<int16_t>(wv[k] & 0xFFFFu));
+							a1 += cc[k] * (static_cast<int32_t>(wv[k]) >> 16);
+						}
+					} else {
+						for (unsigned k = 0; k < vtaps; ++k) {
+							const uint32_t w = xw[k * (pin / 2)];
+							const int32_t c = static_cast<int32_t>(cv[k]);
+							a0 += c * static_cast<int32_t>(static_cast<int16_t>(w & 0xFFFFu));
+							a1 += c * (static_cast<int32_t>(w) >> 16);
+						}
 					}
 					if (a.passes == 2) {
 						reinterpret_cast<uint32_t *>(mid_base)[i * (pm / 2) + q] = pack16x2(pack_biased(a0, a.pixel_max), pack_biased(a1, a.pixel_max));
@@ -891,11 +952,30 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 				} else {
 					const float *xr = xin + top * pin + 2 * q;
 					float a0 = 0.0f, a1 = 0.0f;
-					for (unsigned k = 0; k < vtaps; ++k) {
-						const float2 m = *reinterpret_cast<const float2 *>(xr + k * pin);
-						const float c = __uint_as_float(cv[k]);
-						a0 += c * m.x;
-						a1 += c * m.y;
+					if (vtaps <= static_cast<unsigned>(N)) {
+						// Unrolled over N taps; taps past vtaps leave the sums as they are.
+						float2 mv[N];
+						float cc[N];
+#pragma unroll
+						for (int k = 0; k < N; ++k) {
+							const unsigned kk = min(static_cast<unsigned>(k), vtaps - 1);
+							mv[k] = *reinterpret_cast<const float2 *>(xr + kk * pin);
+							cc[k] = __uint_as_float(cv[kk]);
+						}
+#pragma unroll
+						for (int k = 0; k < N; ++k) {
+							const bool on = static_cast<unsigned>(k) < vtaps;
+							const float s0 = a0 + cc[k] * mv[k].x, s1 = a1 + cc[k] * mv[k].y;
+							a0 = on ? s0 : a0;
+							a1 = on ? s1 : a1;
+						}
+					} else {
+						for (unsigned k = 0; k < vtaps; ++k) {
+							const float2 m = *reinterpret_cast<const float2 *>(xr + k * pin);
+							const float c = __uint_as_float(cv[k]);
+							a0 += c * m.x;
+							a1 += c * m.y;
+						}
 					}
 					if (a.passes == 2) {
 						*reinterpret_cast<float2 *>(reinterpret_cast<float *>(mid_base) + i * pm + 2 * q) = make_float2(a0, a1);
