@@ -402,6 +402,20 @@ __device__ __forceinline__ uint32_t pack_biased(int32_t acc, int32_t pixel_max)
 	return static_cast<uint32_t>(med3_i32((acc + (1 << 13)) >> 14, -32768, pixel_max - 32768));
 }
 
+// Accumulators of the tiled kernels start at the rounding terms instead of
+// 0 (the int32 sum is the same, so are the results): kBiasedInit for
+// pack_biased, kWordInit for pack_word; the *_pre forms then skip the add.
+constexpr int32_t kBiasedInit = 1 << 13;
+constexpr int32_t kWordInit = (32768 << 14) + (1 << 13);
+__device__ __forceinline__ uint32_t pack_biased_pre(int32_t acc, int32_t pixel_max)
+{
+	return static_cast<uint32_t>(med3_i32(acc >> 14, -32768, pixel_max - 32768));
+}
+__device__ __forceinline__ uint16_t pack_word_pre(int32_t acc, int32_t pixel_max)
+{
+	return static_cast<uint16_t>(med3_i32(acc >> 14, 0, pixel_max));
+}
+
 // Shift every 16-bit half of w left by s (s < 16), then bias to int16.
 __device__ __forceinline__ uint32_t shift_bias2(uint32_t w, unsigned s)
 {
@@ -560,9 +574,10 @@ struct HRegs {
 		}
 	}
 
-	__device__ __forceinline__ void apply_word(const uint32_t *row, int32_t &a0, int32_t &a1) const
+	// init: the accumulators' start value (0, kBiasedInit or kWordInit).
+	__device__ __forceinline__ void apply_word(const uint32_t *row, int32_t &a0, int32_t &a1, int32_t init) const
 	{
-		int32_t s0 = 0, s1 = 0;
+		int32_t s0 = init, s1 = init;
 #pragma unroll
 		for (int k = 0; k < N; ++k) {
 			s0 = dot2(row[l0 + k], c0[k], s0);
@@ -728,10 +743,10 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 				const unsigned pm = a.lds_mid_cols;
 				for (unsigned r = 0; r < span; r += 2) {
 					int32_t e0, e1, o0, o1;
-					h.apply_word(xw + r * pw, e0, e1);
-					h.apply_word(xw + (r + 1) * pw, o0, o1);
-					const uint32_t m0 = pack16x2(pack_biased(e0, a.pixel_max), pack_biased(o0, a.pixel_max));
-					const uint32_t m1 = pack16x2(pack_biased(e1, a.pixel_max), pack_biased(o1, a.pixel_max));
+					h.apply_word(xw + r * pw, e0, e1, kBiasedInit);
+					h.apply_word(xw + (r + 1) * pw, o0, o1, kBiasedInit);
+					const uint32_t m0 = pack16x2(pack_biased_pre(e0, a.pixel_max), pack_biased_pre(o0, a.pixel_max));
+					const uint32_t m1 = pack16x2(pack_biased_pre(e1, a.pixel_max), pack_biased_pre(o1, a.pixel_max));
 					*reinterpret_cast<uint2 *>(mid + (r / 2) * pm + jc) = make_uint2(m0, m1);
 				}
 				__syncthreads();
@@ -744,7 +759,7 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 					const unsigned row = i0 + i;
 					const uint32_t *m = mid + ((vtop[i] - r0) / 2) * pm + j4;
 					const uint32_t *cv = vcoef + i * vstride;
-					int32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+					int32_t a0 = kWordInit, a1 = kWordInit, a2 = kWordInit, a3 = kWordInit;
 					if (vtaps <= static_cast<unsigned>(N)) {
 						// N tap pairs unrolled, all LDS reads issued first; pairs past
 						// vtaps get a zero tap over the last row (adds exactly 0).
@@ -780,8 +795,8 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 						}
 					}
 					char *drow = row_ptr(dst, a.dst.stride, row);
-					const uint16_t v0 = pack_word(a0, a.pixel_max), v1 = pack_word(a1, a.pixel_max);
-					const uint16_t v2 = pack_word(a2, a.pixel_max), v3 = pack_word(a3, a.pixel_max);
+					const uint16_t v0 = pack_word_pre(a0, a.pixel_max), v1 = pack_word_pre(a1, a.pixel_max);
+					const uint16_t v2 = pack_word_pre(a2, a.pixel_max), v3 = pack_word_pre(a3, a.pixel_max);
 					if (wide) {
 						*reinterpret_cast<uint2 *>(drow + 2 * static_cast<size_t>(j0 + j4)) = make_uint2(pack16x2(v0, v1), pack16x2(v2, v3));
 					} else if (n4) {
@@ -793,9 +808,9 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 			} else {
 				for (unsigned r = 0; r < span; ++r) {
 					int32_t a0, a1;
-					h.apply_word(xw + r * pw, a0, a1);
+					h.apply_word(xw + r * pw, a0, a1, kWordInit);
 					if (active)
-						store_pair<true>(row_ptr(dst, a.dst.stride, r0 + r), j, pack_word(a0, a.pixel_max), pack_word(a1, a.pixel_max), two, a.out, r0 + r);
+						store_pair<true>(row_ptr(dst, a.dst.stride, r0 + r), j, pack_word_pre(a0, a.pixel_max), pack_word_pre(a1, a.pixel_max), two, a.out, r0 + r);
 				}
 			}
 		} else {
@@ -919,7 +934,8 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 				T v0, v1;
 				if constexpr (WORD) {
 					const uint32_t *xw = reinterpret_cast<const uint32_t *>(xin) + top * (pin / 2) + q;
-					int32_t a0 = 0, a1 = 0;
+					const int32_t init = a.passes == 2 ? kBiasedInit : kWordInit;
+					int32_t a0 = init, a1 = init;
 					if (vtaps <= static_cast<unsigned>(N)) {
 						// Unrolled over N taps (zero taps past vtaps add exactly 0).
 						uint32_t wv[N];
@@ -944,11 +960,11 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 						}
 					}
 					if (a.passes == 2) {
-						reinterpret_cast<uint32_t *>(mid_base)[i * (pm / 2) + q] = pack16x2(pack_biased(a0, a.pixel_max), pack_biased(a1, a.pixel_max));
+						reinterpret_cast<uint32_t *>(mid_base)[i * (pm / 2) + q] = pack16x2(pack_biased_pre(a0, a.pixel_max), pack_biased_pre(a1, a.pixel_max));
 						continue;
 					}
-					v0 = pack_word(a0, a.pixel_max);
-					v1 = pack_word(a1, a.pixel_max);
+					v0 = pack_word_pre(a0, a.pixel_max);
+					v1 = pack_word_pre(a1, a.pixel_max);
 				} else {
 					const float *xr = xin + top * pin + 2 * q;
 					float a0 = 0.0f, a1 = 0.0f;
@@ -1001,8 +1017,8 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 		const unsigned row = i0 + i;
 		if constexpr (WORD) {
 			int32_t a0, a1;
-			h.apply_word(reinterpret_cast<const uint32_t *>(mid_base) + i * (pm / 2), a0, a1);
-			store_pair<true>(row_ptr(dst, a.dst.stride, row), oj, pack_word(a0, a.pixel_max), pack_word(a1, a.pixel_max), otwo, a.out, row);
+			h.apply_word(reinterpret_cast<const uint32_t *>(mid_base) + i * (pm / 2), a0, a1, kWordInit);
+			store_pair<true>(row_ptr(dst, a.dst.stride, row), oj, pack_word_pre(a0, a.pixel_max), pack_word_pre(a1, a.pixel_max), otwo, a.out, row);
 		} else {
 			float v0, v1;
 			h.apply_float(reinterpret_cast<const float *>(mid_base) + i * pm, a.p2.fw, v0, v1);
