@@ -60,6 +60,32 @@ def test_mfma_precision_form_error(case):
     print(json.dumps(report), flush=True)
 
 
+@pytest.mark.parametrize("src_half,dst_half,pad", [(True, False, 0), (False, True, 0), (False, False, 4)],
+                         ids=["half-in", "half-out", "unaligned-rows"])
+def test_mfma_one_launch_general_loads_and_stores(src_half, dst_half, pad):
+    """The one-launch matrix-core form with conversions in its load or store,
+    or with rows not 16-byte aligned (its per-sample paths): within the FP16
+    hi+lo bound of the FP64 oracle applied to the same source values."""
+    sw, sh, dw, dh = 701, 333, 350, 505
+    print(f"case: precision form {sw}x{sh}->{dw}x{dh} half_in={src_half} half_out={dst_half} pad={pad}", flush=True)
+    src = W.fp32_plane(sw, sh, seed=12)
+    if src_half:
+        src = src.astype(np.float16)
+    s = z.image_format(width=sw, height=sh, pixel_type=z.PIXEL_HALF if src_half else z.PIXEL_FLOAT)
+    d = z.image_format(width=dw, height=dh, pixel_type=z.PIXEL_HALF if dst_half else z.PIXEL_FLOAT)
+    g = z.FilterGraph(s, d, z.builder_params(resample_filter=z.RESIZE_LANCZOS, filter_param_a=3.0))
+    g.set_resize_precision(z.RESIZE_PRECISION_MFMA_FP16)
+    assert "one launch" in g.describe()
+    (out, *_) = gpu_util.run_graph(g, [src, None, None, None], [(dh, dw, np.float16 if dst_half else np.float32), None, None, None],
+                                   pad_bytes=pad)
+    f64 = resize_plane_f64(src.astype(np.float64), LANCZOS, 3.0, 0.0, dw, dh)
+    err = np.max(np.abs(out.astype(np.float64) - f64))
+    # HALF output adds its own rounding: half an ulp, 2^-11 for values in [1, 2)
+    # (Lanczos overshoot reaches past 1).
+    bound = 5.0e-4 if dst_half else BOUND[z.RESIZE_PRECISION_MFMA_FP16]
+    assert err < bound, err
+
+
 def test_mfma_form_keeps_exact_integer_and_dithered_paths():
     """WORD resizes stay on the exact integer path whatever the setting."""
     src = W.word_plane(640, 360, seed=4)
