@@ -129,6 +129,7 @@ private:
 	struct FusedPlan {
 		bool ok = false;
 		unsigned tile_h = 0, slab_stride = 0, mid_stride = 0, coef_blocks = 0, lds = 0;
+		unsigned lds_staged = 0; // LDS when the output tile is staged through LDS (0: not possible)
 	};
 	FusedPlan mfma_fused_plan(const Step &s) const;
 	size_t mfma_mid_bytes(const Step &s, unsigned frames) const;

@@ -688,6 +688,9 @@ DeviceProgram::FusedPlan DeviceProgram::mfma_fused_plan(const Step &s) const
 			p.mid_stride = sr;
 			p.coef_blocks = blocks;
 			p.lds = lds;
+			// Output tile through LDS: tile_h rows of 16 kFusedWaves + 4 floats after the slabs.
+			const unsigned staged = std::max(lds, 2 * 4 * 16 * sc + tile_h * (16 * dev::kFusedWaves + 4) * 4);
+			p.lds_staged = staged <= 80 * 1024 ? staged : 0;
 			return p;
 		}
 	}
@@ -839,6 +842,9 @@ void DeviceProgram::run_mfma_resize(const Step &s, const dev::PlaneArg &src, con
 		a.lds_bytes = fp.lds;
 		a.vec_in = (s.ix[0].type == 3 && s.ix[0].op == 0 && aligned16(src)) ? 1 : 0;
 		a.vec_out = (s.ox[0].type == 3 && s.ox[0].op == 0 && aligned16(dst)) ? 1 : 0;
+		a.stage_out = (a.vec_in && a.vec_out && fp.lds_staged) ? 1 : 0;
+		if (a.stage_out)
+			a.lds_bytes = fp.lds_staged;
 		a.th = mt.pass.at(s.tab[s.h_first ? 0 : 1]);
 		a.tv = mt.pass.at(s.tab[s.h_first ? 1 : 0]);
 		a.in = s.ix[0];

@@ -156,6 +156,7 @@ struct MfmaFusedArg {
 	unsigned lds_bytes;
 	int vec_in;            // FLOAT source without conversion, 16-byte aligned rows
 	int vec_out;           // FLOAT output stored as is, 16-byte aligned rows
+	int stage_out;         // with vec_in and vec_out: the output tile is written through LDS (tile_h rows of 68 floats after the slabs)
 	MfmaTab th, tv;        // horizontal (src_w -> dst_w) and vertical (src_h -> dst_h) tables
 	InXform in;
 	OutXform out;

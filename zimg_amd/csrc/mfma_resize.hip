@@ -450,7 +450,7 @@ __global__ void __launch_bounds__(64 * kFusedWaves) __attribute__((amdgpu_waves_
 	} } } } } } } } } } }
 	static_assert(kFusedMaxGroups == 12, "one nested block per row group");
 	__syncthreads();
-	if (!active)
+	if (!active && !(FAST && a.stage_out))
 		return;
 
 	const uint16_t *mh = mid_hi + (16 * w + (lane & 15)) * sr;
@@ -459,7 +459,7 @@ __global__ void __launch_bounds__(64 * kFusedWaves) __attribute__((amdgpu_waves_
 #pragma unroll
 	for (unsigned i = 0; i < kFusedMaxRowTiles; ++i) {
 		acc[i] = f4{ 0.0f, 0.0f, 0.0f, 0.0f };
-		if (t2first + i > t2last)
+		if (!active || t2first + i > t2last)
 			continue;
 		const unsigned off2 = static_cast<unsigned>(v2[i].kb - r0);
 		const uint4 *cb = coef2 + (v2[i].off - cb0) * 128;
@@ -473,6 +473,38 @@ __global__ void __launch_bounds__(64 * kFusedWaves) __attribute__((amdgpu_waves_
 			acc[i] = S::mma(xh, cl, acc[i]);
 			acc[i] = S::mma(xl, ch, acc[i]);
 		}
+	}
+	if (FAST && a.stage_out) {
+		// The workgroup's output tile goes through LDS (rows of 64 floats at a
+		// stride of 68) so that each store instruction writes whole 256-byte
+		// row segments instead of 16 rows x 64 bytes.
+		constexpr unsigned kRow = 16 * kFusedWaves + 4;
+		float *ob = reinterpret_cast<float *>(mid_hi);
+		__syncthreads();
+		if (active) {
+#pragma unroll
+			for (unsigned i = 0; i < kFusedMaxRowTiles; ++i) {
+				if (t2first + i <= t2last)
+					*reinterpret_cast<f4 *>(ob + (16 * i + (lane & 15)) * kRow + 16 * w + 4 * (lane >> 4)) = acc[i];
+			}
+		}
+		__syncthreads();
+		const unsigned row0 = 16 * t2first;
+		const unsigned nrows = min(a.tile_h, a.dst_h - row0);
+		const unsigned col = 16 * t1first + 4 * (tid & 15);
+		for (unsigned r = tid >> 4; r < nrows; r += 64 * kFusedWaves / 16) {
+			const f4 v = *reinterpret_cast<const f4 *>(ob + r * kRow + 4 * (tid & 15));
+			char *drow = dst + static_cast<long long>(row0 + r) * a.dst.stride;
+			if (col + 4 <= a.dst_w) {
+				*reinterpret_cast<float4 *>(drow + 4 * static_cast<size_t>(col)) = make_float4(v[0], v[1], v[2], v[3]);
+			} else {
+#pragma unroll
+				for (int k = 0; k < 4; ++k)
+					if (col + k < a.dst_w)
+						reinterpret_cast<float *>(drow)[col + k] = v[k];
+			}
+		}
+		return;
 	}
 	const unsigned c0 = 16 * t1 + 4 * (lane >> 4);
 #pragma unroll
