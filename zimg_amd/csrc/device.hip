@@ -838,7 +838,6 @@ void DeviceProgram::run_mfma_resize(const Step &s, const dev::PlaneArg &src, con
 		a.tile_h = fp.tile_h;
 		a.slab_stride = fp.slab_stride;
 		a.mid_stride = fp.mid_stride;
-		a.coef_blocks = fp.coef_blocks;
 		a.lds_bytes = fp.lds;
 		a.vec_in = (s.ix[0].type == 3 && s.ix[0].op == 0 && aligned16(src)) ? 1 : 0;
 		a.vec_out = (s.ox[0].type == 3 && s.ox[0].op == 0 && aligned16(dst)) ? 1 : 0;

@@ -152,7 +152,6 @@ struct MfmaFusedArg {
 	unsigned tile_h;       // output rows per workgroup, a multiple of 16
 	unsigned slab_stride;  // halves, >= the widest source window
 	unsigned mid_stride;   // halves, >= 16 x the most intermediate row groups
-	unsigned coef_blocks;  // LDS room for the vertical coefficient blocks of one workgroup
 	unsigned lds_bytes;
 	int vec_in;            // FLOAT source without conversion, 16-byte aligned rows
 	int vec_out;           // FLOAT output stored as is, 16-byte aligned rows
