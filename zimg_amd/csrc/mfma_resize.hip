@@ -263,7 +263,9 @@ __device__ __forceinline__ uint4 lds_run(const uint16_t *base, unsigned run, boo
 // workgroup): A = the transposed intermediate (16 columns x 32 rows, 16-byte
 // LDS runs along a column), B = the vertical coefficient block (copied to
 // LDS at the start); the result lane holds columns 4 (l / 16) + r of output
-// row l % 16. All output tiles accumulate in registers; the stores come last.
+// row l % 16. All output tiles accumulate in registers; the stores come last
+// (FLOAT output goes through an LDS copy of the workgroup's tile, so a store
+// instruction writes whole 256-byte row segments).
 template <bool BF16, bool FAST>
 __global__ void __launch_bounds__(64 * kFusedWaves) __attribute__((amdgpu_waves_per_eu(3, 8))) k_mfma_fused(MfmaFusedArg a)
 {
