@@ -13,20 +13,21 @@ def _torch():
     return torch
 
 
-def _upload(torch, arr, pad_bytes):
+def _upload(torch, arr, pad_bytes, pitch_align=4):
     """arr: (F, h, w) numpy -> (tensor, pitch, frame_stride)."""
     F, h, w = arr.shape
     rowbytes = w * arr.itemsize
-    pitch = (rowbytes + pad_bytes + 3) // 4 * 4
+    pitch = (rowbytes + pad_bytes + pitch_align - 1) // pitch_align * pitch_align
     host = np.zeros((F, h, pitch), np.uint8)
     host[:, :, :rowbytes] = arr.view(np.uint8).reshape(F, h, rowbytes)
     t = torch.from_numpy(host.reshape(-1)).to("cuda")
     return t, pitch, h * pitch
 
 
-def run_graph(graph, src_planes, dst_specs, frames=None, pad_bytes=0, stream=0):
+def run_graph(graph, src_planes, dst_specs, frames=None, pad_bytes=0, stream=0, pitch_align=4):
     """src_planes: list of 4 (numpy (h,w) or (F,h,w), or None).
-    dst_specs: list of 4 ((h, w, dtype) or None). Returns list of numpy outputs."""
+    dst_specs: list of 4 ((h, w, dtype) or None). Row pitches are rounded up to
+    pitch_align bytes. Returns list of numpy outputs."""
     torch = _torch()
     keep = []
     src_bufs, src_fs = [None] * 4, [0] * 4
@@ -36,7 +37,7 @@ def run_graph(graph, src_planes, dst_specs, frames=None, pad_bytes=0, stream=0):
             continue
         a3 = a if a.ndim == 3 else a[None]
         F = a3.shape[0]
-        t, pitch, fs = _upload(torch, np.ascontiguousarray(a3), pad_bytes)
+        t, pitch, fs = _upload(torch, np.ascontiguousarray(a3), pad_bytes, pitch_align)
         keep.append(t)
         src_bufs[p] = (t.data_ptr(), pitch)
         src_fs[p] = fs
@@ -48,7 +49,7 @@ def run_graph(graph, src_planes, dst_specs, frames=None, pad_bytes=0, stream=0):
             continue
         h, w, dt = spec
         rowbytes = w * np.dtype(dt).itemsize
-        pitch = (rowbytes + pad_bytes + 3) // 4 * 4
+        pitch = (rowbytes + pad_bytes + pitch_align - 1) // pitch_align * pitch_align
         t = torch.full((frames * h * pitch,), 0xAB, dtype=torch.uint8, device="cuda")
         dst_t[p] = (t, h, w, dt, pitch)
         dst_bufs[p] = (t.data_ptr(), pitch)

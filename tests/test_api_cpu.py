@@ -191,8 +191,24 @@ def test_single_plane_lanczos_is_one_fused_launch():
     dst = z.image_format(width=3840, height=2160, pixel_type=z.PIXEL_WORD, depth=16)
     g = z.FilterGraph(src, dst, z.builder_params(resample_filter=z.RESIZE_LANCZOS, filter_param_a=3))
     dev = [l for l in g.describe().split("device:\n")[1].splitlines() if l]
-    assert len(dev) == 1 and "passes=2 h_first" in dev[0]
+    launches = [l for l in dev if not l.startswith(" ")]
+    assert len(launches) == 1 and "passes=2 h_first" in launches[0]
+    # both passes on the int8 matrix cores in one launch, no HBM intermediate
+    assert any(l.startswith("  integer matrix cores: one launch") for l in dev)
     assert g.device_tmp_size(8) == 0
+
+
+def test_integer_matrix_core_form_only_where_windows_fit():
+    def dev(sw, sh, dw, dh, ft=z.RESIZE_LANCZOS, a=3.0, b=float("nan"), pt=z.PIXEL_WORD, dpt=z.PIXEL_WORD, depth=16):
+        src = z.image_format(width=sw, height=sh, pixel_type=pt, depth=depth if pt == z.PIXEL_WORD else 8)
+        dst = z.image_format(width=dw, height=dh, pixel_type=dpt, depth=depth if dpt == z.PIXEL_WORD else 8)
+        return z.FilterGraph(src, dst, z.builder_params(resample_filter=ft, filter_param_a=a, filter_param_b=b)).describe()
+    assert "integer matrix cores" in dev(641, 359, 1283, 719, depth=10)
+    assert "integer matrix cores" in dev(1000, 700, 1500, 1050, z.RESIZE_BICUBIC, 0.0, 0.5)
+    assert "integer matrix cores" not in dev(640, 480, 320, 240)           # downscale: windows too wide
+    assert "integer matrix cores" not in dev(1000, 700, 1400, 1050)        # 1.4x Lanczos3: 17-sample windows
+    assert "integer matrix cores" not in dev(640, 480, 1280, 480)          # one pass
+    assert "integer matrix cores" not in dev(640, 480, 1280, 960, pt=z.PIXEL_FLOAT, dpt=z.PIXEL_FLOAT)
 
 
 def test_tiled_resize_configurations_in_bounds():

@@ -68,6 +68,29 @@ def test_word_resize_grid(sw, sh, dw, dh, ft, a, b, depth):
         _assert_equal(out, P.word_resize(src, dw, dh, ft, a, b, depth), f"word resize pad={pad}")
 
 
+IMMA_CASES = [
+    # (sw, sh, dw, dh, filter, a, b, depth, frames)
+    (641, 359, 1283, 719, LANCZOS, 3.0, 0.0, 10, 1),   # ragged: partial column blocks and row tiles
+    (1000, 700, 1500, 1050, BICUBIC, 0.0, 0.5, 16, 2),  # 1.5x: widest windows that fit
+    (500, 400, 1999, 1601, SPLINE36, 0.0, 0.0, 12, 1),
+    (96, 64, 1920, 1080, SPLINE64, 0.0, 0.0, 16, 1),
+    (17, 9, 40, 33, BILINEAR, 0.0, 0.0, 16, 3),         # smaller than one tile
+]
+
+
+@pytest.mark.parametrize("sw,sh,dw,dh,ft,a,b,depth,frames", IMMA_CASES)
+def test_word_resize_integer_matrix_cores(sw, sh, dw, dh, ft, a, b, depth, frames):
+    """Upscales whose 16-output windows fit run on the int8 matrix cores; bit-exact."""
+    print(f"case: WORD int8 matrix cores {sw}x{sh}->{dw}x{dh} filter {ft} depth {depth} frames {frames}", flush=True)
+    g = _word_graph(sw, sh, dw, dh, ft, a, b if ft == BICUBIC else float("nan"), depth)
+    assert "integer matrix cores" in g.describe()
+    src = np.stack([(W.word_plane(sw, sh, seed=sw + 13 * k) >> (16 - depth)).astype(np.uint16) for k in range(frames)])
+    out, *_ = gpu_util.run_graph(g, [src, None, None, None], [(dh, dw, np.uint16), None, None, None], pad_bytes=32, pitch_align=16)
+    out = out if frames > 1 else out[None]
+    for k in range(frames):
+        _assert_equal(out[k], P.word_resize(src[k], dw, dh, ft, a, b, depth), f"frame {k}")
+
+
 FLOAT_CASES = [
     (3840, 2160, 1920, 1080, LANCZOS, 3.0, 0.0),
     (1280, 720, 3840, 2160, BICUBIC, 0.0, 0.5),

@@ -28,6 +28,13 @@ struct TiledConfig {
 	double cost = 0;
 };
 
+// Launch shape of the integer matrix-core form of a WORD resize (ok ==
+// false: the step is not eligible and runs on the tiled kernel).
+struct ImmaPlan {
+	bool ok = false;
+	unsigned rb = 0, mb = 0, mid_pitch = 0, lds = 0;
+};
+
 class DeviceProgram {
 public:
 	struct Buf {
@@ -54,6 +61,7 @@ public:
 		// best candidates by the cost model, cheapest first (for the autotuner)
 		TiledConfig fast{};
 		std::vector<TiledConfig> fast_options;
+		ImmaPlan imma{};
 		// POINT
 		unsigned crop_left = 0, crop_top = 0;
 		int fill = 0;
@@ -121,6 +129,13 @@ private:
 	mutable std::mutex m_mutex;
 	mutable std::map<int, std::shared_ptr<Tables>> m_tables; // by HIP device
 	mutable std::map<int, std::shared_ptr<MfmaTables>> m_mfma_tables; // by HIP device * 4 + mode
+	struct ImmaTables {
+		void *mem = nullptr;
+		std::map<int, const dev::ImmaBlock *> pass; // by 2 x node id + pass (0 horizontal, 1 vertical)
+	};
+	mutable std::map<int, std::shared_ptr<ImmaTables>> m_imma_tables; // by HIP device
+	const ImmaTables &imma_tables_for_current_device() const;
+	ImmaPlan imma_plan(const Step &s) const;
 	int m_precision = 0;
 
 	bool mfma_step(const Step &s) const { return m_precision && s.kind == Step::RESIZE && !s.word && s.ox[0].dither == 0; }

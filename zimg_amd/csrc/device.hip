@@ -37,8 +37,10 @@ DeviceProgram::DeviceProgram(const Plan &plan) : m_plan(plan)
 	lower();
 	fuse();
 	for (auto &s : m_steps)
-		if (!s.dead && s.kind == Step::RESIZE)
+		if (!s.dead && s.kind == Step::RESIZE) {
 			configure_resize(s);
+			s.imma = imma_plan(s);
+		}
 	m_steps.erase(std::remove_if(m_steps.begin(), m_steps.end(), [](const Step &s) { return s.dead; }), m_steps.end());
 
 	// Drop scratch slots that fusion made unreachable and renumber the rest.
@@ -64,6 +66,9 @@ DeviceProgram::~DeviceProgram()
 		if (kv.second && kv.second->mem)
 			(void)hipFree(kv.second->mem);
 	for (auto &kv : m_mfma_tables)
+		if (kv.second && kv.second->mem)
+			(void)hipFree(kv.second->mem);
+	for (auto &kv : m_imma_tables)
 		if (kv.second && kv.second->mem)
 			(void)hipFree(kv.second->mem);
 }
@@ -650,6 +655,146 @@ unsigned odd_stride(unsigned n)
 
 } // namespace
 
+namespace {
+
+// Window starts of the 16-output blocks of a WORD pass for the integer
+// matrix-core form (starts inside [0, src_len - 16], even where possible,
+// when `even`: the horizontal pass), or an empty vector when a block
+// does not fit one 16-sample window, the starts are not monotonic, a tap
+// does not split into two signed bytes (c >= 32640) or a row's sum could
+// leave the int32 range (sum |c| * 32768 + 2^29 + 2^13 >= 2^31).
+std::vector<int> imma_windows(const FilterTable &t, bool even, unsigned src_len)
+{
+	std::vector<int> kb;
+	const unsigned fw = t.filter_width;
+	for (unsigned o = 0; o < t.filter_rows; ++o) {
+		long mag = 0;
+		for (unsigned k = 0; k < fw; ++k) {
+			const int c = t.data_i16[static_cast<size_t>(o) * fw + k];
+			if (c >= 32640)
+				return {};
+			mag += c < 0 ? -c : c;
+		}
+		if (mag > 49000)
+			return {};
+	}
+	for (unsigned o0 = 0; o0 < t.filter_rows; o0 += 16) {
+		// Horizontal windows end inside the row (they are read straight from
+		// the source) and start on an even sample unless that is impossible
+		// (the last windows of an odd-width row).
+		const int k = even ? static_cast<int>(std::min(t.left[o0] & ~1u, src_len - 16)) : static_cast<int>(t.left[o0]);
+		for (unsigned o = o0; o < std::min(t.filter_rows, o0 + 16); ++o)
+			if (static_cast<int>(t.left[o]) < k || static_cast<int>(t.left[o] + fw) > k + 16)
+				return {};
+		if (!kb.empty() && k < kb.back())
+			return {};
+		kb.push_back(k);
+	}
+	return kb;
+}
+
+void imma_blocks(const FilterTable &t, const std::vector<int> &kb, std::vector<dev::ImmaBlock> &out)
+{
+	const unsigned fw = t.filter_width;
+	for (size_t b = 0; b < kb.size(); ++b) {
+		dev::ImmaBlock blk{};
+		blk.kb = kb[b];
+		for (unsigned lane = 0; lane < 64; ++lane) {
+			const unsigned o = 16 * static_cast<unsigned>(b) + lane % 16;
+			const bool live = o < t.filter_rows;
+			int32_t sum = 0;
+			for (unsigned k = 0; live && k < fw; ++k)
+				sum += t.data_i16[static_cast<size_t>(o) * fw + k];
+			blk.init[lane] = (1 << 29) + (1 << 13) + 128 * sum;
+			uint8_t bytes[3][8];
+			for (unsigned i = 0; i < 8; ++i) {
+				const unsigned k = 8 * (lane / 16) + i;
+				const bool high = k < 16; // the xh half of the window
+				const long smp = kb[b] + static_cast<long>(k % 16);
+				int c = 0;
+				if (live && smp >= t.left[o] && smp < static_cast<long>(t.left[o] + fw))
+					c = t.data_i16[static_cast<size_t>(o) * fw + (smp - t.left[o])];
+				const int cl = static_cast<int8_t>(c & 0xFF);
+				const int ch = (c - cl) / 256;
+				bytes[0][i] = static_cast<uint8_t>(high ? ch : 0);
+				bytes[1][i] = static_cast<uint8_t>(high ? cl : ch);
+				bytes[2][i] = static_cast<uint8_t>(high ? 0 : cl);
+			}
+			for (int p = 0; p < 3; ++p)
+				std::memcpy(blk.w[p][lane], bytes[p], 8);
+		}
+		out.push_back(blk);
+	}
+}
+
+} // namespace
+
+ImmaPlan DeviceProgram::imma_plan(const Step &s) const
+{
+	ImmaPlan p;
+	if (!s.word || s.passes != 2 || !s.h_first)
+		return p;
+	if (s.ox[0].op != 0 || s.ox[0].type != 1 || s.ix[0].type != 1 || s.ix[0].op != 0 || s.in_w < 16)
+		return p;
+	const FilterTable &th = *m_plan.nodes[s.tab[0]].table;
+	const FilterTable &tv = *m_plan.nodes[s.tab[1]].table;
+	if (th.filter_rows != s.width || tv.filter_rows != s.height)
+		return p;
+	const std::vector<int> kh = imma_windows(th, true, s.in_w), kv = imma_windows(tv, false, s.in_h);
+	if (kh.empty() || kv.empty())
+		return p;
+	const unsigned nv = static_cast<unsigned>(kv.size());
+	const unsigned rb = 8;
+	unsigned mb = 0;
+	for (unsigned b0 = 0; b0 < nv; b0 += rb)
+		for (unsigned b = b0; b < std::min(nv, b0 + rb); ++b)
+			mb = std::max(mb, static_cast<unsigned>(kv[b] + 16 - kv[b0] + 15) / 16);
+	if (mb > dev::kImmaMaxMb)
+		return p;
+	p.rb = rb;
+	p.mb = mb;
+	p.mid_pitch = 16 * mb + 8; // 2 mod 4 dwords (bank spread of the kernel's reads and writes)
+	p.lds = 2 * dev::kImmaTileW * p.mid_pitch;
+	p.ok = true;
+	return p;
+}
+
+const DeviceProgram::ImmaTables &DeviceProgram::imma_tables_for_current_device() const
+{
+	int device = 0;
+	check_hip(hipGetDevice(&device), "hipGetDevice");
+	std::lock_guard<std::mutex> lock(m_mutex);
+	auto it = m_imma_tables.find(device);
+	if (it != m_imma_tables.end())
+		return *it->second;
+	std::vector<int> ids;
+	std::vector<dev::ImmaBlock> blocks;
+	std::vector<size_t> first;
+	for (auto &s : m_steps) {
+		if (s.dead || s.kind != Step::RESIZE || !s.imma.ok)
+			continue;
+		for (int q = 0; q < 2; ++q) {
+			const int key = 2 * s.tab[q] + q;
+			if (std::find(ids.begin(), ids.end(), key) != ids.end())
+				continue;
+			const FilterTable &t = *m_plan.nodes[s.tab[q]].table;
+			ids.push_back(key);
+			first.push_back(blocks.size());
+			imma_blocks(t, imma_windows(t, q == 0, q == 0 ? s.in_w : s.in_h), blocks);
+		}
+	}
+	auto tab = std::make_shared<ImmaTables>();
+	if (!blocks.empty()) {
+		const size_t bytes = blocks.size() * sizeof(dev::ImmaBlock);
+		check_hip(hipMalloc(&tab->mem, bytes), "hipMalloc(integer matrix-core tables)");
+		check_hip(hipMemcpy(tab->mem, blocks.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(integer matrix-core tables)");
+		for (size_t n = 0; n < ids.size(); ++n)
+			tab->pass[ids[n]] = static_cast<const dev::ImmaBlock *>(tab->mem) + first[n];
+	}
+	m_imma_tables[device] = tab;
+	return *tab;
+}
+
 DeviceProgram::FusedPlan DeviceProgram::mfma_fused_plan(const Step &s) const
 {
 	FusedPlan p;
@@ -1089,7 +1234,31 @@ void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void
 		}
 		switch (s.kind) {
 		case Step::RESIZE: {
-			const dev::ResizeArg a = resize_arg(s, resolve(s.in[0]), resolve(s.out[0]), frames, tabs);
+			const dev::PlaneArg in = resolve(s.in[0]), out = resolve(s.out[0]);
+			if (s.imma.ok && aligned16(in) && aligned16(out)) {
+				const ImmaTables &it = imma_tables_for_current_device();
+				dev::ImmaArg a{};
+				a.src = in;
+				a.dst = out;
+				a.frames = frames;
+				a.src_w = s.in_w;
+				a.src_h = s.in_h;
+				a.dst_w = s.width;
+				a.dst_h = s.height;
+				a.pixel_max = s.pixel_max;
+				a.in = s.ix[0];
+				a.h = it.pass.at(2 * s.tab[0]);
+				a.v = it.pass.at(2 * s.tab[1] + 1);
+				a.nh = (s.width + 15) / 16;
+				a.nv = (s.height + 15) / 16;
+				a.rb = s.imma.rb;
+				a.mb = s.imma.mb;
+				a.mid_pitch = s.imma.mid_pitch;
+				a.lds = s.imma.lds;
+				rc = dev::launch_resize_imma(a, stream);
+				break;
+			}
+			const dev::ResizeArg a = resize_arg(s, in, out, frames, tabs);
 			rc = dev::launch_resize(a, stream);
 			break;
 		}
@@ -1166,6 +1335,11 @@ std::string DeviceProgram::describe() const
 		else
 			std::snprintf(buf, sizeof(buf), "%s %ux%u: %s\n", kinds[s.kind], s.width, s.height, s.label.c_str());
 		r += buf;
+		if (s.kind == Step::RESIZE && s.imma.ok) {
+			std::snprintf(buf, sizeof(buf), "  integer matrix cores: one launch, %ux%u tiles, lds=%u\n", dev::kImmaTileW, 16 * s.imma.rb,
+			              s.imma.lds);
+			r += buf;
+		}
 		if (!s.dead && mfma_step(s)) {
 			const FusedPlan fp = mfma_fused_plan(s);
 			if (fp.ok)

@@ -74,6 +74,40 @@ constexpr int kFastTapBuckets[] = { 2, 3, 4, 5, 6, 8, 12, 16, 24, 32 };
 // Output columns of a horizontal-first tile (two per thread).
 constexpr unsigned kFastTileW = 512;
 
+// WORD resize on the int8 matrix cores (horizontal pass first, both passes
+// in one launch, bit-exact; WORD source without a depth shift, horizontal
+// windows inside the row). A biased sample x - 32768 = 256 xh + xl' + 128
+// and a Q1.14 tap c = 256 ch + cl are split into signed bytes; a 16-output
+// block of a pass is three 16x16x32 products over a 16-sample window,
+// A = [xh | xl'] against [ch | 0] (hh), [cl | ch] (mid) and [0 | cl] (ll),
+// and the int32 sum is 65536 hh + 256 mid + ll + init, with
+// init = 128 sum(c) + the rounding and unbias terms of the pack.
+// One record per block of 16 outputs: w[p][lane] holds the lane's 8
+// coefficient bytes of product p in the MFMA operand layout (lane l: output
+// l % 16, window k = 8 (l / 16) .. + 7, k < 16 the xh half), init[lane] the
+// start of the lane's accumulators, kb the window's first input sample.
+struct alignas(16) ImmaBlock {
+	uint32_t w[3][64][2];
+	int32_t init[64];
+	int32_t kb;
+	int32_t pad[3];
+};
+// Output columns of an integer matrix-core tile (8 blocks of 16, 2 per wave).
+constexpr unsigned kImmaTileW = 128;
+struct ImmaArg {
+	PlaneArg src, dst;
+	unsigned frames, src_w, src_h, dst_w, dst_h;
+	int pixel_max;
+	InXform in;
+	const ImmaBlock *h, *v; // blocks of the horizontal and vertical pass
+	unsigned nh, nv;
+	unsigned rb;            // 16-row output blocks per tile (at most 8: two per wave)
+	unsigned mb;            // 16-row blocks of intermediate rows per tile (1 .. kImmaMaxMb)
+	unsigned mid_pitch;     // bytes per intermediate column of one byte plane
+	unsigned lds;
+};
+constexpr unsigned kImmaMaxMb = 6;
+
 struct PointArg {
 	PlaneArg src, dst;
 	unsigned frames, width, height;
@@ -174,6 +208,7 @@ int launch_colorspace(const CspArg &a, void *stream);
 int launch_error_diffusion(const EdArg &a, void *stream);
 int launch_mfma_resize(const MfmaArg &a, void *stream);
 int launch_mfma_fused(const MfmaFusedArg &a, void *stream);
+int launch_resize_imma(const ImmaArg &a, void *stream);
 // Largest register capacity (tap pairs) the persistent batched kernel handles.
 constexpr int kJobsMaxTaps = 12;
 // Persistent launch over WORD tiled resize descriptors (device arrays):

@@ -1034,6 +1034,207 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 	resize_tile<WORD, N, HF>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
 
+// WORD resize on the int8 matrix cores, both passes in one launch
+// (resize_line_h_u16_c and resize_line_v_u16_c, resize_impl.cpp:44-59,78-95,
+// with the intermediate WORD row of pack_pixel_u16 in between). The
+// arithmetic of the split is described beside ImmaBlock; every product and
+// sum is an exact integer, so the outputs are the reference's.
+//
+// A workgroup owns kImmaTileW output columns and 16 rb output rows.
+// Horizontal pass: wave w takes column blocks 2w, 2w + 1 for each of the MB
+// 16-row blocks of intermediate rows from the tile's first vertical window;
+// A = 16 rows x [xh | xl'] (lane l: row l % 16, window samples
+// 8 (l / 16 % 2) .. + 7, high bytes for l < 32, low bytes else), read as
+// 16 bytes straight from the source row (all of a wave's loads are issued
+// before the first product; each window lies inside its row and starts on
+// an even sample except at the end of an odd-width row, ODD). The result
+// (lane: rows 4 (l / 16) + r of column l % 16) is stored as bytes (x ^ 0x80)
+// into two LDS planes, high and low, one column after another (mid_pitch
+// bytes each), so a column's rows are contiguous. Vertical pass: wave w
+// takes row blocks w, w + 4, ...; per column block A = 16 columns x
+// [yh | yl'] of the block's window rows (three dwords aligned with
+// v_alignbyte), B the block's coefficients, so D holds 4 consecutive output
+// columns of one output row per lane. mid_pitch = 2 mod 4 dwords: the
+// 16 columns of a read or write fall on even banks and the second group of
+// 16 lanes on odd banks.
+typedef int32_t imma_v4i __attribute__((ext_vector_type(4)));
+typedef uint32_t imma_u4 __attribute__((ext_vector_type(4), aligned(4)));
+
+// The kernel asks for at least 4 waves per SIMD (at most 128 VGPRs):
+// otherwise the compiler keeps the matrix results in AGPRs and copies each
+// one out.
+__device__ __forceinline__ int32_t imma_pack(int32_t hh, int32_t mid, int32_t ll, int32_t pixel_max)
+{
+	// ((hh << 8) + mid) << 8 + ll = 65536 hh + 256 mid + ll in wrapping 32-bit
+	// arithmetic (the true sum fits), as two shift-adds: the empty asm keeps
+	// the compiler from reassociating them into two shifts and a three-way
+	// add. (Only compiler-generated instructions read the matrix results, so
+	// the compiler inserts the wait states those reads need.)
+	uint32_t t = (static_cast<uint32_t>(hh) << 8) + static_cast<uint32_t>(mid);
+	asm("" : "+v"(t));
+	const uint32_t s = (t << 8) + static_cast<uint32_t>(ll);
+	return med3_i32(static_cast<int32_t>(s) >> 14, 0, pixel_max);
+}
+
+template <int MB, bool ODD>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) k_resize_imma(ImmaArg a)
+{
+	extern __shared__ __align__(16) unsigned char smem[];
+	const unsigned lane = threadIdx.x & 63;
+	const unsigned w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const unsigned half = (lane >> 4) & 1;
+	const unsigned f = blockIdx.z;
+	const unsigned j0 = blockIdx.x * kImmaTileW, i0 = blockIdx.y * 16 * a.rb;
+	const unsigned hb0 = blockIdx.x * (kImmaTileW / 16), vb0 = blockIdx.y * a.rb;
+	const int rbase = a.v[vb0].kb; // first intermediate row of the tile
+	unsigned char *midh = smem;
+	unsigned char *midl = midh + kImmaTileW * a.mid_pitch;
+
+	// Horizontal coefficients of the wave's two column blocks (blocks past the
+	// plane repeat the last one; their columns are not stored), then the
+	// source windows (rows past the plane repeat its last row: they meet only
+	// zero vertical taps).
+	uint32_t hw[2][3][2];
+	int32_t hinit[2];
+	int hkb[2];
+#pragma unroll
+	for (int q = 0; q < 2; ++q) {
+		const ImmaBlock &b = a.h[min(hb0 + 2 * w + q, a.nh - 1)];
+#pragma unroll
+		for (int p = 0; p < 3; ++p) {
+			const uint2 v = *reinterpret_cast<const uint2 *>(b.w[p][lane]);
+			hw[q][p][0] = v.x;
+			hw[q][p][1] = v.y;
+		}
+		hinit[q] = b.init[lane];
+		hkb[q] = b.kb;
+	}
+	// Vertical coefficients of the wave's row blocks w and w + 4, loaded with
+	// the source so that no memory round trip sits between the passes.
+	uint2 vw[2][3];
+	int32_t vinit[2];
+	int vkb[2];
+#pragma unroll
+	for (int k = 0; k < 2; ++k) {
+		const ImmaBlock &b = a.v[min(vb0 + w + 4 * k, a.nv - 1)];
+#pragma unroll
+		for (int p = 0; p < 3; ++p)
+			vw[k][p] = *reinterpret_cast<const uint2 *>(b.w[p][lane]);
+		vinit[k] = b.init[lane];
+		vkb[k] = b.kb;
+	}
+	const char *src = frame_ptr(a.src, f);
+	imma_u4 xr[MB][2];
+#pragma unroll
+	for (int m = 0; m < MB; ++m) {
+		const char *srow = row_ptr(src, a.src.stride, min(static_cast<unsigned>(rbase) + 16 * m + (lane & 15), a.src_h - 1));
+#pragma unroll
+		for (int q = 0; q < 2; ++q) {
+			const unsigned e = (hkb[q] & ~1) + 8 * half; // even start at or before the lane's first sample
+			const imma_u4 x = *reinterpret_cast<const imma_u4 *>(srow + 2 * static_cast<size_t>(e));
+			if constexpr (ODD) {
+				// Odd-width rows: a window may start on an odd sample; then the
+				// lane's samples are e + 1 .. e + 8, the last one read alone.
+				const uint32_t nx = reinterpret_cast<const uint16_t *>(srow)[min(e + 8, a.src_w - 1)];
+				const unsigned sh = (hkb[q] & 1) * 2;
+				xr[m][q] = imma_u4{ __builtin_amdgcn_alignbyte(x.y, x.x, sh), __builtin_amdgcn_alignbyte(x.z, x.y, sh),
+				                    __builtin_amdgcn_alignbyte(x.w, x.z, sh), __builtin_amdgcn_alignbyte(nx, x.w, sh) };
+			} else {
+				xr[m][q] = x;
+			}
+		}
+	}
+
+	// Horizontal pass.
+	{
+		const uint32_t sel = lane < 32 ? 0x07050301u : 0x06040200u; // high or low bytes of 4 samples
+#pragma unroll
+		for (int m = 0; m < MB; ++m) {
+#pragma unroll
+			for (int q = 0; q < 2; ++q) {
+				const imma_u4 x = xr[m][q];
+				// biased sample x ^ 0x8000: its high byte is x.b1 ^ 0x80, the low byte
+				// minus 128 is x.b0 ^ 0x80
+				const uint32_t ax = __builtin_amdgcn_perm(x.y, x.x, sel) ^ 0x80808080u;
+				const uint32_t ay = __builtin_amdgcn_perm(x.w, x.z, sel) ^ 0x80808080u;
+				const long A = static_cast<long>((static_cast<uint64_t>(ay) << 32) | ax);
+				const long Bh = static_cast<long>((static_cast<uint64_t>(hw[q][0][1]) << 32) | hw[q][0][0]);
+				const long Bm = static_cast<long>((static_cast<uint64_t>(hw[q][1][1]) << 32) | hw[q][1][0]);
+				const long Bl = static_cast<long>((static_cast<uint64_t>(hw[q][2][1]) << 32) | hw[q][2][0]);
+				const imma_v4i zero = { 0, 0, 0, 0 };
+				const imma_v4i init = { hinit[q], hinit[q], hinit[q], hinit[q] };
+				const imma_v4i hh = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bh, zero, 0, 0, 0);
+				const imma_v4i md = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bm, zero, 0, 0, 0);
+				const imma_v4i ll = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bl, init, 0, 0, 0);
+				const uint32_t y0 = imma_pack(hh[0], md[0], ll[0], a.pixel_max), y1 = imma_pack(hh[1], md[1], ll[1], a.pixel_max);
+				const uint32_t y2 = imma_pack(hh[2], md[2], ll[2], a.pixel_max), y3 = imma_pack(hh[3], md[3], ll[3], a.pixel_max);
+				const uint32_t p01 = __builtin_amdgcn_perm(y1, y0, 0x05010400u); // y0.b0 y1.b0 y0.b1 y1.b1
+				const uint32_t p23 = __builtin_amdgcn_perm(y3, y2, 0x05010400u);
+				const uint32_t lo = __builtin_amdgcn_perm(p23, p01, 0x05040100u) ^ 0x80808080u;
+				const uint32_t hi = __builtin_amdgcn_perm(p23, p01, 0x07060302u) ^ 0x80808080u;
+				const unsigned off = (16 * (2 * w + q) + (lane & 15)) * a.mid_pitch + 16 * m + 4 * (lane >> 4);
+				*reinterpret_cast<uint32_t *>(midh + off) = hi;
+				*reinterpret_cast<uint32_t *>(midl + off) = lo;
+			}
+		}
+	}
+	__syncthreads();
+
+	// Vertical pass.
+	char *dst = frame_ptr_w(a.dst, f);
+	const unsigned char *plane = (lane < 32 ? midh : midl) + (lane & 15) * a.mid_pitch;
+#pragma unroll
+	for (int k = 0; k < 2; ++k) {
+		const unsigned vb = w + 4 * k;
+		if (vb >= a.rb)
+			break;
+		const uint2 vh = vw[k][0], vm = vw[k][1], vl = vw[k][2];
+		const unsigned o = static_cast<unsigned>(vkb[k] - rbase) + 8 * half;
+		const unsigned sh = o & 3;
+		const long Bh = static_cast<long>((static_cast<uint64_t>(vh.y) << 32) | vh.x);
+		const long Bm = static_cast<long>((static_cast<uint64_t>(vm.y) << 32) | vm.x);
+		const long Bl = static_cast<long>((static_cast<uint64_t>(vl.y) << 32) | vl.x);
+		const imma_v4i zero = { 0, 0, 0, 0 };
+		const imma_v4i init = { vinit[k], vinit[k], vinit[k], vinit[k] };
+		const unsigned row = i0 + 16 * vb + (lane & 15);
+		char *drow = row_ptr(dst, a.dst.stride, min(row, a.dst_h - 1));
+		// Column block c + 1 is multiplied before block c is packed and stored.
+		imma_v4i acc[2][3];
+		auto mul = [&](unsigned c, imma_v4i *r) {
+			const uint32_t *p = reinterpret_cast<const uint32_t *>(plane + 16 * c * a.mid_pitch + (o & ~3u));
+			const uint32_t d0 = p[0], d1 = p[1], d2 = p[2];
+			const uint32_t ax = __builtin_amdgcn_alignbyte(d1, d0, sh), ay = __builtin_amdgcn_alignbyte(d2, d1, sh);
+			const long A = static_cast<long>((static_cast<uint64_t>(ay) << 32) | ax);
+			r[0] = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bh, zero, 0, 0, 0);
+			r[1] = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bm, zero, 0, 0, 0);
+			r[2] = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bl, init, 0, 0, 0);
+		};
+		mul(0, acc[0]);
+#pragma unroll
+		for (unsigned c = 0; c < kImmaTileW / 16; ++c) {
+			if (c + 1 < kImmaTileW / 16)
+				mul(c + 1, acc[(c + 1) & 1]);
+			const imma_v4i hh = acc[c & 1][0], md = acc[c & 1][1], ll = acc[c & 1][2];
+			const uint32_t y0 = imma_pack(hh[0], md[0], ll[0], a.pixel_max), y1 = imma_pack(hh[1], md[1], ll[1], a.pixel_max);
+			const uint32_t y2 = imma_pack(hh[2], md[2], ll[2], a.pixel_max), y3 = imma_pack(hh[3], md[3], ll[3], a.pixel_max);
+			const unsigned col = j0 + 16 * c + 4 * (lane >> 4);
+			if (row < a.dst_h) {
+				if (col + 4 <= a.dst_w) {
+					*reinterpret_cast<uint2 *>(drow + 2 * static_cast<size_t>(col)) = make_uint2(pack16x2(y0, y1), pack16x2(y2, y3));
+				} else {
+					uint16_t *d = reinterpret_cast<uint16_t *>(drow) + col;
+					if (col < a.dst_w)
+						d[0] = static_cast<uint16_t>(y0);
+					if (col + 1 < a.dst_w)
+						d[1] = static_cast<uint16_t>(y1);
+					if (col + 2 < a.dst_w)
+						d[2] = static_cast<uint16_t>(y2);
+				}
+			}
+		}
+	}
+}
+
 // Persistent batched resize: each workgroup walks the (cost-ordered) tile
 // list of many WORD resize descriptors (register capacity up to
 // kJobsMaxTaps pairs), one tile at a time. Descriptors and tiles are
@@ -1462,6 +1663,26 @@ int launch_resize_jobs(const ResizeArg *descs, const JobTile *tiles, unsigned nt
 	if (!ntiles)
 		return 0;
 	hipLaunchKernelGGL(k_resize_jobs, dim3(grid), dim3(kThreads), lds, static_cast<hipStream_t>(stream), descs, tiles, ntiles);
+	return static_cast<int>(hipGetLastError());
+}
+
+int launch_resize_imma(const ImmaArg &a, void *stream)
+{
+	if (!a.frames || !a.dst_w || !a.dst_h)
+		return 0;
+	dim3 grid((a.dst_w + kImmaTileW - 1) / kImmaTileW, (a.dst_h + 16 * a.rb - 1) / (16 * a.rb), a.frames);
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	const bool odd = a.src_w & 1;
+	switch (a.mb) {
+#define ZAMD_IMMA(n) \
+	case n: \
+		if (odd) hipLaunchKernelGGL((k_resize_imma<n, true>), grid, dim3(kThreads), a.lds, s, a); \
+		else hipLaunchKernelGGL((k_resize_imma<n, false>), grid, dim3(kThreads), a.lds, s, a); \
+		break;
+	ZAMD_IMMA(1) ZAMD_IMMA(2) ZAMD_IMMA(3) ZAMD_IMMA(4) ZAMD_IMMA(5) ZAMD_IMMA(6)
+#undef ZAMD_IMMA
+	default: return static_cast<int>(hipErrorInvalidValue);
+	}
 	return static_cast<int>(hipGetLastError());
 }
 
