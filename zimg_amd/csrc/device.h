@@ -32,7 +32,7 @@ struct TiledConfig {
 // false: the step is not eligible and runs on the tiled kernel).
 struct ImmaPlan {
 	bool ok = false;
-	unsigned rb = 0, mb = 0, mid_pitch = 0, lds = 0;
+	unsigned rb = 0, mb = 0, mid_pitch = 0, in_cols = 0, in_pitch = 0, lds = 0;
 };
 
 class DeviceProgram {

@@ -658,12 +658,12 @@ unsigned odd_stride(unsigned n)
 namespace {
 
 // Window starts of the 16-output blocks of a WORD pass for the integer
-// matrix-core form (starts inside [0, src_len - 16], even where possible,
-// when `even`: the horizontal pass), or an empty vector when a block
+// matrix-core form (even starts when `even`: the horizontal pass), or an
+// empty vector when a block
 // does not fit one 16-sample window, the starts are not monotonic, a tap
 // does not split into two signed bytes (c >= 32640) or a row's sum could
 // leave the int32 range (sum |c| * 32768 + 2^29 + 2^13 >= 2^31).
-std::vector<int> imma_windows(const FilterTable &t, bool even, unsigned src_len)
+std::vector<int> imma_windows(const FilterTable &t, bool even)
 {
 	std::vector<int> kb;
 	const unsigned fw = t.filter_width;
@@ -679,10 +679,7 @@ std::vector<int> imma_windows(const FilterTable &t, bool even, unsigned src_len)
 			return {};
 	}
 	for (unsigned o0 = 0; o0 < t.filter_rows; o0 += 16) {
-		// Horizontal windows end inside the row (they are read straight from
-		// the source) and start on an even sample unless that is impossible
-		// (the last windows of an odd-width row).
-		const int k = even ? static_cast<int>(std::min(t.left[o0] & ~1u, src_len - 16)) : static_cast<int>(t.left[o0]);
+		const int k = static_cast<int>(even ? t.left[o0] & ~1u : t.left[o0]);
 		for (unsigned o = o0; o < std::min(t.filter_rows, o0 + 16); ++o)
 			if (static_cast<int>(t.left[o]) < k || static_cast<int>(t.left[o] + fw) > k + 16)
 				return {};
@@ -693,35 +690,35 @@ std::vector<int> imma_windows(const FilterTable &t, bool even, unsigned src_len)
 	return kb;
 }
 
-void imma_blocks(const FilterTable &t, const std::vector<int> &kb, std::vector<dev::ImmaBlock> &out)
+// horizontal: lane group g holds window samples 4 g + 0..3 and + 8..11 (the
+// kernel's LDS read order); vertical: 8 g + 0..7.
+void imma_blocks(const FilterTable &t, const std::vector<int> &kb, bool horizontal, std::vector<dev::ImmaBlock> &out)
 {
 	const unsigned fw = t.filter_width;
 	for (size_t b = 0; b < kb.size(); ++b) {
 		dev::ImmaBlock blk{};
 		blk.kb = kb[b];
-		for (unsigned lane = 0; lane < 64; ++lane) {
-			const unsigned o = 16 * static_cast<unsigned>(b) + lane % 16;
-			const bool live = o < t.filter_rows;
+		for (unsigned n = 0; n < 16; ++n) {
+			const unsigned o = 16 * static_cast<unsigned>(b) + n;
 			int32_t sum = 0;
-			for (unsigned k = 0; live && k < fw; ++k)
+			for (unsigned k = 0; o < t.filter_rows && k < fw; ++k)
 				sum += t.data_i16[static_cast<size_t>(o) * fw + k];
-			blk.init[lane] = (1 << 29) + (1 << 13) + 128 * sum;
-			uint8_t bytes[3][8];
+			blk.init[n] = (1 << 29) + (1 << 13) + 128 * sum;
+		}
+		for (unsigned lane = 0; lane < 32; ++lane) {
+			const unsigned o = 16 * static_cast<unsigned>(b) + lane % 16, g = lane / 16;
+			uint8_t bytes[16];
 			for (unsigned i = 0; i < 8; ++i) {
-				const unsigned k = 8 * (lane / 16) + i;
-				const bool high = k < 16; // the xh half of the window
-				const long smp = kb[b] + static_cast<long>(k % 16);
+				const unsigned pos = horizontal ? 4 * g + (i & 3) + 8 * (i >> 2) : 8 * g + i;
+				const long smp = kb[b] + static_cast<long>(pos);
 				int c = 0;
-				if (live && smp >= t.left[o] && smp < static_cast<long>(t.left[o] + fw))
+				if (o < t.filter_rows && smp >= t.left[o] && smp < static_cast<long>(t.left[o] + fw))
 					c = t.data_i16[static_cast<size_t>(o) * fw + (smp - t.left[o])];
 				const int cl = static_cast<int8_t>(c & 0xFF);
-				const int ch = (c - cl) / 256;
-				bytes[0][i] = static_cast<uint8_t>(high ? ch : 0);
-				bytes[1][i] = static_cast<uint8_t>(high ? cl : ch);
-				bytes[2][i] = static_cast<uint8_t>(high ? 0 : cl);
+				bytes[i] = static_cast<uint8_t>((c - cl) / 256);
+				bytes[8 + i] = static_cast<uint8_t>(cl);
 			}
-			for (int p = 0; p < 3; ++p)
-				std::memcpy(blk.w[p][lane], bytes[p], 8);
+			std::memcpy(blk.c[lane], bytes, 16);
 		}
 		out.push_back(blk);
 	}
@@ -734,16 +731,23 @@ ImmaPlan DeviceProgram::imma_plan(const Step &s) const
 	ImmaPlan p;
 	if (!s.word || s.passes != 2 || !s.h_first)
 		return p;
-	if (s.ox[0].op != 0 || s.ox[0].type != 1 || s.ix[0].type != 1 || s.ix[0].op != 0 || s.in_w < 16)
+	if (s.ox[0].op != 0 || s.ox[0].type != 1 || s.ix[0].type > 1 || s.ix[0].op > 1)
 		return p;
 	const FilterTable &th = *m_plan.nodes[s.tab[0]].table;
 	const FilterTable &tv = *m_plan.nodes[s.tab[1]].table;
 	if (th.filter_rows != s.width || tv.filter_rows != s.height)
 		return p;
-	const std::vector<int> kh = imma_windows(th, true, s.in_w), kv = imma_windows(tv, false, s.in_h);
+	const std::vector<int> kh = imma_windows(th, true), kv = imma_windows(tv, false);
 	if (kh.empty() || kv.empty())
 		return p;
-	const unsigned nv = static_cast<unsigned>(kv.size());
+	const unsigned nh = static_cast<unsigned>(kh.size()), nv = static_cast<unsigned>(kv.size());
+	const unsigned cpt = dev::kImmaTileW / 16;
+	unsigned in_cols = 0;
+	for (unsigned b0 = 0; b0 < nh; b0 += cpt) {
+		const int cb = kh[b0] & ~15;
+		for (unsigned b = b0; b < std::min(nh, b0 + cpt); ++b)
+			in_cols = std::max(in_cols, static_cast<unsigned>((kh[b] + 16 - cb + 15) & ~15));
+	}
 	const unsigned rb = 8;
 	unsigned mb = 0;
 	for (unsigned b0 = 0; b0 < nv; b0 += rb)
@@ -754,8 +758,10 @@ ImmaPlan DeviceProgram::imma_plan(const Step &s) const
 	p.rb = rb;
 	p.mb = mb;
 	p.mid_pitch = 16 * mb + 8; // 2 mod 4 dwords (bank spread of the kernel's reads and writes)
-	p.lds = 2 * dev::kImmaTileW * p.mid_pitch;
-	p.ok = true;
+	p.in_cols = in_cols;
+	p.in_pitch = odd_stride(in_cols);
+	p.lds = 2 * dev::kImmaTileW * p.mid_pitch + std::max(2 * 16 * mb * p.in_pitch, 2 * 4 * 16 * dev::kImmaOutPitch);
+	p.ok = p.lds <= 64 * 1024;
 	return p;
 }
 
@@ -780,7 +786,7 @@ const DeviceProgram::ImmaTables &DeviceProgram::imma_tables_for_current_device()
 			const FilterTable &t = *m_plan.nodes[s.tab[q]].table;
 			ids.push_back(key);
 			first.push_back(blocks.size());
-			imma_blocks(t, imma_windows(t, q == 0, q == 0 ? s.in_w : s.in_h), blocks);
+			imma_blocks(t, imma_windows(t, q == 0), q == 0, blocks);
 		}
 	}
 	auto tab = std::make_shared<ImmaTables>();
@@ -1254,6 +1260,8 @@ void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void
 				a.rb = s.imma.rb;
 				a.mb = s.imma.mb;
 				a.mid_pitch = s.imma.mid_pitch;
+				a.in_cols = s.imma.in_cols;
+				a.in_pitch = s.imma.in_pitch;
 				a.lds = s.imma.lds;
 				rc = dev::launch_resize_imma(a, stream);
 				break;

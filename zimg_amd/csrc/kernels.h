@@ -75,20 +75,22 @@ constexpr int kFastTapBuckets[] = { 2, 3, 4, 5, 6, 8, 12, 16, 24, 32 };
 constexpr unsigned kFastTileW = 512;
 
 // WORD resize on the int8 matrix cores (horizontal pass first, both passes
-// in one launch, bit-exact; WORD source without a depth shift, horizontal
-// windows inside the row). A biased sample x - 32768 = 256 xh + xl' + 128
+// in one launch, bit-exact). A biased sample x - 32768 = 256 xh + xl' + 128
 // and a Q1.14 tap c = 256 ch + cl are split into signed bytes; a 16-output
 // block of a pass is three 16x16x32 products over a 16-sample window,
 // A = [xh | xl'] against [ch | 0] (hh), [cl | ch] (mid) and [0 | cl] (ll),
 // and the int32 sum is 65536 hh + 256 mid + ll + init, with
 // init = 128 sum(c) + the rounding and unbias terms of the pack.
-// One record per block of 16 outputs: w[p][lane] holds the lane's 8
-// coefficient bytes of product p in the MFMA operand layout (lane l: output
-// l % 16, window k = 8 (l / 16) .. + 7, k < 16 the xh half), init[lane] the
-// start of the lane's accumulators, kb the window's first input sample.
+// One record per block of 16 outputs: c[l] holds, for lane l < 32 of the
+// MFMA operand (output l % 16), the 8 signed bytes ch and then the 8 bytes cl
+// of its window samples: 8 (l / 16) + 0..7 for the vertical pass,
+// 4 (l / 16) + 0..3 and + 8..11 for the horizontal pass. Lanes 32-63 take
+// the same samples of the xl' half, so the three operands are [ch | 0],
+// [cl | ch] and [0 | cl] over lanes 0-31 | 32-63. init[n] is the
+// accumulator start of output n, kb the window's first input sample.
 struct alignas(16) ImmaBlock {
-	uint32_t w[3][64][2];
-	int32_t init[64];
+	uint32_t c[32][4];
+	int32_t init[16];
 	int32_t kb;
 	int32_t pad[3];
 };
@@ -104,9 +106,13 @@ struct ImmaArg {
 	unsigned rb;            // 16-row output blocks per tile (at most 8: two per wave)
 	unsigned mb;            // 16-row blocks of intermediate rows per tile (1 .. kImmaMaxMb)
 	unsigned mid_pitch;     // bytes per intermediate column of one byte plane
+	unsigned in_cols, in_pitch; // staged source columns and their LDS row pitch (samples)
 	unsigned lds;
 };
 constexpr unsigned kImmaMaxMb = 6;
+// Row pitch (samples) of a wave's LDS copy of 16 output rows of its tile;
+// the staged source area holds at least 4 such copies.
+constexpr unsigned kImmaOutPitch = kImmaTileW + 8;
 
 struct PointArg {
 	PlaneArg src, dst;

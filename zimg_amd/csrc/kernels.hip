@@ -1040,25 +1040,23 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 // arithmetic of the split is described beside ImmaBlock; every product and
 // sum is an exact integer, so the outputs are the reference's.
 //
-// A workgroup owns kImmaTileW output columns and 16 rb output rows.
-// Horizontal pass: wave w takes column blocks 2w, 2w + 1 for each of the MB
-// 16-row blocks of intermediate rows from the tile's first vertical window;
-// A = 16 rows x [xh | xl'] (lane l: row l % 16, window samples
-// 8 (l / 16 % 2) .. + 7, high bytes for l < 32, low bytes else), read as
-// 16 bytes straight from the source row (all of a wave's loads are issued
-// before the first product; each window lies inside its row and starts on
-// an even sample except at the end of an odd-width row, ODD). The result
-// (lane: rows 4 (l / 16) + r of column l % 16) is stored as bytes (x ^ 0x80)
-// into two LDS planes, high and low, one column after another (mid_pitch
-// bytes each), so a column's rows are contiguous. Vertical pass: wave w
-// takes row blocks w, w + 4, ...; per column block A = 16 columns x
-// [yh | yl'] of the block's window rows (three dwords aligned with
-// v_alignbyte), B the block's coefficients, so D holds 4 consecutive output
-// columns of one output row per lane. mid_pitch = 2 mod 4 dwords: the
-// 16 columns of a read or write fall on even banks and the second group of
-// 16 lanes on odd banks.
+// A workgroup owns kImmaTileW output columns and 16 rb output rows. It
+// stages its source window (16 MB rows from the first vertical window, the
+// columns of its 8 horizontal windows) in LDS as biased samples, then:
+// horizontal pass: wave w takes column blocks 2w, 2w + 1 for each of the MB
+// 16-row blocks; A = 16 rows x [xh | xl'] (lane l: row l % 16, window
+// samples 4 (l / 16 % 2) + 0..3 and + 8..11, high bytes for l < 32, low
+// bytes ^ 0x80 else; the coefficient table uses the same order). The result (lane: rows 4 (l / 16) + r of column l % 16) is stored
+// as bytes (x ^ 0x80) into two LDS planes, high and low, one column after
+// another (mid_pitch bytes each), so a column's rows are contiguous.
+// Vertical pass: wave w takes row blocks w and w + 4; per column block
+// A = 16 columns x [yh | yl'] of the block's window rows (three dwords
+// aligned with v_alignbyte), B the block's coefficients, so D holds 4
+// consecutive output columns of one output row per lane. mid_pitch = 2 mod 4
+// dwords: the 16 columns of a read or write fall on even banks and the
+// second group of 16 lanes on odd banks. All coefficients are loaded before
+// the staging, so no memory round trip sits between the passes.
 typedef int32_t imma_v4i __attribute__((ext_vector_type(4)));
-typedef uint32_t imma_u4 __attribute__((ext_vector_type(4), aligned(4)));
 
 // The kernel asks for at least 4 waves per SIMD (at most 128 VGPRs):
 // otherwise the compiler keeps the matrix results in AGPRs and copies each
@@ -1076,7 +1074,7 @@ __device__ __forceinline__ int32_t imma_pack(int32_t hh, int32_t mid, int32_t ll
 	return med3_i32(static_cast<int32_t>(s) >> 14, 0, pixel_max);
 }
 
-template <int MB, bool ODD>
+template <int MB>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) k_resize_imma(ImmaArg a)
 {
 	extern __shared__ __align__(16) unsigned char smem[];
@@ -1091,76 +1089,66 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 	unsigned char *midl = midh + kImmaTileW * a.mid_pitch;
 
 	// Horizontal coefficients of the wave's two column blocks (blocks past the
-	// plane repeat the last one; their columns are not stored), then the
-	// source windows (rows past the plane repeat its last row: they meet only
-	// zero vertical taps).
-	uint32_t hw[2][3][2];
+	// plane repeat the last one; their columns are not stored).
+	uint4 hc[2];
 	int32_t hinit[2];
 	int hkb[2];
 #pragma unroll
 	for (int q = 0; q < 2; ++q) {
 		const ImmaBlock &b = a.h[min(hb0 + 2 * w + q, a.nh - 1)];
-#pragma unroll
-		for (int p = 0; p < 3; ++p) {
-			const uint2 v = *reinterpret_cast<const uint2 *>(b.w[p][lane]);
-			hw[q][p][0] = v.x;
-			hw[q][p][1] = v.y;
-		}
-		hinit[q] = b.init[lane];
+		hc[q] = *reinterpret_cast<const uint4 *>(b.c[lane & 31]);
+		hinit[q] = b.init[lane & 15];
 		hkb[q] = b.kb;
 	}
-	// Vertical coefficients of the wave's row blocks w and w + 4, loaded with
-	// the source so that no memory round trip sits between the passes.
-	uint2 vw[2][3];
+	// Vertical coefficients of the wave's row blocks w and w + 4.
+	uint4 vc[2];
 	int32_t vinit[2];
 	int vkb[2];
 #pragma unroll
 	for (int k = 0; k < 2; ++k) {
 		const ImmaBlock &b = a.v[min(vb0 + w + 4 * k, a.nv - 1)];
-#pragma unroll
-		for (int p = 0; p < 3; ++p)
-			vw[k][p] = *reinterpret_cast<const uint2 *>(b.w[p][lane]);
-		vinit[k] = b.init[lane];
+		vc[k] = *reinterpret_cast<const uint4 *>(b.c[lane & 31]);
+		vinit[k] = b.init[lane & 15];
 		vkb[k] = b.kb;
 	}
+	// The three B operands from a record's (ch, cl): [ch | 0], [cl | ch], [0 | cl].
+	const bool lo_half = lane < 32;
+	auto operands = [&](const uint4 &c, long &Bh, long &Bm, long &Bl) {
+		const uint64_t ch = (static_cast<uint64_t>(c.y) << 32) | c.x, cl = (static_cast<uint64_t>(c.w) << 32) | c.z;
+		Bh = static_cast<long>(lo_half ? ch : 0);
+		Bm = static_cast<long>(lo_half ? cl : ch);
+		Bl = static_cast<long>(lo_half ? 0 : cl);
+	};
+	// Stage the source window (16 MB rows from the first vertical window, the
+	// columns of the tile's 8 horizontal windows) as biased samples with
+	// 16-byte loads of whole row segments; rows and columns past the plane are
+	// stored as zero samples (they meet only zero taps).
 	const char *src = frame_ptr(a.src, f);
-	imma_u4 xr[MB][2];
-#pragma unroll
-	for (int m = 0; m < MB; ++m) {
-		const char *srow = row_ptr(src, a.src.stride, min(static_cast<unsigned>(rbase) + 16 * m + (lane & 15), a.src_h - 1));
-#pragma unroll
-		for (int q = 0; q < 2; ++q) {
-			const unsigned e = (hkb[q] & ~1) + 8 * half; // even start at or before the lane's first sample
-			const imma_u4 x = *reinterpret_cast<const imma_u4 *>(srow + 2 * static_cast<size_t>(e));
-			if constexpr (ODD) {
-				// Odd-width rows: a window may start on an odd sample; then the
-				// lane's samples are e + 1 .. e + 8, the last one read alone.
-				const uint32_t nx = reinterpret_cast<const uint16_t *>(srow)[min(e + 8, a.src_w - 1)];
-				const unsigned sh = (hkb[q] & 1) * 2;
-				xr[m][q] = imma_u4{ __builtin_amdgcn_alignbyte(x.y, x.x, sh), __builtin_amdgcn_alignbyte(x.z, x.y, sh),
-				                    __builtin_amdgcn_alignbyte(x.w, x.z, sh), __builtin_amdgcn_alignbyte(nx, x.w, sh) };
-			} else {
-				xr[m][q] = x;
-			}
-		}
-	}
+	const int cb = a.h[hb0].kb & ~15; // first staged source column
+	uint16_t *xin = reinterpret_cast<uint16_t *>(smem + 2 * kImmaTileW * a.mid_pitch);
+	stage_tile<true, 4>(xin, a.in_pitch, src, a.src.stride, static_cast<unsigned>(rbase), 16 * MB, a.src_h,
+	                    static_cast<unsigned>(cb), a.in_cols, a.src_w, a.in);
+	__syncthreads();
 
 	// Horizontal pass.
 	{
 		const uint32_t sel = lane < 32 ? 0x07050301u : 0x06040200u; // high or low bytes of 4 samples
+		const uint32_t lx = lane < 32 ? 0u : 0x80808080u;           // low byte - 128 = low byte ^ 0x80
+		const uint32_t *xw = reinterpret_cast<const uint32_t *>(xin);
+		const unsigned pw = a.in_pitch / 2;
 #pragma unroll
 		for (int m = 0; m < MB; ++m) {
+			const uint32_t *row = xw + (16 * m + (lane & 15)) * pw;
 #pragma unroll
 			for (int q = 0; q < 2; ++q) {
-				const imma_u4 x = xr[m][q];
-				// biased sample x ^ 0x8000: its high byte is x.b1 ^ 0x80, the low byte
-				// minus 128 is x.b0 ^ 0x80
-				const uint32_t ax = __builtin_amdgcn_perm(x.y, x.x, sel) ^ 0x80808080u;
-				const uint32_t ay = __builtin_amdgcn_perm(x.w, x.z, sel) ^ 0x80808080u;
+				// lane group g holds window samples 4 (g % 2) + 0..3 and + 8..11:
+				// the two groups of a half-wave read different banks of a row
+				const uint32_t *x = row + (hkb[q] - cb) / 2 + 2 * half;
+				const uint32_t d0 = x[0], d1 = x[1], d4 = x[4], d5 = x[5];
+				const uint32_t ax = __builtin_amdgcn_perm(d1, d0, sel) ^ lx, ay = __builtin_amdgcn_perm(d5, d4, sel) ^ lx;
 				const long A = static_cast<long>((static_cast<uint64_t>(ay) << 32) | ax);
-				const long Bh = static_cast<long>((static_cast<uint64_t>(hw[q][0][1]) << 32) | hw[q][0][0]);
-				const long Bm = static_cast<long>((static_cast<uint64_t>(hw[q][1][1]) << 32) | hw[q][1][0]);
-				const long Bl = static_cast<long>((static_cast<uint64_t>(hw[q][2][1]) << 32) | hw[q][2][0]);
+				long Bh, Bm, Bl;
+				operands(hc[q], Bh, Bm, Bl);
 				const imma_v4i zero = { 0, 0, 0, 0 };
 				const imma_v4i init = { hinit[q], hinit[q], hinit[q], hinit[q] };
 				const imma_v4i hh = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bh, zero, 0, 0, 0);
@@ -1180,25 +1168,24 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 	}
 	__syncthreads();
 
-	// Vertical pass.
+	// Vertical pass. The staged source is no longer read: each wave copies its
+	// output rows there before storing them.
 	char *dst = frame_ptr_w(a.dst, f);
+	uint16_t *ot = reinterpret_cast<uint16_t *>(xin) + w * 16 * kImmaOutPitch;
 	const unsigned char *plane = (lane < 32 ? midh : midl) + (lane & 15) * a.mid_pitch;
 #pragma unroll
 	for (int k = 0; k < 2; ++k) {
 		const unsigned vb = w + 4 * k;
 		if (vb >= a.rb)
 			break;
-		const uint2 vh = vw[k][0], vm = vw[k][1], vl = vw[k][2];
 		const unsigned o = static_cast<unsigned>(vkb[k] - rbase) + 8 * half;
 		const unsigned sh = o & 3;
-		const long Bh = static_cast<long>((static_cast<uint64_t>(vh.y) << 32) | vh.x);
-		const long Bm = static_cast<long>((static_cast<uint64_t>(vm.y) << 32) | vm.x);
-		const long Bl = static_cast<long>((static_cast<uint64_t>(vl.y) << 32) | vl.x);
+		long Bh, Bm, Bl;
+		operands(vc[k], Bh, Bm, Bl);
 		const imma_v4i zero = { 0, 0, 0, 0 };
 		const imma_v4i init = { vinit[k], vinit[k], vinit[k], vinit[k] };
-		const unsigned row = i0 + 16 * vb + (lane & 15);
-		char *drow = row_ptr(dst, a.dst.stride, min(row, a.dst_h - 1));
-		// Column block c + 1 is multiplied before block c is packed and stored.
+		// Column block c + 1 is multiplied before block c is packed into the
+		// wave's LDS copy of its 16 output rows.
 		imma_v4i acc[2][3];
 		auto mul = [&](unsigned c, imma_v4i *r) {
 			const uint32_t *p = reinterpret_cast<const uint32_t *>(plane + 16 * c * a.mid_pitch + (o & ~3u));
@@ -1217,21 +1204,29 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 			const imma_v4i hh = acc[c & 1][0], md = acc[c & 1][1], ll = acc[c & 1][2];
 			const uint32_t y0 = imma_pack(hh[0], md[0], ll[0], a.pixel_max), y1 = imma_pack(hh[1], md[1], ll[1], a.pixel_max);
 			const uint32_t y2 = imma_pack(hh[2], md[2], ll[2], a.pixel_max), y3 = imma_pack(hh[3], md[3], ll[3], a.pixel_max);
-			const unsigned col = j0 + 16 * c + 4 * (lane >> 4);
+			*reinterpret_cast<uint2 *>(ot + (lane & 15) * kImmaOutPitch + 16 * c + 4 * (lane >> 4)) = make_uint2(pack16x2(y0, y1), pack16x2(y2, y3));
+		}
+		__builtin_amdgcn_wave_barrier();
+		// Whole row segments out: 16 lanes per row, 16 bytes each.
+#pragma unroll
+		for (unsigned u = 0; u < 4; ++u) {
+			const unsigned e = 64 * u + lane, r = e / 16, chunk = e % 16;
+			const unsigned row = i0 + 16 * vb + r, col = j0 + 8 * chunk;
+			const uint4 v = *reinterpret_cast<const uint4 *>(ot + r * kImmaOutPitch + 8 * chunk);
 			if (row < a.dst_h) {
-				if (col + 4 <= a.dst_w) {
-					*reinterpret_cast<uint2 *>(drow + 2 * static_cast<size_t>(col)) = make_uint2(pack16x2(y0, y1), pack16x2(y2, y3));
+				uint16_t *d = reinterpret_cast<uint16_t *>(row_ptr(dst, a.dst.stride, row)) + col;
+				if (col + 8 <= a.dst_w) {
+					*reinterpret_cast<uint4 *>(d) = v;
 				} else {
-					uint16_t *d = reinterpret_cast<uint16_t *>(drow) + col;
-					if (col < a.dst_w)
-						d[0] = static_cast<uint16_t>(y0);
-					if (col + 1 < a.dst_w)
-						d[1] = static_cast<uint16_t>(y1);
-					if (col + 2 < a.dst_w)
-						d[2] = static_cast<uint16_t>(y2);
+					const uint32_t vv[4] = { v.x, v.y, v.z, v.w };
+#pragma unroll
+					for (unsigned i = 0; i < 7; ++i)
+						if (col + i < a.dst_w)
+							d[i] = static_cast<uint16_t>(vv[i / 2] >> (16 * (i & 1)));
 				}
 			}
 		}
+		__builtin_amdgcn_wave_barrier();
 	}
 }
 
@@ -1672,12 +1667,10 @@ int launch_resize_imma(const ImmaArg &a, void *stream)
 		return 0;
 	dim3 grid((a.dst_w + kImmaTileW - 1) / kImmaTileW, (a.dst_h + 16 * a.rb - 1) / (16 * a.rb), a.frames);
 	hipStream_t s = static_cast<hipStream_t>(stream);
-	const bool odd = a.src_w & 1;
 	switch (a.mb) {
 #define ZAMD_IMMA(n) \
 	case n: \
-		if (odd) hipLaunchKernelGGL((k_resize_imma<n, true>), grid, dim3(kThreads), a.lds, s, a); \
-		else hipLaunchKernelGGL((k_resize_imma<n, false>), grid, dim3(kThreads), a.lds, s, a); \
+		hipLaunchKernelGGL((k_resize_imma<n>), grid, dim3(kThreads), a.lds, s, a); \
 		break;
 	ZAMD_IMMA(1) ZAMD_IMMA(2) ZAMD_IMMA(3) ZAMD_IMMA(4) ZAMD_IMMA(5) ZAMD_IMMA(6)
 #undef ZAMD_IMMA
