@@ -1081,9 +1081,16 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 	const unsigned lane = threadIdx.x & 63;
 	const unsigned w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const unsigned half = (lane >> 4) & 1;
-	const unsigned f = blockIdx.z;
-	const unsigned j0 = blockIdx.x * kImmaTileW, i0 = blockIdx.y * 16 * a.rb;
-	const unsigned hb0 = blockIdx.x * (kImmaTileW / 16), vb0 = blockIdx.y * a.rb;
+	// Workgroups go to the 8 XCDs in turn; each XCD takes a contiguous run of
+	// tiles (row-major over the frame batch), so tiles that share source rows
+	// and columns meet in one L2.
+	const unsigned nx = gridDim.x, ny = gridDim.y, n = nx * ny * gridDim.z;
+	const unsigned lin = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+	const unsigned xcd = lin % 8, q8 = n / 8, r8 = n % 8;
+	const unsigned t = (xcd < r8 ? xcd * (q8 + 1) : r8 * (q8 + 1) + (xcd - r8) * q8) + lin / 8;
+	const unsigned bx = t % nx, by = (t / nx) % ny, f = t / (nx * ny);
+	const unsigned j0 = bx * kImmaTileW, i0 = by * 16 * a.rb;
+	const unsigned hb0 = bx * (kImmaTileW / 16), vb0 = by * a.rb;
 	const int rbase = a.v[vb0].kb; // first intermediate row of the tile
 	unsigned char *midh = smem;
 	unsigned char *midl = midh + kImmaTileW * a.mid_pitch;
