@@ -163,10 +163,12 @@ def test_graph_lowering_matches_reference_trace(case):
         assert got.startswith(want), (got, want)
 
 
-def test_unsupported_alpha_premultiplication_is_reported():
+def test_straight_alpha_resize_lowers_to_premultiplied_point_steps():
     src = _fmt_from_spec({"base": "rgb", "alpha": 1})
-    dst = _fmt_from_spec({"base": "rgb", "alpha": 0})
-    assert _err_of(src, dst) == z.ERROR_UNSUPPORTED_OPERATION
+    dst = _fmt_from_spec({"base": "rgb", "alpha": 1, "width": 128, "height": 96})
+    dev = z.FilterGraph(src, dst).describe().split("device:\n")[1]
+    assert sum(": premultiply[" in l for l in dev.splitlines()) == 3
+    assert sum(": unpremultiply[" in l for l in dev.splitlines()) == 3
 
 
 def test_device_tmp_size_scales_with_frames():

@@ -91,6 +91,53 @@ def test_word_resize_integer_matrix_cores(sw, sh, dw, dh, ft, a, b, depth, frame
         _assert_equal(out[k], P.word_resize(src[k], dw, dh, ft, a, b, depth), f"frame {k}")
 
 
+def _alpha_fmt(w, h, alpha):
+    return z.image_format(width=w, height=h, pixel_type=z.PIXEL_FLOAT, color_family=z.COLOR_RGB,
+                          matrix_coefficients=z.MATRIX_RGB, transfer_characteristics=z.TRANSFER_BT709,
+                          color_primaries=z.PRIMARIES_BT709, alpha=alpha)
+
+
+def test_straight_alpha_resize_premultiplies_bit_exact():
+    """RGB + straight alpha resized: premultiply (x * alpha, simple_filters.cpp:76-88), resize of all
+    four planes, unpremultiply (x / clamp(alpha, 0, 1), 0 where alpha is 0, :98-110)."""
+    print("case: FLOAT RGBA straight alpha 200x150->333x277 bicubic", flush=True)
+    r = np.random.default_rng(5)
+    w, h, dw, dh = 200, 150, 333, 277
+    rgb = [r.random((h, w), dtype=np.float32) for _ in range(3)]
+    alpha = r.random((h, w), dtype=np.float32)
+    alpha[r.random((h, w)) < 0.2] = 0.0                   # transparent pixels
+    alpha[:, :20] = 0.0                                    # a transparent band: resized alpha exactly 0
+    alpha[r.random((h, w)) < 0.05] = np.float32(1.25)      # out of range: premultiply does not clamp
+    g = z.FilterGraph(_alpha_fmt(w, h, z.ALPHA_STRAIGHT), _alpha_fmt(dw, dh, z.ALPHA_STRAIGHT))
+    outs = gpu_util.run_graph(g, rgb + [alpha], [(dh, dw, np.float32)] * 4)
+    prem = [(c * alpha).astype(np.float32) for c in rgb]
+    res = [resize_plane(p, BICUBIC, 0.0, 0.5, dw, dh) for p in prem]
+    ra = resize_plane(alpha, BICUBIC, 0.0, 0.5, dw, dh)
+    ca = np.clip(ra, np.float32(0.0), np.float32(1.0))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        want = [np.where(ca == 0, np.float32(0.0), (p / np.where(ca == 0, np.float32(1.0), ca))).astype(np.float32) for p in res]
+    for k in range(3):
+        _assert_equal(outs[k], want[k], f"plane {k}")
+    _assert_equal(outs[3], ra, "alpha")
+    assert np.count_nonzero(ca == 0) > 0 and np.count_nonzero(ra > 1) > 0
+
+
+def test_premultiplied_to_straight_alpha_bit_exact():
+    print("case: FLOAT RGBA premultiplied -> straight 96x64", flush=True)
+    r = np.random.default_rng(6)
+    w, h = 96, 64
+    alpha = r.random((h, w), dtype=np.float32)
+    alpha[r.random((h, w)) < 0.25] = 0.0
+    rgb = [(r.random((h, w), dtype=np.float32) * alpha).astype(np.float32) for _ in range(3)]
+    g = z.FilterGraph(_alpha_fmt(w, h, z.ALPHA_PREMULTIPLIED), _alpha_fmt(w, h, z.ALPHA_STRAIGHT))
+    outs = gpu_util.run_graph(g, rgb + [alpha], [(h, w, np.float32)] * 4)
+    ca = np.clip(alpha, np.float32(0.0), np.float32(1.0))
+    for k in range(3):
+        want = np.where(ca == 0, np.float32(0.0), rgb[k] / np.where(ca == 0, np.float32(1.0), ca)).astype(np.float32)
+        _assert_equal(outs[k], want, f"plane {k}")
+    _assert_equal(outs[3], alpha, "alpha")
+
+
 FLOAT_CASES = [
     (3840, 2160, 1920, 1080, LANCZOS, 3.0, 0.0),
     (1280, 720, 3840, 2160, BICUBIC, 0.0, 0.5),

@@ -66,6 +66,7 @@ public:
 		unsigned crop_left = 0, crop_top = 0;
 		int fill = 0;
 		uint32_t fill_bits = 0;
+		int alpha_op = 0;                  // 1 premultiply, 2 unpremultiply (alpha plane in in[1])
 		// CSP
 		CspProgram prog;
 		// ED (planes of equal size share one launch)

@@ -81,7 +81,9 @@ void DeviceProgram::lower()
 	std::map<int, std::vector<int>> sink_planes;
 
 	for (const Node &n : nodes) {
-		const int ndeps = n.kind == NodeKind::COLORSPACE ? 3 : (n.kind == NodeKind::SOURCE || n.kind == NodeKind::FILL ? 0 : 1);
+		const int ndeps = n.kind == NodeKind::COLORSPACE ? 3
+		                  : (n.kind == NodeKind::PREMUL || n.kind == NodeKind::UNPREMUL) ? 2
+		                  : (n.kind == NodeKind::SOURCE || n.kind == NodeKind::FILL ? 0 : 1);
 		for (int d = 0; d < ndeps; ++d)
 			readers[key(n.dep[d], n.dep_out[d])]++;
 	}
@@ -211,6 +213,17 @@ void DeviceProgram::lower()
 			s.fill_bits = n.fill_bits;
 			s.label = "fill";
 			break;
+		case NodeKind::PREMUL:
+		case NodeKind::UNPREMUL:
+			if (in_type != PixelType::FLOAT || n.out.type != PixelType::FLOAT)
+				fail(ZIMG_ERROR_UNKNOWN, "alpha operation on a non-FLOAT plane");
+			s.kind = Step::POINT;
+			s.alpha_op = n.kind == NodeKind::PREMUL ? 1 : 2;
+			s.in[1] = where[key(n.dep[1], n.dep_out[1])];
+			s.ix[1] = dev::InXform{ type_code(PixelType::FLOAT), 0, 0, 1.0f, 0.0f };
+			std::snprintf(label, sizeof(label), "%s[%d]", s.alpha_op == 1 ? "premultiply" : "unpremultiply", n.plane);
+			s.label = label;
+			break;
 		default:
 			fail(ZIMG_ERROR_UNKNOWN, "unexpected node");
 		}
@@ -247,7 +260,7 @@ void DeviceProgram::fuse()
 		int n = 0;
 		for (auto &s : m_steps) {
 			if (s.dead) continue;
-			const int nin = s.kind == Step::CSP ? 3 : 1;
+			const int nin = s.kind == Step::CSP ? 3 : s.alpha_op ? 2 : 1;
 			for (int k = 0; k < nin; ++k)
 				if (same(s.in[k], b)) ++n;
 		}
@@ -257,14 +270,16 @@ void DeviceProgram::fuse()
 	auto plain_out = [](const dev::OutXform &x) { return x.op == 0; };
 	constexpr int kFloat = 3, kHalf = 2;
 	auto is_point_depth_in = [&](const Step &s) {
-		return s.kind == Step::POINT && !s.fill && !s.crop_left && !s.crop_top && s.ix[0].op != 0 && s.ox[0].op == 0 &&
+		return s.kind == Step::POINT && !s.alpha_op && !s.fill && !s.crop_left && !s.crop_top && s.ix[0].op != 0 && s.ox[0].op == 0 &&
 		       (s.ix[0].op == 1 || s.ox[0].type == kFloat);
 	};
 	auto is_point_to_half = [&](const Step &s) {
-		return s.kind == Step::POINT && !s.fill && !s.crop_left && !s.crop_top && s.ix[0].op == 3 && s.ix[0].type == kFloat &&
+		return s.kind == Step::POINT && !s.alpha_op && !s.fill && !s.crop_left && !s.crop_top && s.ix[0].op == 3 && s.ix[0].type == kFloat &&
 		       s.ox[0].op == 0 && s.ox[0].type == kHalf;
 	};
-	auto is_point_quant = [](const Step &s) { return s.kind == Step::POINT && !s.fill && !s.crop_left && !s.crop_top && s.ix[0].op == 0 && s.ox[0].op == 1; };
+	auto is_point_quant = [](const Step &s) {
+		return s.kind == Step::POINT && !s.alpha_op && !s.fill && !s.crop_left && !s.crop_top && s.ix[0].op == 0 && s.ox[0].op == 1;
+	};
 
 	bool changed = true;
 	while (changed) {
@@ -282,7 +297,7 @@ void DeviceProgram::fuse()
 					Step &B = m_steps[ib];
 					if (B.dead)
 						continue;
-					const int nin = B.kind == Step::CSP ? 3 : 1;
+					const int nin = B.kind == Step::CSP ? 3 : B.alpha_op ? 2 : 1;
 					int kin = -1;
 					for (int k = 0; k < nin; ++k)
 						if (same(B.in[k], mid)) kin = k;
@@ -1284,6 +1299,9 @@ void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void
 			a.out = s.ox[0];
 			a.fill = s.fill;
 			a.fill_bits = s.fill_bits;
+			a.alpha_op = s.alpha_op;
+			if (s.alpha_op)
+				a.alpha = resolve(s.in[1]);
 			rc = dev::launch_point(a, stream);
 			break;
 		}

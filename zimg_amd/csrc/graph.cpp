@@ -283,6 +283,30 @@ class Planner {
 		m_state.p[PV].format = f;
 	}
 
+	// premultiply / unpremultiply (graphbuilder.cpp:661-692): one node per
+	// colour plane, reading the colour plane and the alpha plane (FLOAT 4:4:4).
+	void alpha_op(NodeKind kind, AlphaType result)
+	{
+		if (m_state.p[PY].format.type != PixelType::FLOAT || m_state.p[PA].format.type != PixelType::FLOAT ||
+		    (m_state.has_chroma() && (m_state.p[PU].width != m_state.p[PY].width || m_state.p[PU].height != m_state.p[PY].height)))
+			internal("alpha operation outside FLOAT 4:4:4");
+		trace(kind == NodeKind::PREMUL ? "premultiply" : "unpremultiply");
+		for (int q = 0; q < (m_state.has_chroma() ? 3 : 1); ++q) {
+			Node n;
+			n.kind = kind;
+			n.width = m_state.p[q].width;
+			n.height = m_state.p[q].height;
+			n.out = m_state.p[q].format;
+			n.plane = q;
+			n.dep[0] = m_ids[q].node;
+			n.dep_out[0] = m_ids[q].out;
+			n.dep[1] = m_ids[PA].node;
+			n.dep_out[1] = m_ids[PA].out;
+			m_ids[q] = PlaneRef{ add_node(n), 0 };
+		}
+		m_state.alpha = result;
+	}
+
 	void add_opaque_alpha(AlphaType type)
 	{
 		trace("add_opaque");
@@ -628,16 +652,36 @@ public:
 		if (m_state.has_alpha())
 			add_source(PA, m_state.p[PA].width, m_state.p[PA].height);
 
-		// connect_internal (graphbuilder.cpp:1021-1072). Premultiplication is not lowered.
-		if (needs_premul(target))
-			fail(ZIMG_ERROR_UNSUPPORTED_OPERATION, "alpha premultiplication not supported");
+		// connect_internal (graphbuilder.cpp:1021-1072).
+		if (needs_premul(target)) {
+			const auto orig_plane = m_state.p[PA];
+			const PlaneRef orig_ref = m_ids[PA];
+			const State tmp = float_444(m_state, true);
+			connect_color_channels(tmp);
+			connect_plane(tmp, Mode::ALPHA, false);
+			alpha_op(NodeKind::PREMUL, AlphaType::PREMULTIPLIED);
+			if (target.has_alpha() && target.p[PA] == orig_plane) {
+				m_ids[PA] = orig_ref;
+				m_state.p[PA] = orig_plane;
+			}
+		}
 		if (m_state.has_alpha() && !target.has_alpha()) {
 			trace("discard_alpha");
 			m_ids[PA] = PlaneRef{};
 			m_state.alpha = AlphaType::NONE;
 		}
-		if (m_state.alpha == AlphaType::PREMULTIPLIED && target.alpha == AlphaType::STRAIGHT)
-			fail(ZIMG_ERROR_UNSUPPORTED_OPERATION, "alpha unpremultiplication not supported");
+		if (m_state.alpha == AlphaType::PREMULTIPLIED && target.alpha == AlphaType::STRAIGHT) {
+			const auto orig_plane = m_state.p[PA];
+			const PlaneRef orig_ref = m_ids[PA];
+			const State tmp = float_444(target, true);
+			connect_color_channels(tmp);
+			connect_plane(tmp, Mode::ALPHA, false);
+			alpha_op(NodeKind::UNPREMUL, AlphaType::STRAIGHT);
+			if (target.has_alpha() && target.p[PA] == orig_plane) {
+				m_ids[PA] = orig_ref;
+				m_state.p[PA] = orig_plane;
+			}
+		}
 
 		connect_color_channels(target);
 		if (m_state.has_alpha()) {

@@ -48,7 +48,9 @@ struct BuildParams {
 	bool chromatic_adaptation = false;
 };
 
-enum class NodeKind : int { SOURCE, DEPTH, RESIZE_H, RESIZE_V, COLORSPACE, COPY_RECT, FILL };
+// PREMUL / UNPREMUL: FLOAT colour plane (dep 0) times / divided by the FLOAT
+// alpha plane (dep 1), PremultiplyFilter / UnpremultiplyFilter (simple_filters.cpp:76-110).
+enum class NodeKind : int { SOURCE, DEPTH, RESIZE_H, RESIZE_V, COLORSPACE, COPY_RECT, FILL, PREMUL, UNPREMUL };
 
 // A node produces one plane (COLORSPACE produces three; consumers name the output index).
 struct Node {

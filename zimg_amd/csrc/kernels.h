@@ -122,6 +122,8 @@ struct PointArg {
 	OutXform out;
 	int fill;           // 1: ignore src, write fill_bits
 	uint32_t fill_bits;
+	int alpha_op;       // 1: FLOAT src x alpha, 2: FLOAT src / alpha (premultiply, unpremultiply)
+	PlaneArg alpha;     // FLOAT alpha plane of alpha_op
 };
 
 struct CspArg {

@@ -1273,6 +1273,22 @@ __global__ void __launch_bounds__(kThreads) k_point(PointArg a)
 	}
 	const char *srow = row_ptr(frame_ptr(a.src, f), a.src.stride, i + a.crop_top);
 	const unsigned js = j + a.crop_left;
+	if (a.alpha_op) {
+		// PremultiplyFilter / UnpremultiplyFilter (simple_filters.cpp:76-110), FLOAT
+		// planes: x * alpha (the reference multiplies by the unclamped alpha);
+		// x / alpha with alpha clamped to [0, 1] and 0 where it is 0.
+		const float x = reinterpret_cast<const float *>(srow)[j];
+		const float al = reinterpret_cast<const float *>(row_ptr(frame_ptr(a.alpha, f), a.alpha.stride, i))[j];
+		float r;
+		if (a.alpha_op == 1) {
+			r = x * al;
+		} else {
+			const float c = fminf(fmaxf(al, 0.0f), 1.0f);
+			r = c == 0.0f ? 0.0f : x / c;
+		}
+		reinterpret_cast<float *>(drow)[j] = r;
+		return;
+	}
 	if (a.in.op == 1) {
 		store_uint(drow, j, a.out.type, load_as_uint(srow, js, a.in.type) << a.in.shift);
 		return;
