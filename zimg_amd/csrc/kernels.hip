@@ -590,6 +590,17 @@ struct HRegs {
 	__device__ __forceinline__ void apply_float(const float *row, int fw, float &v0, float &v1) const
 	{
 		float a0 = 0.0f, a1 = 0.0f;
+		if (fw == N) {
+			// every tap is real: the same sums without selects
+#pragma unroll
+			for (int k = 0; k < N; ++k) {
+				a0 = a0 + c0[k] * row[l0 + k];
+				a1 = a1 + c1[k] * row[l1 + k];
+			}
+			v0 = a0;
+			v1 = a1;
+			return;
+		}
 #pragma unroll
 		for (int k = 0; k < N; ++k) {
 			const int kk = k < fw ? k : fw - 1; // stays inside the window; the sum ignores it
@@ -832,7 +843,23 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 					const float *m = mid + (vtop[i] - r0) * pm + j4;
 					const uint32_t *cv = vcoef + i * vstride;
 					float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
-					if (vtaps <= static_cast<unsigned>(N)) {
+					if (vtaps == static_cast<unsigned>(N)) {
+						// every tap is real: N taps unrolled, all LDS reads issued first
+						float4 mm[N];
+						float cc[N];
+#pragma unroll
+						for (int k = 0; k < N; ++k) {
+							mm[k] = *reinterpret_cast<const float4 *>(m + k * pm);
+							cc[k] = __uint_as_float(cv[k]);
+						}
+#pragma unroll
+						for (int k = 0; k < N; ++k) {
+							a0 = a0 + cc[k] * mm[k].x;
+							a1 = a1 + cc[k] * mm[k].y;
+							a2 = a2 + cc[k] * mm[k].z;
+							a3 = a3 + cc[k] * mm[k].w;
+						}
+					} else if (vtaps < static_cast<unsigned>(N)) {
 						// N taps unrolled, all LDS reads issued first; taps past vtaps
 						// leave the sums as they are (select), so every kept addition
 						// is the reference's, in its order.
@@ -968,7 +995,21 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 				} else {
 					const float *xr = xin + top * pin + 2 * q;
 					float a0 = 0.0f, a1 = 0.0f;
-					if (vtaps <= static_cast<unsigned>(N)) {
+					if (vtaps == static_cast<unsigned>(N)) {
+						// every tap is real: unrolled over N taps without selects
+						float2 mv[N];
+						float cc[N];
+#pragma unroll
+						for (int k = 0; k < N; ++k) {
+							mv[k] = *reinterpret_cast<const float2 *>(xr + k * pin);
+							cc[k] = __uint_as_float(cv[k]);
+						}
+#pragma unroll
+						for (int k = 0; k < N; ++k) {
+							a0 = a0 + cc[k] * mv[k].x;
+							a1 = a1 + cc[k] * mv[k].y;
+						}
+					} else if (vtaps < static_cast<unsigned>(N)) {
 						// Unrolled over N taps; taps past vtaps leave the sums as they are.
 						float2 mv[N];
 						float cc[N];
