@@ -587,20 +587,33 @@ struct HRegs {
 		a1 = s1;
 	}
 
-	__device__ __forceinline__ void apply_float(const float *row, int fw, float &v0, float &v1) const
+	// The first M taps, all real: the same sums without selects.
+	template <int M>
+	__device__ __forceinline__ void apply_float_n(const float *row, float &v0, float &v1) const
 	{
 		float a0 = 0.0f, a1 = 0.0f;
-		if (fw == N) {
-			// every tap is real: the same sums without selects
 #pragma unroll
-			for (int k = 0; k < N; ++k) {
-				a0 = a0 + c0[k] * row[l0 + k];
-				a1 = a1 + c1[k] * row[l1 + k];
-			}
-			v0 = a0;
-			v1 = a1;
+		for (int k = 0; k < M; ++k) {
+			a0 = a0 + c0[k] * row[l0 + k];
+			a1 = a1 + c1[k] * row[l1 + k];
+		}
+		v0 = a0;
+		v1 = a1;
+	}
+
+	__device__ __forceinline__ void apply_float(const float *row, int fw, float &v0, float &v1) const
+	{
+		if (fw == N) {
+			apply_float_n<N>(row, v0, v1);
 			return;
 		}
+		if constexpr (N > 2) {
+			if (fw == N - 1) {
+				apply_float_n<N - 1>(row, v0, v1);
+				return;
+			}
+		}
+		float a0 = 0.0f, a1 = 0.0f;
 #pragma unroll
 		for (int k = 0; k < N; ++k) {
 			const int kk = k < fw ? k : fw - 1; // stays inside the window; the sum ignores it
