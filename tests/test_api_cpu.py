@@ -8,6 +8,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 import zimg_amd as z
@@ -198,6 +199,26 @@ def test_single_plane_lanczos_is_one_fused_launch():
     # both passes on the int8 matrix cores in one launch, no HBM intermediate
     assert any(l.startswith("  integer matrix cores: one launch") for l in dev)
     assert g.device_tmp_size(8) == 0
+
+
+def test_integer_matrix_core_plans_in_bounds():
+    """Graph build re-derives every LDS index of the integer matrix-core kernel
+    for its plan and fails on any outside its arrays: a sweep of random WORD
+    upscales (filters, odd sizes, shifts from chroma siting) must all build."""
+    rng = np.random.default_rng(11)
+    filters = [(z.RESIZE_BILINEAR, float("nan"), float("nan")), (z.RESIZE_BICUBIC, 0.0, 0.5),
+               (z.RESIZE_SPLINE16, float("nan"), float("nan")), (z.RESIZE_SPLINE36, float("nan"), float("nan")),
+               (z.RESIZE_LANCZOS, 3.0, float("nan"))]
+    seen = 0
+    for _ in range(150):
+        sw, sh = int(rng.integers(16, 700)), int(rng.integers(16, 500))
+        dw, dh = int(sw * rng.uniform(1.5, 6.0)), int(sh * rng.uniform(1.5, 6.0))
+        ft, a, b = filters[int(rng.integers(len(filters)))]
+        src = z.image_format(width=sw, height=sh, pixel_type=z.PIXEL_WORD, depth=int(rng.choice([10, 12, 16])))
+        dst = z.image_format(width=dw, height=dh, pixel_type=z.PIXEL_WORD, depth=src.depth)
+        g = z.FilterGraph(src, dst, z.builder_params(resample_filter=ft, filter_param_a=a, filter_param_b=b))
+        seen += "integer matrix cores" in g.describe()
+    assert seen > 100
 
 
 def test_integer_matrix_core_form_only_where_windows_fit():

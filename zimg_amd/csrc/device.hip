@@ -777,6 +777,29 @@ ImmaPlan DeviceProgram::imma_plan(const Step &s) const
 	p.in_pitch = odd_stride(in_cols);
 	p.lds = 2 * dev::kImmaTileW * p.mid_pitch + std::max(2 * 16 * mb * p.in_pitch, 2 * 4 * 16 * dev::kImmaOutPitch);
 	p.ok = p.lds <= 64 * 1024;
+	if (p.ok) {
+		// Every LDS index the kernel forms, re-derived with its formulas: the
+		// horizontal windows (2 x 4 dwords from (kb - cb) / 2 + 2 (g % 2)) lie in the
+		// staged columns, the vertical windows (three dwords from byte
+		// (kb - rbase + 8 (g % 2)) & ~3) in the computed intermediate rows and
+		// in the column's mid_pitch bytes. Blocks past the plane repeat the last.
+		for (unsigned t0 = 0; t0 < nh; t0 += cpt) {
+			const int cb = kh[t0] & ~15;
+			for (unsigned k = 0; k < cpt; ++k) {
+				const int b = kh[std::min(t0 + k, nh - 1)];
+				if (b < cb || (b - cb) % 2 || static_cast<unsigned>(b - cb) / 2 + 2 + 5 >= p.in_pitch / 2 ||
+				    static_cast<unsigned>(b - cb) + 16 > in_cols)
+					fail(ZIMG_ERROR_UNKNOWN, "integer matrix-core plan: horizontal window outside the staged columns");
+			}
+		}
+		for (unsigned t0 = 0; t0 < nv; t0 += rb) {
+			for (unsigned k = 0; k < rb; ++k) {
+				const int d = kv[std::min(t0 + k, nv - 1)] - kv[t0];
+				if (d < 0 || static_cast<unsigned>(d) + 16 > 16 * mb || ((static_cast<unsigned>(d) + 8) & ~3u) + 12 > p.mid_pitch)
+					fail(ZIMG_ERROR_UNKNOWN, "integer matrix-core plan: vertical window outside the intermediate rows");
+			}
+		}
+	}
 	return p;
 }
 
