@@ -1100,9 +1100,12 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 // horizontal pass: wave w takes column blocks 2w, 2w + 1 for each of the MB
 // 16-row blocks; A = 16 rows x [xh | xl'] (lane l: row l % 16, window
 // samples 4 (l / 16 % 2) + 0..3 and + 8..11, high bytes for l < 32, low
-// bytes ^ 0x80 else; the coefficient table uses the same order). The result (lane: rows 4 (l / 16) + r of column l % 16) is stored
-// as bytes (x ^ 0x80) into two LDS planes, high and low, one column after
-// another (mid_pitch bytes each), so a column's rows are contiguous.
+// bytes ^ 0x80 else; the coefficient table uses the same order). The
+// result (lane: rows 4 (l / 16) + r of column l % 16) is stored as bytes
+// (x ^ 0x80) into two LDS planes, high and low, one column after another
+// (mid_pitch bytes each), so a column's rows are contiguous. Each wave then
+// copies its output rows to the staged source area and stores them as whole
+// row segments.
 // Vertical pass: wave w takes row blocks w and w + 4; per column block
 // A = 16 columns x [yh | yl'] of the block's window rows (three dwords
 // aligned with v_alignbyte), B the block's coefficients, so D holds 4
