@@ -444,15 +444,26 @@ __device__ __forceinline__ void stage_tile(lds_t<WORD> *xin, unsigned pin, const
 	const unsigned cpr = nc / ce; // chunks per row
 	const unsigned total = nr * cpr;
 	const unsigned last_full = src_w >= ce ? (src_w - ce) / ce * ce : 0; // last chunk start inside the row
+	// Chunk e = base + u kThreads + thread as (row, column), advanced by
+	// kThreads chunks at a time without a division; chunks past the end are
+	// the last chunk (row nr - 1, column cpr - 1).
+	const unsigned step_rows = kThreads / cpr, step_cols = kThreads % cpr;
+	unsigned row = threadIdx.x / cpr, col = threadIdx.x - (threadIdx.x / cpr) * cpr;
 	for (unsigned base = 0; base < total; base += U * kThreads) {
 		uint4 raw[U];
 		unsigned rr[U], cc[U];
 		bool full[U];
 #pragma unroll
 		for (int u = 0; u < U; ++u) {
-			const unsigned e = min(base + u * kThreads + threadIdx.x, total - 1);
-			rr[u] = e / cpr;
-			cc[u] = c0 + (e - rr[u] * cpr) * ce;
+			const bool past = row >= nr;
+			rr[u] = past ? nr - 1 : row;
+			cc[u] = c0 + (past ? cpr - 1 : col) * ce;
+			col += step_cols;
+			row += step_rows;
+			if (col >= cpr) {
+				col -= cpr;
+				++row;
+			}
 			const unsigned r = min(r0 + rr[u], rows_valid - 1);
 			full[u] = r0 + rr[u] < rows_valid && cc[u] + ce <= src_w;
 			const unsigned cl = min(cc[u], last_full);
