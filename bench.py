@@ -34,7 +34,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--frames", type=int, default=64, help="distinct frames per step (>256 MB of input+output defeats the MALL)")
+    ap.add_argument("--frames", type=int, default=None,
+                    help="distinct frames per step (>256 MB of input+output defeats the MALL); default 64, and 85 for "
+                         "uhd_pq_to_hlg, whose error diffusion runs one workgroup per (frame, plane): 85 x 3 = 255 "
+                         "workgroups fill the 256 CUs")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="bounded CPU baseline sample (seconds of work)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="fhd_709_to_2020", choices=sorted(WORKLOADS))
@@ -282,6 +285,8 @@ def main():
                                  "launches_per_step": b["per_job_launches"]},
             "speedup_vs_per_job": round(b["fps"] / b["per_job_fps"], 3)}))
         return
+    if args.frames is None:
+        args.frames = 85 if primary == "uhd_pq_to_hlg" else 64
     secondary = args.secondary if args.secondary and args.secondary != primary else None
     m = measure(torch, primary, args.frames, args.steps, args.warmup, args.precision)
     m2 = measure(torch, secondary, min(args.frames, 32), max(args.steps // 2, 5), args.warmup, args.precision) if secondary else None
