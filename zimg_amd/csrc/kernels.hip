@@ -526,7 +526,28 @@ __device__ __forceinline__ void stage_tile(lds_t<WORD> *xin, unsigned pin, const
 						float x = in.type == 2 ? half_to_float(static_cast<uint16_t>(bits)) : static_cast<float>(bits);
 						return in.op == 2 ? x * in.scale + in.offset : x;
 					};
-					if (esz == 2) {
+					if (esz == 2 && in.type != 2) {
+						// WORD: the conversion chosen once per chunk (uniform), not per sample
+						const float sc = in.op == 2 ? in.scale : 1.0f, of = in.op == 2 ? in.offset : 0.0f;
+						auto cw = [&](uint32_t bits) {
+							const float x = static_cast<float>(bits);
+							return in.op == 2 ? x * sc + of : x;
+						};
+						if (in.op == 2) {
+#pragma unroll
+							for (unsigned k = 0; k < 8; k += 4) {
+								const uint32_t lo = wv[k / 2], hi = wv[k / 2 + 1];
+								*reinterpret_cast<float4 *>(d + k) = make_float4(static_cast<float>(lo & 0xFFFFu) * sc + of, static_cast<float>(lo >> 16) * sc + of,
+								                                                 static_cast<float>(hi & 0xFFFFu) * sc + of, static_cast<float>(hi >> 16) * sc + of);
+							}
+						} else {
+#pragma unroll
+							for (unsigned k = 0; k < 8; k += 4) {
+								const uint32_t lo = wv[k / 2], hi = wv[k / 2 + 1];
+								*reinterpret_cast<float4 *>(d + k) = make_float4(cw(lo & 0xFFFFu), cw(lo >> 16), cw(hi & 0xFFFFu), cw(hi >> 16));
+							}
+						}
+					} else if (esz == 2) {
 #pragma unroll
 						for (unsigned k = 0; k < 8; k += 4) {
 							const uint32_t lo = wv[k / 2], hi = wv[k / 2 + 1];
