@@ -1,23 +1,39 @@
 #!/usr/bin/env python3
 """Benchmark of the per-plane path on one MI355X.
 
-Workload (one step = one batched launch sequence over --frames distinct frames
-already resident in HBM):
+A step is one batched launch sequence of a workload over F distinct frames
+already resident in HBM (F = 64 by default, so each step moves more than the
+256 MB MALL holds). Workloads (sizes and inputs in zimg_amd/workloads.py):
+
   fhd_709_to_2020  FHD 1920x1080 YUV 4:2:0 10-bit limited, BT.709 -> BT.2020 NCL,
-                   bicubic chroma, FP32 intermediates (the headline colourspace case)
-Also reported (secondary, same clock discipline):
-  uhd_lanczos      one WORD plane 1920x1080 -> 3840x2160 Lanczos3 (one fused launch)
+                   bicubic chroma, FP32 intermediates          (primary: `value`)
+  uhd_lanczos      WORD 1920x1080 -> 3840x2160 Lanczos3
+  fp32_down        FP32 3840x2160 -> 1920x1080 Lanczos3   (exact, and FP16 hi+lo MFMA)
+  fp32_up          FP32 1280x720 -> 3840x2160 bicubic     (exact, and FP16 hi+lo MFMA)
+  batch64          64 uneven 4:2:0 jobs (seeded list), one persistent launch
+  uhd_pq_to_hlg    UHD RGB FP32 PQ -> HLG -> 12-bit error diffusion
 
-value = frames/s over the timed steps. roofline.achieved = algorithmic bytes
-per launch sequence / its average duration, measured with HIP events recorded
-on the stream the kernels run on. cpu_baseline = the C oracle (the project's
-restatement of the reference C path) on this host, one core, bounded sample.
+Each workload reports frames/s over K timed steps (HIP events on the launch
+stream) and a roofline object: `achieved` is the workload's algorithmic bytes
+per frame (input planes read once, output planes written once) times the
+frames one launch of the dominant kernel processes, divided by that kernel's
+average duration (HIP events around each stage, on the same stream);
+`whole_graph` divides the same bytes by the whole step. `traffic` is the
+dominant kernel's HBM bytes per launch from the committed rocprofv3 counter
+summary (profiles/<workload>_counters.txt); `binding` names what bounds the
+kernel when it is not HBM (VALU issue, from the same counters).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--frames F]
+cpu_baseline: the oracle (the project's C restatement of the reference's C
+path) on this host: single-thread median of >= 10 runs, and frame-parallel
+over the usable cores (one frame per thread, the reference harness pattern of
+graphapp.cpp:355-426), with the CPU model.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
 """
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -27,6 +43,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+METRIC = "frames/s + HBM GB/s, UHD Lanczos & 709→2020; MFMA/learned vs plain at ≤0.5 LSB"
+WORKLOADS = ["fhd_709_to_2020", "uhd_lanczos", "fp32_down", "fp32_up", "batch64", "uhd_pq_to_hlg"]
 
 
 def parse():
@@ -34,16 +52,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--frames", type=int, default=None,
-                    help="distinct frames per step (>256 MB of input+output defeats the MALL); default 64, and 85 for "
-                         "uhd_pq_to_hlg, whose error diffusion runs one workgroup per (frame, plane): 85 x 3 = 255 "
-                         "workgroups fill the 256 CUs")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="bounded CPU baseline sample (seconds of work)")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="fhd_709_to_2020", choices=sorted(WORKLOADS))
-    ap.add_argument("--secondary", default="uhd_lanczos", help="second workload reported in the same line ('' for none)")
-    ap.add_argument("--precision", default="exact", choices=["exact", "fp16", "bf16"],
-                    help="FLOAT resize arithmetic: exact FP32 order, or matrix cores with FP16/BF16 hi+lo operands")
+    ap.add_argument("--frames", type=int, default=None, help="distinct frames per step (default 64)")
+    ap.add_argument("--workload", default=None, choices=WORKLOADS,
+                    help="measure one workload only (its line becomes the primary); default: all, fhd_709_to_2020 primary")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--cpu-runs", type=int, default=10, help="runs per CPU baseline mode (median reported)")
     return ap.parse_args()
 
 
@@ -86,6 +99,7 @@ def device_batch(torch, graph, planes, out_shapes, frames):
 
 
 def time_steps(torch, launch, steps, warmup):
+    """Seconds for `steps` launches, HIP events on the current (launch) stream."""
     for _ in range(warmup):
         launch()
     torch.cuda.synchronize()
@@ -96,22 +110,102 @@ def time_steps(torch, launch, steps, warmup):
         launch()
     end.record()
     torch.cuda.synchronize()
-    return start.elapsed_time(end) / 1000.0  # seconds (events on the launch stream)
+    return start.elapsed_time(end) / 1000.0
 
 
-WORKLOADS = {"fhd_709_to_2020", "uhd_lanczos", "fp32_down", "fp32_up", "uhd_pq_to_hlg", "batch64"}
+def build_workload(name, frames):
+    """(graph, input stacks, output shapes, algorithmic bytes per frame, dtype)."""
+    import zimg_amd as z
+    from zimg_amd import workloads as W
+    if name in ("fp32_down", "fp32_up"):
+        sw, sh, dw, dh, filt = (3840, 2160, 1920, 1080, z.RESIZE_LANCZOS) if name == "fp32_down" else (1280, 720, 3840, 2160, z.RESIZE_BICUBIC)
+        s, d, p = W.fp32_formats(sw, sh, dw, dh, filt, 3.0 if filt == z.RESIZE_LANCZOS else float("nan"))
+        g = z.FilterGraph(s, d, p)
+        base = W.fp32_plane(sw, sh, seed=8)
+        planes = [np.stack([np.roll(base, 17 * k, axis=1) for k in range(frames)]), None, None, None]
+        return g, planes, [(dh, dw, np.float32), None, None, None], (sw * sh + dw * dh) * 4, "f32"
+    if name == "uhd_pq_to_hlg":
+        s, d, p = W.uhd_pq_to_hlg_formats()
+        g = z.FilterGraph(s, d, p)
+        R, G, B = W.pq_rgb(3840, 2160, seed=9)
+        planes = [np.stack([np.roll(c, 29 * k, axis=1) for k in range(frames)]) for c in (R, G, B)] + [None]
+        return g, planes, [(2160, 3840, np.uint16)] * 3 + [None], 3840 * 2160 * 3 * (4 + 2), "f32 in, u16 (12-bit) out"
+    if name == "fhd_709_to_2020":
+        s, d, p = W.fhd_709_to_2020_formats()
+        g = z.FilterGraph(s, d, p)
+        Ys, Us, Vs = [], [], []
+        for k in range(frames):
+            Y, U, V = W.yuv420_10bit(1920, 1080, seed=1000 + k)
+            Ys.append(Y), Us.append(U), Vs.append(V)
+        planes = [np.stack(Ys), np.stack(Us), np.stack(Vs), None]
+        outs = [(1080, 1920, np.uint16), (540, 960, np.uint16), (540, 960, np.uint16), None]
+        return g, planes, outs, 2 * (1920 * 1080 * 2 + 2 * 960 * 540 * 2), "u16/f32 (10-bit in/out, FP32 intermediates)"
+    s, d, p = W.uhd_lanczos_formats()
+    g = z.FilterGraph(s, d, p)
+    base = W.word_plane(1920, 1080, seed=7)
+    planes = [np.stack([np.roll(base, 13 * k, axis=1) for k in range(frames)]), None, None, None]
+    return g, planes, [(2160, 3840, np.uint16), None, None, None], 1920 * 1080 * 2 + 3840 * 2160 * 2, "u16 (Q1.14 integer taps)"
+
+
+def counters_of(key):
+    """Key: value lines of the committed rocprofv3 counter summary profiles/<key>_counters.txt."""
+    path = os.path.join(ROOT, "profiles", f"{key}_counters.txt")
+    out = {}
+    if not os.path.exists(path):
+        return out
+    for line in open(path):
+        if ":" in line and not line.startswith((" ", "#")):
+            k, v = line.split(":", 1)
+            try:
+                out[k.strip()] = float(v)
+            except ValueError:
+                out[k.strip()] = v.strip()
+    return out
+
+
+def measure(torch, name, frames, steps, warmup, precision=0):
+    g, planes, outs, algo_bytes, dtype = build_workload(name, frames)
+    if precision:
+        g.set_resize_precision(precision)
+        dtype += f" (resize on MFMA, {'fp16' if precision == 1 else 'bf16'} hi+lo, FP32 accumulate)"
+    launch, keep = device_batch(torch, g, planes, outs, frames)
+    elapsed = time_steps(torch, launch, steps, warmup)
+    per_step = elapsed / steps
+    stages = g.device_stages()
+    acc = None
+    for _ in range(steps):
+        st = launch.profiled()
+        acc = [a + b[0] for a, b in zip(acc, st)] if acc else [b[0] for b in st]
+    stage_ms = [ms / steps for ms in acc]
+    k = int(np.argmax(stage_ms))
+    key = name if not precision else f"{name}_{'fp16' if precision == 1 else 'bf16'}"
+    ctr = counters_of(key)
+    dom_ms = stage_ms[k]
+    achieved = algo_bytes * frames / (dom_ms * 1e-3) / 1e9
+    whole = algo_bytes * frames / per_step / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": ctr.get("traffic_bytes_per_launch"), "kernel": stages[k] if k < len(stages) else str(k),
+            "kernel_ms_per_launch": round(dom_ms, 4), "algorithmic_bytes_per_launch": algo_bytes * frames,
+            "whole_graph": {"achieved": round(whole, 1), "frac": round(whole / PEAK_HBM_GBS, 4)},
+            "stages": [{"ms": round(ms, 4), "stage": (stages[i] if i < len(stages) else str(i))[:72]} for i, ms in enumerate(stage_ms)]}
+    if "binding" in ctr:
+        roof["binding"] = {kk: ctr[kk] for kk in ("binding", "valu_insts_per_px", "valu_issue_frac", "mfma_busy_frac") if kk in ctr}
+    del keep
+    torch.cuda.empty_cache()
+    return {"fps": round(frames * steps / elapsed, 1), "ms_per_step": round(per_step * 1e3, 4), "frames": frames, "launches": len(stages),
+            "dtype": dtype, "roofline": roof}
 
 
 def measure_batch(torch, steps, warmup):
-    """64 uneven YUV 4:2:0 jobs (seeded list, zimg_amd/workloads.py): one persistent
-    autotuned launch replayed from a HIP graph, against each job's own launches."""
+    """64 uneven jobs: one persistent launch (autotuned, replayed from a HIP graph)
+    against each job's own launches."""
     import zimg_amd as z
     from zimg_amd import workloads as W
     jobs = W.batch_jobs(64)
     stream = torch.cuda.current_stream().cuda_stream
     cache, keep, graphs, srcs, dsts = {}, [], [], [], []
     algo = 0
-    for k, j in enumerate(jobs):
+    for j in jobs:
         key = (tuple(j["src"]), j["bits"])
         if key not in cache:
             cache[key] = W.batch_job_planes(j, seed=900 + len(cache))
@@ -139,124 +233,94 @@ def measure_batch(torch, steps, warmup):
     t_job = time_steps(torch, per_job, steps, warmup)
     jb = z.DeviceJobs(graphs, srcs, dsts, None, z.DEVICE_JOBS_AUTOTUNE, stream)
     t_pers = time_steps(torch, lambda: jb.run(stream), steps, warmup)
-    launches = sum(len(g.describe().split("device:\n")[1].splitlines()) for g in graphs)
-    return {"fps": 64 * steps / t_pers, "ms_per_step": t_pers / steps * 1e3, "achieved_gbs": algo * steps / t_pers / 1e9,
-            "per_job_fps": 64 * steps / t_job, "per_job_gbs": algo * steps / t_job / 1e9, "per_job_launches": launches,
-            "describe": jb.describe().strip(), "algo_bytes_per_step": algo}
+    ctr = counters_of("batch64")
+    achieved = algo * steps / t_pers / 1e9
+    per_job_launches = sum(len(g.device_stages()) for g in graphs)
+    return {"fps": round(64 * steps / t_pers, 1), "unit": "jobs/s", "ms_per_step": round(t_pers / steps * 1e3, 4), "frames": 64,
+            "launches": 1, "dtype": "u8/u16 in, WORD Q1.14 taps, u8 out (Bayer dither)",
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": ctr.get("traffic_bytes_per_launch"),
+                         "kernel": jb.describe().strip(), "kernel_ms_per_launch": round(t_pers / steps * 1e3, 4),
+                         "algorithmic_bytes_per_launch": algo},
+            "per_job_launches": {"fps": round(64 * steps / t_job, 1), "achieved_gbs": round(algo * steps / t_job / 1e9, 1),
+                                 "launches_per_step": per_job_launches},
+            "speedup_vs_per_job": round(t_job / t_pers, 3)}
 
 
-def build_workload(name, frames):
-    import zimg_amd as z
-    from zimg_amd import workloads as W
-    if name in ("fp32_down", "fp32_up"):
-        sw, sh, dw, dh, filt = (3840, 2160, 1920, 1080, z.RESIZE_LANCZOS) if name == "fp32_down" else (1280, 720, 3840, 2160, z.RESIZE_BICUBIC)
-        s, d, p = W.fp32_formats(sw, sh, dw, dh, filt, 3.0 if filt == z.RESIZE_LANCZOS else float("nan"))
-        g = z.FilterGraph(s, d, p)
-        base = W.fp32_plane(sw, sh, seed=8)
-        planes = [np.stack([np.roll(base, 17 * k, axis=1) for k in range(frames)]), None, None, None]
-        return g, planes, [(dh, dw, np.float32), None, None, None], (sw * sh + dw * dh) * 4, "f32"
-    if name == "uhd_pq_to_hlg":
-        s, d, p = W.uhd_pq_to_hlg_formats()
-        g = z.FilterGraph(s, d, p)
-        R, G, B = W.pq_rgb(3840, 2160, seed=9)
-        planes = [np.stack([np.roll(c, 29 * k, axis=1) for k in range(frames)]) for c in (R, G, B)] + [None]
-        return g, planes, [(2160, 3840, np.uint16)] * 3 + [None], 3840 * 2160 * 3 * (4 + 2), "f32 in, u16 (12-bit) out"
-    if name == "fhd_709_to_2020":
-        s, d, p = W.fhd_709_to_2020_formats()
-        g = z.FilterGraph(s, d, p)
-        Ys, Us, Vs = [], [], []
-        for k in range(frames):
-            Y, U, V = W.yuv420_10bit(1920, 1080, seed=1000 + k)
-            Ys.append(Y), Us.append(U), Vs.append(V)
-        planes = [np.stack(Ys), np.stack(Us), np.stack(Vs), None]
-        outs = [(1080, 1920, np.uint16), (540, 960, np.uint16), (540, 960, np.uint16), None]
-        algo_bytes = 2 * (1920 * 1080 * 2 + 2 * 960 * 540 * 2)
-        return g, planes, outs, algo_bytes, "u16/f32 (10-bit in/out, FP32 intermediates)"
-    s, d, p = W.uhd_lanczos_formats()
-    g = z.FilterGraph(s, d, p)
-    base = W.word_plane(1920, 1080, seed=7)
-    # distinct frames cheaply: roll the seeded plane
-    planes = [np.stack([np.roll(base, 13 * k, axis=1) for k in range(frames)]), None, None, None]
-    outs = [(2160, 3840, np.uint16), None, None, None]
-    algo_bytes = 1920 * 1080 * 2 + 3840 * 2160 * 2
-    return g, planes, outs, algo_bytes, "u16 (Q1.14 integer taps)"
+# ---- CPU baseline --------------------------------------------------------------
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
-def cpu_baseline(name, seconds):
-    """The C oracle on this host, one thread, whole frames until `seconds` of work."""
+def usable_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_frame_fn(name):
+    """One whole frame of the workload through the oracle (tests/pipelines.py)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import pipelines as P
     from zimg_amd import workloads as W
-    n = 0
-    t0 = time.perf_counter()
     if name == "fhd_709_to_2020":
         Y, U, V = W.yuv420_10bit(1920, 1080, seed=99)
-        while True:
-            P.yuv420_10_709_to_2020(Y, U, V)
-            n += 1
-            if time.perf_counter() - t0 >= seconds:
-                break
-    else:
+        return lambda: P.yuv420_10_709_to_2020(Y, U, V)
+    if name == "uhd_lanczos":
         src = W.word_plane(1920, 1080, seed=99)
-        while True:
-            P.word_resize(src, 3840, 2160)
-            n += 1
-            if time.perf_counter() - t0 >= seconds:
-                break
-    dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "frames/s", "cores": 1, "kind": "port",
-            "sample": f"{n} whole {name} frames through the C oracle (tests/pipelines.py), {dt:.1f} s, 1 thread"}
-
-
-PRECISION = {"exact": 0, "fp16": 1, "bf16": 2}
-
-
-def measure(torch, name, frames, steps, warmup, precision="exact"):
-    g, planes, outs, algo_bytes, dtype = build_workload(name, frames)
-    if precision != "exact":
-        g.set_resize_precision(PRECISION[precision])
-        dtype += f" (resize on MFMA, {precision} hi+lo, FP32 accumulate)"
-    launch, keep = device_batch(torch, g, planes, outs, frames)
-    elapsed = time_steps(torch, launch, steps, warmup)
-    per_step = elapsed / steps
-    fps = frames * steps / elapsed
-    achieved = algo_bytes * frames / per_step / 1e9
-    launches = [l for l in g.describe().split("device:\n")[1].splitlines() if l]
-    # Per-stage durations from HIP events recorded on the launch stream around
-    # each device stage, averaged over `steps` further steps.
-    acc = None
-    for _ in range(steps):
-        st = launch.profiled()
-        acc = [(a[0] + b[0], b[1]) for a, b in zip(acc, st)] if acc else list(st)
-    stages = [{"stage": launches[k] if k < len(launches) else str(k), "ms": ms / steps, "bytes_per_frame": by,
-               "gbs": by * frames / (ms / steps * 1e-3) / 1e9 if ms > 0 else 0.0} for k, (ms, by) in enumerate(acc)]
-    return {"fps": fps, "ms_per_step": per_step * 1e3, "achieved_gbs": achieved, "algo_bytes_per_frame": algo_bytes,
-            "launches": len(launches), "dtype": dtype, "graph": launches, "stages": stages}
-
-
-def traffic_of(kernel_key):
-    """HBM bytes per launch of a kernel, from a committed rocprofv3 --pmc summary
-    (profiles/<key>_traffic.txt, a line 'traffic_bytes_per_launch: N'), if present."""
-    path = os.path.join(ROOT, "profiles", f"{kernel_key}_traffic.txt")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
+        return lambda: P.word_resize(src, 3840, 2160)
+    if name in ("fp32_down", "fp32_up"):
+        import zimg_amd as z
+        sw, sh, dw, dh, ft, a, b = (3840, 2160, 1920, 1080, z.RESIZE_LANCZOS, 3.0, 0.0) if name == "fp32_down" else \
+            (1280, 720, 3840, 2160, z.RESIZE_BICUBIC, 0.0, 0.5)
+        src = W.fp32_plane(sw, sh, seed=99)
+        return lambda: P.float_resize(src, dw, dh, ft, a, b)
+    if name == "uhd_pq_to_hlg":
+        R, G, B = W.pq_rgb(3840, 2160, seed=99)
+        return lambda: P.rgb_pq_to_hlg_12bit_ed(R, G, B)
     return None
 
 
-def roofline(m, workload, frames, precision="exact"):
-    """Dominant device stage: algorithmic bytes per launch / its average launch
-    duration (HIP events on the launch stream); whole-graph figure alongside."""
-    dom = max(m["stages"], key=lambda st: st["ms"])
-    achieved = dom["gbs"]
-    traffic = traffic_of(f"{workload}_dominant" if precision == "exact" else f"{workload}_{precision}_dominant")
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-            "traffic": traffic, "kernel": dom["stage"], "kernel_ms_per_launch": round(dom["ms"], 4),
-            "algorithmic_bytes_per_launch": dom["bytes_per_frame"] * frames,
-            "whole_graph": {"achieved": round(m["achieved_gbs"], 1), "frac": round(m["achieved_gbs"] / PEAK_HBM_GBS, 4)},
-            "stages": [{"ms": round(st["ms"], 4), "gbs": round(st["gbs"], 1), "stage": st["stage"][:60]} for st in m["stages"]]}
+def cpu_baseline(name, runs, full):
+    """Oracle on this host: median seconds per frame of `runs` single-thread runs;
+    with `full`, also frame-parallel runs over the usable cores (median of `runs`)."""
+    fn = cpu_frame_fn(name)
+    if fn is None:
+        return None
+    fn()  # warm-up
+    n1 = runs if full else max(3, runs // 3)
+    single = []
+    for _ in range(n1):
+        t0 = time.perf_counter()
+        fn()
+        single.append(time.perf_counter() - t0)
+    med1 = statistics.median(single)
+    out = {"value": round(1.0 / med1, 2), "unit": "frames/s", "cores": 1, "kind": "port",
+           "sample": f"{name}: median of {n1} single-thread whole frames through the oracle (tests/pipelines.py)",
+           "cpu": cpu_model()}
+    if full:
+        from concurrent.futures import ThreadPoolExecutor
+        n = usable_cores()
+        par = []
+        with ThreadPoolExecutor(max_workers=n) as ex:
+            list(ex.map(lambda _: fn(), range(n)))
+            for _ in range(runs):
+                t0 = time.perf_counter()
+                list(ex.map(lambda _: fn(), range(n)))
+                par.append(time.perf_counter() - t0)
+        medn = statistics.median(par)
+        out["frame_parallel"] = {"value": round(n / medn, 2), "cores": n,
+                                 "sample": f"median of {runs} runs of {n} frames, one frame per thread"}
+    return out
 
 
 def main():
@@ -269,48 +333,38 @@ def main():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
-    primary = args.workload
-    if primary == "batch64":
-        b = measure_batch(torch, args.steps, args.warmup)
-        print(json.dumps({
-            "metric": "frames/s + HBM GB/s, UHD Lanczos & 709→2020; MFMA/learned vs plain at ≤0.5 LSB",
-            "value": round(b["fps"], 1), "unit": "jobs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": round(b["ms_per_step"], 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "u8/u16 in, WORD Q1.14 taps, u8 out", "data": "synthetic (seeded job list and planes, zimg_amd/workloads.py)",
-            "config": {"workload": "batch64", "jobs_per_step": 64, "form": "persistent autotuned launch replayed from a HIP graph"},
-            "roofline": {"bound": "hbm", "achieved": round(b["achieved_gbs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                         "frac": round(b["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": None, "kernel": b["describe"]},
-            "cpu_baseline": None,
-            "per_job_launches": {"fps": round(b["per_job_fps"], 1), "achieved_gbs": round(b["per_job_gbs"], 1),
-                                 "launches_per_step": b["per_job_launches"]},
-            "speedup_vs_per_job": round(b["fps"] / b["per_job_fps"], 3)}))
-        return
-    if args.frames is None:
-        args.frames = 85 if primary == "uhd_pq_to_hlg" else 64
-    secondary = args.secondary if args.secondary and args.secondary != primary else None
-    m = measure(torch, primary, args.frames, args.steps, args.warmup, args.precision)
-    m2 = measure(torch, secondary, min(args.frames, 32), max(args.steps // 2, 5), args.warmup, args.precision) if secondary else None
-    cpu = None if args.no_cpu or primary not in ("fhd_709_to_2020", "uhd_lanczos") else cpu_baseline(primary, args.cpu_seconds)
-
+    frames = args.frames or 64
+    todo = [args.workload] if args.workload else WORKLOADS
+    results = {}
+    for name in todo:
+        if name == "batch64":
+            results[name] = measure_batch(torch, args.steps, args.warmup)
+        else:
+            results[name] = measure(torch, name, frames, args.steps, args.warmup)
+            if name in ("fp32_down", "fp32_up"):
+                results[name]["mfma_fp16"] = measure(torch, name, frames, args.steps, args.warmup, precision=1)
+        if not args.no_cpu:
+            results[name]["cpu_baseline"] = cpu_baseline(name, args.cpu_runs, full=name in ("fhd_709_to_2020", "uhd_lanczos"))
+        print(f"# {name}: {results[name]['fps']} {results[name].get('unit', 'frames/s')}", file=sys.stderr, flush=True)
+    primary = todo[0]
+    m = results[primary]
     line = {
-        "metric": "frames/s + HBM GB/s, UHD Lanczos & 709→2020; MFMA/learned vs plain at ≤0.5 LSB",
-        "value": round(m["fps"], 1),
-        "unit": "frames/s",
+        "metric": METRIC,
+        "value": m["fps"],
+        "unit": m.get("unit", "frames/s"),
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(m["ms_per_step"], 4),
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": m["dtype"],
         "data": "synthetic (seeded, zimg_amd/workloads.py), inputs resident in HBM",
-        "config": {"workload": primary, "frames_per_step": args.frames, "graph_launches": m["launches"], "precision": args.precision},
-        "roofline": roofline(m, primary, args.frames, args.precision),
-        "cpu_baseline": cpu,
-        "secondary": None if not m2 else {
-            "workload": secondary, "fps": round(m2["fps"], 1), "achieved_gbs": round(m2["achieved_gbs"], 1),
-            "frac": round(m2["achieved_gbs"] / PEAK_HBM_GBS, 4), "launches": m2["launches"]},
+        "config": {"workload": primary, "frames_per_step": m["frames"], "graph_launches": m["launches"], "precision": "exact"},
+        "roofline": m["roofline"],
+        "cpu_baseline": m.get("cpu_baseline"),
+        "configs": {k: v for k, v in results.items() if k != primary},
     }
     print(json.dumps(line))
 

@@ -63,3 +63,78 @@ def rgb_pq_to_hlg_12bit_ed(R, G, B, peak=203.0):
     Rc, Gc, Bc = colorspace_apply(ops, R, G, B)
     s, o = scale_offset((FLOAT, 32, 0, 0, 0), (WORD, 12, 1, 0, 0))
     return [dither(p, FLOAT, WORD, s, o, 12, dither_type=3) for p in (Rc, Gc, Bc)]
+
+
+# ---- plane geometry of the reference's graph builder ---------------------------
+# internal_state (graphbuilder.cpp:368-392): every plane carries its size and an
+# active window in its own sample units; chroma planes are scaled from luma
+# (chroma_from_luma, :331-346) and shifted by chroma siting and field parity
+# (apply_pixel_siting, :348-366, with the offsets of :64-171). A resize between
+# two planes maps the destination window onto the source (resize_plane,
+# :769-776).
+
+CHROMA_LOC = {0: ("left", "center"), 1: ("center", "center"), 2: ("left", "top"), 3: ("center", "top"),
+              4: ("left", "bottom"), 5: ("center", "bottom")}
+
+
+def _chroma_offset_w(loc, sub):
+    return -0.5 + 0.5 * sub if loc == "left" else 0.0
+
+
+def _chroma_offset_h(loc, sub):
+    if loc == "top":
+        return -0.5 + 0.5 * sub
+    if loc == "bottom":
+        return 0.5 - 0.5 * sub
+    return 0.0
+
+
+def _parity_adjust(parity, off):
+    return -0.25 + off / 2 if parity == 1 else 0.25 + off / 2 if parity == 2 else off
+
+
+def _luma_parity(parity):
+    return -0.25 if parity == 1 else 0.25 if parity == 2 else 0.0
+
+
+def plane_geometry(width, height, grey=False, ssw=0, ssh=0, parity=0, chroma_location=0, active=None):
+    """[(width, height, left, top, active_width, active_height)] for Y, then U and V."""
+    al, at, aw, ah = active if active is not None else (0.0, 0.0, float(width), float(height))
+    planes = [[width, height, al, at, aw, ah]]
+    if not grey:
+        sw, sh = 1.0 / (1 << ssw), 1.0 / (1 << ssh)
+        u = [width >> ssw, height >> ssh, al * sw, at * sh, aw * sw, ah * sh]
+        planes += [u, list(u)]
+    planes[0][3] -= _luma_parity(parity)
+    if not grey:
+        loc_w, loc_h = CHROMA_LOC[chroma_location]
+        sub_w = planes[1][0] / planes[0][0]
+        sub_h = planes[1][1] / planes[0][1]
+        ow = _chroma_offset_w(loc_w, sub_w)
+        oh = _parity_adjust(parity, _chroma_offset_h(loc_h, sub_h))
+        for q in (1, 2):
+            planes[q][2] -= ow
+            planes[q][3] -= oh
+    return [tuple(p) for p in planes]
+
+
+def resize_params(src_plane, dst_plane):
+    """(shift_w, shift_h, subwidth, subheight) of a resize between two planes, or
+    None when nothing about the plane changes (graphbuilder.cpp:769-776)."""
+    if tuple(src_plane) == tuple(dst_plane):
+        return None
+    _, _, sl, st, saw, sah = src_plane
+    dw, dh, dl, dt, daw, dah = dst_plane
+    scale_w = daw / saw
+    scale_h = dah / sah
+    return (sl - dl / scale_w, st - dt / scale_h, saw * (dw / daw), sah * (dh / dah))
+
+
+def resize_geom(img, src_plane, dst_plane, ftype, a, b, depth=16):
+    """One plane through resize_plane with the builder's geometry."""
+    rp = resize_params(src_plane, dst_plane)
+    if rp is None:
+        return img.copy()
+    sw_, sh_, subw, subh = rp
+    return resize_plane(img, ftype, a, b, dst_plane[0], dst_plane[1], shift_w=sw_, shift_h=sh_, subwidth=subw, subheight=subh,
+                        depth=depth)

@@ -183,9 +183,8 @@ def test_device_tmp_size_scales_with_frames():
     one, four = g.device_tmp_size(1), g.device_tmp_size(4)
     assert one > 0 and four == 4 * one
     # fused lowering: one launch per chroma resize pair, one colourspace launch
-    dev = g.describe().split("device:\n")[1]
-    lines = [l for l in dev.splitlines() if l]
-    assert sum(l.startswith("colorspace") for l in lines) == 1 and "quantize[0]" in dev
+    lines = g.device_stages()
+    assert sum(l.startswith("colorspace") for l in lines) == 1 and "quantize[0]" in g.describe()
     assert len(lines) == 5
 
 
@@ -193,11 +192,10 @@ def test_single_plane_lanczos_is_one_fused_launch():
     src = z.image_format(width=1920, height=1080, pixel_type=z.PIXEL_WORD, depth=16)
     dst = z.image_format(width=3840, height=2160, pixel_type=z.PIXEL_WORD, depth=16)
     g = z.FilterGraph(src, dst, z.builder_params(resample_filter=z.RESIZE_LANCZOS, filter_param_a=3))
-    dev = [l for l in g.describe().split("device:\n")[1].splitlines() if l]
-    launches = [l for l in dev if not l.startswith(" ")]
+    launches = g.device_stages()
     assert len(launches) == 1 and "passes=2 h_first" in launches[0]
     # both passes on the int8 matrix cores in one launch, no HBM intermediate
-    assert any(l.startswith("  integer matrix cores: one launch") for l in dev)
+    assert "\n  integer matrix cores: one launch" in g.describe()
     assert g.device_tmp_size(8) == 0
 
 
@@ -288,6 +286,51 @@ def test_batch_job_list_fixture_and_jobs_tmp_size():
     assert jobs == fx["jobs"]
     graphs = [z.FilterGraph(*W.batch_job_graph(j)) for j in jobs]
     for g in graphs:
-        dev = g.describe().split("device:\n")[1].splitlines()
+        dev = g.device_stages()
         assert len(dev) == 3 and all(l.startswith("resize") and "tiled=" in l and "tiled=0" not in l for l in dev), dev
     assert z.device_jobs_tmp_size(graphs) == 0
+
+
+def _geom_of(spec):
+    """plane_geometry of a trace-fixture format (64x48 base)."""
+    import pipelines as P
+    g = lambda k, d=0: spec.get(k, d)
+    grey = spec["base"] == "yuv" and g("color_family", z.COLOR_YUV) == z.COLOR_GREY
+    act = None
+    if "active_left" in spec:
+        act = (float(g("active_left")), float(g("active_top")), float(g("active_width")), float(g("active_height")))
+    return P.plane_geometry(g("width", 64), g("height", 48), grey=grey, ssw=g("subsample_w"), ssh=g("subsample_h"),
+                            parity=g("field_parity"), chroma_location=g("chroma_location", z.CHROMA_CENTER), active=act)
+
+
+RESIZE_TRACE = re.compile(r"resize\[(\d)\]: \[(\d+), (\d+)\] => \[(\d+), (\d+)\] \(([-\d.]+), ([-\d.]+), ([-\d.]+), ([-\d.]+)\)")
+
+
+@pytest.mark.parametrize("name", ["resize_chromaloc_upsample", "resize_chromaloc_downsample", "resize_deinterlace_top",
+                                  "resize_deinterlace_bottom", "resize_interlace_to_interlace", "upscale_colorspace_tile",
+                                  "downscale_colorspace_tile"])
+def test_geometry_oracle_matches_reference_traces(name):
+    """The tests' own plane geometry (tests/pipelines.py, restating
+    graphbuilder.cpp:64-171,331-366,769-776) gives the shift and subwindow
+    every full resize trace line of the reference's graph tests records.
+    Colourspace cases resize into a 4:4:4 state the size of the smaller image
+    (graphbuilder.cpp:963-983): its planes are the luma plane's geometry."""
+    import pipelines as P
+    case = next(c for c in CASES["cases"] if c["name"] == name)
+    src, dst = _geom_of(case["src"]), _geom_of(case["dst"])
+    csp = "colorspace" in case["trace"]
+    lines = [RESIZE_TRACE.match(t) for t in case["trace"] if t.startswith("resize[")]
+    assert lines and all(lines)
+    for m in lines:
+        q = int(m.group(1))
+        if csp:
+            # 4:4:4 intermediate at the smaller size, active window of that image's luma
+            small = min(src, dst, key=lambda gg: gg[0][0])
+            inter = small[0]
+            pair = (src[q], inter) if (int(m.group(2)), int(m.group(3))) == src[q][:2] else (inter, dst[q])
+        else:
+            pair = (src[q], dst[q])
+        assert pair[0][:2] == (int(m.group(2)), int(m.group(3))) and pair[1][:2] == (int(m.group(4)), int(m.group(5)))
+        want = tuple(float(m.group(k)) for k in range(6, 10))
+        got = P.resize_params(*pair)
+        assert got is not None and np.allclose(got, want, atol=5e-7), (name, q, got, want)

@@ -119,7 +119,7 @@ def test_half_resize_to_half():
     src = (rng.random((sh, sw), dtype=np.float32) * 2 - 0.5).astype(np.float16).view(np.uint16)
     g = z.FilterGraph(_grey(sw, sh, z.PIXEL_HALF, 16), _grey(dw, dh, z.PIXEL_HALF, 16),
                       z.builder_params(resample_filter=z.RESIZE_BICUBIC, filter_param_a=0.0, filter_param_b=0.5))
-    launches = [l for l in g.describe().split("device:\n")[1].splitlines() if l]
+    launches = g.device_stages()
     assert len(launches) == 1, launches
     (out, *_) = gpu_util.run_graph(g, [src, None, None, None], [(dh, dw, np.uint16), None, None, None], pad_bytes=6)
     want = to_half(resize_plane(half_to_float(src), BICUBIC, 0.0, 0.5, dw, dh), FLOAT)
@@ -140,7 +140,7 @@ def test_half_yuv_to_half_rgb():
                color_primaries=z.PRIMARIES_BT709)
     rgb = dict(yuv, color_family=z.COLOR_RGB, matrix_coefficients=z.MATRIX_RGB)
     g = z.FilterGraph(_grey(w, h, z.PIXEL_HALF, 16, **yuv), _grey(w, h, z.PIXEL_HALF, 16, **rgb), z.builder_params())
-    launches = [l for l in g.describe().split("device:\n")[1].splitlines() if l]
+    launches = g.device_stages()
     assert len(launches) == 1, launches
     outs = gpu_util.run_graph(g, planes + [None], [(h, w, np.uint16)] * 3 + [None])
     n, ops, _ = colorspace_ops((M_709, T_709, P_709), (M_RGB, T_709, P_709))

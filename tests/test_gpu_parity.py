@@ -314,3 +314,39 @@ def test_failing_callback_reports_error():
     with pytest.raises(z.ZimgError) as ei:
         g.process(src, dst, None, bad, None)
     assert ei.value.code == z.ERROR_USER_CALLBACK_FAILED
+
+
+def test_host_process_concurrent_callers():
+    """zimg_filter_graph_process is const and reentrant (zimg.h:495-499; the
+    frame-parallel pattern of graphapp.cpp:355-377): 4 host threads x 3 calls
+    on one graph, each with its own buffers and no tmp, give the device
+    path's outputs bit for bit. Calls after the first round allocate nothing:
+    the graph's staging-set count stays where the first round left it."""
+    from concurrent.futures import ThreadPoolExecutor
+    w, h = 320, 180
+    print(f"case: host process, 4 threads x 3 calls, 4:2:0 709->2020 {w}x{h}", flush=True)
+    s, d, p = W.fhd_709_to_2020_formats(w, h)
+    g = z.FilterGraph(s, d, p)
+    frames = [W.yuv420_10bit(w, h, seed=40 + k) for k in range(12)]
+    specs = [(h, w, np.uint16), (h // 2, w // 2, np.uint16), (h // 2, w // 2, np.uint16), None]
+    want = [gpu_util.run_graph(g, list(f) + [None], specs)[:3] for f in frames]
+
+    def one(k):
+        Y, U, V = frames[k]
+        outs = [np.zeros((h, w), np.uint16), np.zeros((h // 2, w // 2), np.uint16), np.zeros((h // 2, w // 2), np.uint16)]
+        src = z.image_buffer([(a.ctypes.data, a.strides[0]) for a in (Y, U, V)] + [None])
+        dst = z.image_buffer([(a.ctypes.data, a.strides[0]) for a in outs] + [None])
+        g.process(src, dst, None)
+        return outs
+
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        got = list(ex.map(one, range(12)))
+    sets = g.host_staging_sets()
+    assert 1 <= sets <= 4, sets
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        got2 = list(ex.map(one, range(12)))
+    for k in range(12):
+        for q in range(3):
+            _assert_equal(got[k][q], want[k][q], f"frame {k} plane {q}")
+            _assert_equal(got2[k][q], want[k][q], f"frame {k} plane {q} (second round)")
+    assert g.host_staging_sets() <= 4

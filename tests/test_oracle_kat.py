@@ -13,8 +13,8 @@ import numpy as np
 import pytest
 
 from oracle_lib import (BICUBIC, BILINEAR, BYTE, FLOAT, HALF, LANCZOS, POINT, SPLINE16, SPLINE36, SPLINE64, WORD, blue_noise,
-                        c_u8_p, dither, half_to_float, oracle, oracle_filter, ptr, resize_h, resize_v, scale_offset, to_float,
-                        to_half)
+                        c_u8_p, dither, half_to_float, left_shift, oracle, oracle_filter, ptr, resize_h, resize_h_first, resize_v,
+                        scale_offset, to_float, to_half)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -132,6 +132,76 @@ def test_float_resize_kernel_sequential_order():
                 acc = np.float32(acc + np.float32(data[j, k] * img[r, left[j] + k]))
             want[r, j] = acc
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_float_vertical_kernel_sequential_order():
+    """The vertical FP32 pass also accumulates from 0 in tap order, one column
+    at a time (resize_impl.cpp:97-114). Inputs are chosen so that any other
+    association order (pairwise, reversed, a start at the first product, FMA)
+    rounds differently: large and tiny magnitudes alternate down each column."""
+    rng = np.random.default_rng(12)
+    h, w = 41, 7
+    img = (rng.random((h, w), dtype=np.float32) * np.where(np.arange(h)[:, None] % 2 == 0, 1e4, 1e-3)).astype(np.float32)
+    fw, data, _, left = filt = oracle_filter(LANCZOS, 4, 0, h, 19, 0.3, float(h))
+    got = resize_v(img, filt)
+    want = np.zeros((19, w), np.float32)
+    for i in range(19):
+        for j in range(w):
+            acc = np.float32(0)
+            for k in range(fw):
+                acc = np.float32(acc + np.float32(data[i, k] * img[left[i] + k, j]))
+            want[i, j] = acc
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    # The orders a plausible slip would produce differ from it somewhere.
+    rev = np.zeros_like(want)
+    for i in range(19):
+        for j in range(w):
+            acc = np.float32(0)
+            for k in reversed(range(fw)):
+                acc = np.float32(acc + np.float32(data[i, k] * img[left[i] + k, j]))
+            rev[i, j] = acc
+    assert not np.array_equal(rev.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("type_in,type_out,depth_in,shift", [(BYTE, WORD, 8, 8), (BYTE, WORD, 8, 2), (WORD, WORD, 10, 6),
+                                                            (WORD, WORD, 10, 2), (BYTE, BYTE, 6, 2)])
+def test_left_shift_every_code(type_in, type_out, depth_in, shift):
+    """IntegerLeftShift (depth_convert.cpp:23-30, 104-144): every code of the
+    input depth becomes x << shift in the output storage type, on a strided
+    plane whose padding must stay untouched."""
+    codes = np.arange(1 << depth_in, dtype=np.uint8 if type_in == BYTE else np.uint16)
+    img = np.resize(codes, (max(1, codes.size // 64), 64)).astype(codes.dtype)
+    got = left_shift(img, type_in, type_out, shift)
+    want = (img.astype(np.uint32) << shift).astype(np.uint8 if type_out == BYTE else np.uint16)
+    assert np.array_equal(got, want)
+    assert set(np.unique(got).tolist()) == set((codes.astype(np.uint32) << shift).tolist())
+
+
+def test_pass_order_choice():
+    """resize_h_first (resize.cpp:12-18): the pass with the cheaper total tap
+    work goes first, a horizontal tap weighted twice a vertical one. Known
+    answers from the reference's own cases (2x/2x and 3x upscales: H first;
+    1/2 x 1/2 downscale: V first), then a brute-force count over real filter
+    widths and output sizes for random scale pairs with a clear margin."""
+    assert resize_h_first(2.0, 2.0)
+    assert resize_h_first(3.0, 3.0)
+    assert not resize_h_first(0.5, 0.5)
+    rng = np.random.default_rng(21)
+    checked = 0
+    for _ in range(400):
+        sw, sh = int(rng.integers(40, 400)), int(rng.integers(40, 400))
+        dw, dh = int(rng.integers(20, 900)), int(rng.integers(20, 900))
+        fw_h = oracle_filter(LANCZOS, 3, 0, sw, dw)[0]
+        fw_v = oracle_filter(LANCZOS, 3, 0, sh, dh)[0]
+        # H first: the H pass makes sh x dw outputs of fw_h taps, then V makes dh x dw of fw_v.
+        cost_h = 2 * sh * dw * fw_h + dh * dw * fw_v
+        # V first: the V pass makes dh x sw outputs, then H makes dh x dw.
+        cost_v = dh * sw * fw_v + 2 * dh * dw * fw_h
+        if abs(cost_h - cost_v) < 0.2 * min(cost_h, cost_v):
+            continue
+        checked += 1
+        assert resize_h_first(dw / sw, dh / sh) == (cost_h < cost_v), (sw, sh, dw, dh)
+    assert checked > 200
 
 
 def test_half_conversions_ieee():

@@ -322,6 +322,19 @@ class FilterGraph:
         lib().zimg_filter_graph_describe(self._h, buf, n + 1)
         return buf.value.decode()
 
+    def device_stages(self):
+        """One line per device launch of the lowered graph (indented detail
+        lines and the host staging summary left out)."""
+        dev = self.describe().split("device:\n", 1)[1]
+        return [l for l in dev.splitlines() if l and not l[0].isspace() and not l.startswith("host staging sets:")]
+
+    def host_staging_sets(self):
+        """Staging sets the host-buffer path (process) has created so far."""
+        for line in self.describe().splitlines():
+            if line.startswith("host staging sets:"):
+                return int(line.split(":")[1])
+        return 0
+
     def plan_trace(self):
         text = self.describe()
         plan = text.split("device:\n")[0].split("plan:\n", 1)[1]

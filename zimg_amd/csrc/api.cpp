@@ -21,28 +21,52 @@
 #include "graph.h"
 #include "zimg.h"
 
+namespace {
+
+// Staging of one host-buffer call (zimg_filter_graph_process): pinned host
+// rows of every plane, their device copies plus the program's device scratch,
+// and the stream the call runs on. Created on first need, then reused.
+struct HostStaging {
+	int device = -1;
+	char *host = nullptr;
+	size_t host_size = 0;
+	void *dev = nullptr;
+	size_t dev_size = 0;
+	hipStream_t stream = nullptr;
+
+	HostStaging(const HostStaging &) = delete;
+	HostStaging &operator=(const HostStaging &) = delete;
+	HostStaging() = default;
+	~HostStaging()
+	{
+		int cur = 0;
+		if (hipGetDevice(&cur) != hipSuccess)
+			return;
+		(void)hipSetDevice(device);
+		if (stream)
+			(void)hipStreamDestroy(stream);
+		if (dev)
+			(void)hipFree(dev);
+		if (host)
+			(void)hipHostFree(host);
+		(void)hipSetDevice(cur);
+	}
+};
+
+} // namespace
+
 struct zimg_filter_graph {
 	std::unique_ptr<zamd::DeviceProgram> prog;
 	unsigned src_subsample_h = 0, dst_subsample_h = 0;
 	unsigned src_bpp = 1, dst_bpp = 1;
 
-	// Host-path staging in HBM (zimg_filter_graph_process), allocated on first use.
-	mutable std::mutex host_mutex;
-	mutable void *dev_mem = nullptr;
-	mutable size_t dev_mem_size = 0;
-	mutable int dev_mem_device = -1;
-
-	~zimg_filter_graph()
-	{
-		if (dev_mem) {
-			int cur = 0;
-			if (hipGetDevice(&cur) == hipSuccess) {
-				(void)hipSetDevice(dev_mem_device);
-				(void)hipFree(dev_mem);
-				(void)hipSetDevice(cur);
-			}
-		}
-	}
+	// Staging sets of zimg_filter_graph_process. A call takes a free set of its
+	// device (or makes one), so concurrent calls run on their own streams and
+	// a steady sequence of calls allocates nothing; the lock covers only the
+	// hand-over.
+	mutable std::mutex pool_mutex;
+	mutable std::vector<std::unique_ptr<HostStaging>> pool;
+	mutable unsigned pool_sets = 0;
 };
 
 namespace {
@@ -420,13 +444,9 @@ void zimg_filter_graph_free(zimg_filter_graph *ptr) { delete ptr; }
 zimg_error_code_e zimg_filter_graph_get_tmp_size(const zimg_filter_graph *ptr, size_t *out)
 {
 	try {
-		const Plan &plan = program(ptr).plan();
-		size_t total = 0;
-		for (int q = 0; q < 4; ++q) {
-			total += static_cast<size_t>(plan.src_w[q]) * plan.src_h[q] * ptr->src_bpp;
-			total += static_cast<size_t>(plan.dst_w[q]) * plan.dst_h[q] * ptr->dst_bpp;
-		}
-		*out = total;
+		program(ptr);
+		// The host path stages planes through the graph's own pinned buffers.
+		*out = 0;
 		return ZIMG_ERROR_SUCCESS;
 	} catch (...) {
 		return handle_current_exception();
@@ -459,6 +479,7 @@ zimg_error_code_e zimg_filter_graph_process(const zimg_filter_graph *ptr, const 
                                             zimg_filter_graph_callback unpack_cb, void *unpack_user,
                                             zimg_filter_graph_callback pack_cb, void *pack_user)
 {
+	(void)tmp; // zimg_filter_graph_get_tmp_size is 0: staging belongs to the graph
 	try {
 		const DeviceProgram &prog = program(ptr);
 		const Plan &plan = prog.plan();
@@ -467,15 +488,6 @@ zimg_error_code_e zimg_filter_graph_process(const zimg_filter_graph *ptr, const 
 		if (src->version < API_2_0 || dst->version < API_2_0)
 			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "API version invalid");
 
-		// Host staging: the caller's tmp holds every source and destination plane packed.
-		size_t tmp_need = 0;
-		zimg_filter_graph_get_tmp_size(ptr, &tmp_need);
-		std::vector<char> own;
-		char *host = static_cast<char *>(tmp);
-		if (!host) {
-			own.resize(tmp_need);
-			host = own.data();
-		}
 		size_t src_off[4] = {}, dst_off[4] = {};
 		size_t o = 0;
 		for (int q = 0; q < 4; ++q) {
@@ -488,6 +500,54 @@ zimg_error_code_e zimg_filter_graph_process(const zimg_filter_graph *ptr, const 
 		}
 		const size_t src_bytes = dst_off[0];
 		const size_t dst_bytes = o - src_bytes;
+		const size_t dtmp_off = (src_bytes + dst_bytes + 255) / 256 * 256;
+		const size_t dev_need = dtmp_off + prog.tmp_size(1);
+
+		int device = 0;
+		if (hipGetDevice(&device) != hipSuccess)
+			fail(ZIMG_ERROR_UNKNOWN, "no HIP device");
+
+		// Take a staging set of this device, or make one (first use, or more
+		// concurrent callers than before).
+		std::unique_ptr<HostStaging> st;
+		{
+			std::lock_guard<std::mutex> lock(ptr->pool_mutex);
+			for (auto it = ptr->pool.begin(); it != ptr->pool.end(); ++it) {
+				if ((*it)->device == device && (*it)->host_size >= o && (*it)->dev_size >= dev_need) {
+					st = std::move(*it);
+					ptr->pool.erase(it);
+					break;
+				}
+			}
+		}
+		if (!st) {
+			st = std::make_unique<HostStaging>();
+			st->device = device;
+			if (hipHostMalloc(reinterpret_cast<void **>(&st->host), std::max<size_t>(o, 1), hipHostMallocDefault) != hipSuccess)
+				fail(ZIMG_ERROR_OUT_OF_MEMORY, "pinned staging allocation failed");
+			st->host_size = o;
+			if (hipMalloc(&st->dev, std::max<size_t>(dev_need, 1)) != hipSuccess)
+				fail(ZIMG_ERROR_OUT_OF_MEMORY, "device staging allocation failed");
+			st->dev_size = dev_need;
+			if (hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking) != hipSuccess)
+				fail(ZIMG_ERROR_UNKNOWN, "hipStreamCreate failed");
+			std::lock_guard<std::mutex> lock(ptr->pool_mutex);
+			++ptr->pool_sets;
+		}
+		// The set goes back to the pool however the call ends.
+		struct Return {
+			const zimg_filter_graph *g;
+			std::unique_ptr<HostStaging> &st;
+			~Return()
+			{
+				if (!st)
+					return;
+				std::lock_guard<std::mutex> lock(g->pool_mutex);
+				g->pool.push_back(std::move(st));
+			}
+		} give_back{ ptr, st };
+		char *host = st->host;
+		char *dmem = static_cast<char *>(st->dev);
 
 		// Unpack: rows in groups of the vertical subsampling unit (zimg.h:455-482).
 		const unsigned ssh = ptr->src_subsample_h;
@@ -506,26 +566,8 @@ zimg_error_code_e zimg_filter_graph_process(const zimg_filter_graph *ptr, const 
 			}
 		}
 
-		std::lock_guard<std::mutex> lock(ptr->host_mutex);
-		int device = 0;
-		if (hipGetDevice(&device) != hipSuccess)
-			fail(ZIMG_ERROR_UNKNOWN, "no HIP device");
-		const size_t dtmp = prog.tmp_size(1);
-		const size_t need = src_bytes + dst_bytes + dtmp + 1024;
-		if (!ptr->dev_mem || ptr->dev_mem_size < need || ptr->dev_mem_device != device) {
-			if (ptr->dev_mem)
-				(void)hipFree(ptr->dev_mem);
-			ptr->dev_mem = nullptr;
-			if (hipMalloc(&ptr->dev_mem, need) != hipSuccess)
-				fail(ZIMG_ERROR_OUT_OF_MEMORY, "device staging allocation failed");
-			ptr->dev_mem_size = need;
-			ptr->dev_mem_device = device;
-		}
-		char *dmem = static_cast<char *>(ptr->dev_mem);
-		const size_t dtmp_off = (src_bytes + dst_bytes + 255) / 256 * 256;
-		if (hipMemcpy(dmem, host, src_bytes, hipMemcpyHostToDevice) != hipSuccess)
+		if (hipMemcpyAsync(dmem, host, src_bytes, hipMemcpyHostToDevice, st->stream) != hipSuccess)
 			fail(ZIMG_ERROR_UNKNOWN, "upload failed");
-
 		DeviceBuffers ds, dd;
 		for (int q = 0; q < 4; ++q) {
 			if (plan.src_w[q]) {
@@ -537,11 +579,11 @@ zimg_error_code_e zimg_filter_graph_process(const zimg_filter_graph *ptr, const 
 				dd.stride[q] = static_cast<long long>(plan.dst_w[q]) * ptr->dst_bpp;
 			}
 		}
-		prog.run(ds, dd, dmem + dtmp_off, 1, nullptr);
-		if (hipDeviceSynchronize() != hipSuccess)
-			fail(ZIMG_ERROR_UNKNOWN, "device execution failed");
-		if (hipMemcpy(host + src_bytes, dmem + src_bytes, dst_bytes, hipMemcpyDeviceToHost) != hipSuccess)
+		prog.run(ds, dd, dmem + dtmp_off, 1, st->stream);
+		if (hipMemcpyAsync(host + src_bytes, dmem + src_bytes, dst_bytes, hipMemcpyDeviceToHost, st->stream) != hipSuccess)
 			fail(ZIMG_ERROR_UNKNOWN, "download failed");
+		if (hipStreamSynchronize(st->stream) != hipSuccess)
+			fail(ZIMG_ERROR_UNKNOWN, "device execution failed");
 
 		const unsigned dsh = ptr->dst_subsample_h;
 		for (unsigned i = 0; i < plan.dst_h[0]; i += 1U << dsh) {
@@ -628,6 +670,13 @@ zimg_filter_graph *zimg_filter_graph_build(const zimg_image_format *src_format, 
 		Plan plan = build_plan(s, d, bp);
 		auto g = std::make_unique<zimg_filter_graph>();
 		g->prog = std::make_unique<DeviceProgram>(plan);
+		// Coefficient tables go to the current device now, so that processing
+		// on it allocates nothing (no device: they upload on first use).
+		int ndev = 0;
+		if (hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0)
+			g->prog->prepare_current_device();
+		else
+			(void)hipGetLastError();
 		g->src_subsample_h = s.color == ColorFamily::GREY ? 0 : s.subsample_h;
 		g->dst_subsample_h = d.color == ColorFamily::GREY ? 0 : d.subsample_h;
 		g->src_bpp = pixel_size(s.type);
@@ -718,6 +767,11 @@ zimg_error_code_e zimg_filter_graph_set_resize_precision(zimg_filter_graph *ptr,
 		if (!ptr || !ptr->prog)
 			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null graph");
 		ptr->prog->set_resize_precision(mode);
+		int ndev = 0;
+		if (hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0)
+			ptr->prog->prepare_current_device();
+		else
+			(void)hipGetLastError();
 		return ZIMG_ERROR_SUCCESS;
 	} catch (...) {
 		return handle_current_exception();
@@ -897,6 +951,12 @@ size_t zimg_filter_graph_describe(const zimg_filter_graph *ptr, char *buf, size_
 	try {
 		const DeviceProgram &prog = program(ptr);
 		std::string s = "plan:\n" + prog.plan().describe() + "device:\n" + prog.describe();
+		unsigned sets = 0;
+		{
+			std::lock_guard<std::mutex> lock(ptr->pool_mutex);
+			sets = ptr->pool_sets;
+		}
+		s += "host staging sets: " + std::to_string(sets) + "\n";
 		if (buf && n) {
 			std::strncpy(buf, s.c_str(), n);
 			buf[n - 1] = '\0';

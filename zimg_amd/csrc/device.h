@@ -6,6 +6,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -108,6 +109,15 @@ public:
 		std::map<int, dev::PassArg> pass; // by node id
 	};
 
+	// Uploads every table the program's launches read on the current HIP
+	// device (resize passes, integer and FP16/BF16 matrix-core blocks of the
+	// current precision mode). Idempotent; run() does it on first use, so
+	// callers that must not allocate later (graph capture, the allocation-free
+	// entry point) call it up front.
+	void prepare_current_device() const;
+	// True when every table run() needs is already resident on the current device.
+	bool resident_on_current_device() const;
+
 	// Steps and per-device tables, for launches batched across programs.
 	const std::vector<Step> &steps() const { return m_steps; }
 	const Tables &tables_for_current_device() const;
@@ -138,6 +148,7 @@ private:
 	const ImmaTables &imma_tables_for_current_device() const;
 	ImmaPlan imma_plan(const Step &s) const;
 	int m_precision = 0;
+	mutable std::set<int> m_prepared; // HIP device * 4 + precision mode
 
 	bool mfma_step(const Step &s) const { return m_precision && s.kind == Step::RESIZE && !s.word && s.ox[0].dither == 0; }
 	// Launch shape of the one-launch two-pass matrix-core resize (ok == false:

@@ -170,6 +170,9 @@ void DeviceProgram::lower()
 				}
 				std::snprintf(label, sizeof(label), "%s[%d]", n.out.type == PixelType::HALF ? "to_half" : "to_float", n.plane);
 			} else if (n.dither == Dither::ERROR_DIFFUSION) {
+				if (n.width > dev::kEdMaxWidth)
+					fail(ZIMG_ERROR_UNSUPPORTED_OPERATION, "error diffusion supports planes up to " + std::to_string(dev::kEdMaxWidth) +
+					                                           " samples wide");
 				s.kind = Step::ED;
 				s.ed_scale[0] = n.scale;
 				s.ed_offset[0] = n.offset;
@@ -1141,6 +1144,32 @@ const DeviceProgram::Tables &DeviceProgram::tables_for_current_device() const
 	}
 	m_tables[device] = tab;
 	return *tab;
+}
+
+void DeviceProgram::prepare_current_device() const
+{
+	int device = 0;
+	check_hip(hipGetDevice(&device), "hipGetDevice");
+	tables_for_current_device();
+	for (const Step &s : m_steps) {
+		if (s.dead || s.kind != Step::RESIZE)
+			continue;
+		if (mfma_step(s))
+			mfma_tables_for_current_device();
+		else if (s.imma.ok)
+			imma_tables_for_current_device();
+	}
+	std::lock_guard<std::mutex> lock(m_mutex);
+	m_prepared.insert(device * 4 + m_precision);
+}
+
+bool DeviceProgram::resident_on_current_device() const
+{
+	int device = 0;
+	if (hipGetDevice(&device) != hipSuccess)
+		return false;
+	std::lock_guard<std::mutex> lock(m_mutex);
+	return m_prepared.count(device * 4 + m_precision) != 0;
 }
 
 bool DeviceProgram::aligned16(const dev::PlaneArg &p)

@@ -25,15 +25,32 @@ dev::PlaneArg plane_of(const DeviceBuffers &b, int idx) { return dev::PlaneArg{ 
 std::mutex g_tune_mutex;
 std::map<std::string, TiledConfig> g_tuned;
 
+// FNV-1a over a table's windows and taps: two steps share a tuned tile shape
+// only when every staged window the kernel forms is the same.
+uint64_t table_hash(const FilterTable *t, uint64_t h)
+{
+	if (!t)
+		return h;
+	auto mix = [&](const void *p, size_t n) {
+		const unsigned char *b = static_cast<const unsigned char *>(p);
+		for (size_t k = 0; k < n; ++k)
+			h = (h ^ b[k]) * 1099511628211ULL;
+	};
+	mix(t->left.data(), t->left.size() * sizeof(t->left[0]));
+	mix(t->data_i16.data(), t->data_i16.size() * sizeof(t->data_i16[0]));
+	mix(&t->filter_width, sizeof(t->filter_width));
+	return h;
+}
+
 std::string shape_key(const DeviceProgram &prog, const DeviceProgram::Step &s)
 {
 	char buf[256];
 	const auto &nodes = prog.plan().nodes;
 	const FilterTable &t1 = *nodes[s.tab[0]].table;
 	const FilterTable *t2 = s.passes == 2 ? nodes[s.tab[1]].table.get() : nullptr;
-	std::snprintf(buf, sizeof(buf), "%ux%u>%ux%u p%d h%d w%d fw%u/%u l%u/%u in%d/%d out%d/%d", s.in_w, s.in_h, s.width, s.height, s.passes,
-	              s.h_first ? 1 : 0, s.word ? 1 : 0, t1.filter_width, t2 ? t2->filter_width : 0, t1.left.empty() ? 0 : t1.left.back(),
-	              t2 && !t2->left.empty() ? t2->left.back() : 0, s.ix[0].type, s.ix[0].op, s.ox[0].type, s.ox[0].op);
+	const uint64_t h = table_hash(t2, table_hash(&t1, 14695981039346656037ULL));
+	std::snprintf(buf, sizeof(buf), "%ux%u>%ux%u p%d h%d w%d in%d/%d out%d/%d t%016llx", s.in_w, s.in_h, s.width, s.height, s.passes,
+	              s.h_first ? 1 : 0, s.word ? 1 : 0, s.ix[0].type, s.ix[0].op, s.ox[0].type, s.ox[0].op, static_cast<unsigned long long>(h));
 	return buf;
 }
 
@@ -118,7 +135,10 @@ DeviceJobs::DeviceJobs(std::vector<Job> jobs, void *tmp, unsigned flags, void *s
 	size_t off = 0;
 	for (const Job &j : m_jobs) {
 		const DeviceProgram &prog = *j.prog;
-		const DeviceProgram::Tables &tabs = prog.tables_for_current_device(); // uploads tables outside any capture
+		// Every table the job's launches read (its own launch sequence included)
+		// is uploaded here, outside the capture below.
+		prog.prepare_current_device();
+		const DeviceProgram::Tables &tabs = prog.tables_for_current_device();
 		m_tmp_off.push_back(off);
 		const bool pers = eligible(prog, j.src, j.dst);
 		m_persistent.push_back(pers ? 1 : 0);

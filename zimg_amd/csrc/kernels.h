@@ -134,6 +134,10 @@ struct CspArg {
 	CspProgram prog;
 };
 
+// Widest plane the error-diffusion wavefront takes (its LDS ring of rows
+// handed between strips holds at most 1024 floats per wave).
+constexpr unsigned kEdMaxWidth = 16257;
+
 // Error diffusion over up to 4 planes of equal size in one launch (grid.y = plane).
 struct EdArg {
 	PlaneArg src[4], dst[4];
