@@ -422,6 +422,24 @@ typedef enum zimg_resize_precision_e {
 ZIMG_VISIBILITY
 zimg_error_code_e zimg_filter_graph_set_resize_precision(zimg_filter_graph *ptr, int mode);
 
+/* device extension, no counterpart in the reference: the allocation-free form
+ * of zimg_filter_graph_process_device_batch, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The graph's intermediate planes live in a library
+ * scratch array of 128 MiB per GPU (held until the process ends) instead of
+ * a caller tmp; a call whose zimg_filter_graph_get_device_tmp_size(count)
+ * exceeds it fails with ZIMG_ERROR_ILLEGAL_ARGUMENT. Nothing is allocated:
+ * the graph's coefficient tables must already be on the current device
+ * (zimg_filter_graph_build and _set_resize_precision upload them to the
+ * device current at the time), else ZIMG_ERROR_UNSUPPORTED_OPERATION. All
+ * calls on one GPU share its scratch: calls that overlap in time on one GPU
+ * (other streams, other threads) race and must be serialised by the caller. */
+ZIMG_VISIBILITY
+zimg_error_code_e zimg_filter_graph_process_device_batch_noalloc(const zimg_filter_graph *ptr, unsigned count,
+                                                                 const zimg_image_buffer_const *src, const ptrdiff_t src_frame_stride[4],
+                                                                 const zimg_image_buffer *dst, const ptrdiff_t dst_frame_stride[4],
+                                                                 void *stream);
+
 /* device extension: zimg_filter_graph_process_device_batch with HIP events
  * recorded around each device stage on `stream`; waits for the stream, then
  * writes up to max_stages stage durations (ms) and algorithmic bytes per

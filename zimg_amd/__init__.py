@@ -65,7 +65,7 @@ EXPORTS = [
     "zimg_filter_graph_get_input_buffering", "zimg_filter_graph_get_output_buffering", "zimg_filter_graph_process",
     "zimg_image_format_default", "zimg_graph_builder_params_default", "zimg_filter_graph_build",
     "zimg_filter_graph_get_device_tmp_size", "zimg_filter_graph_process_device",
-    "zimg_filter_graph_process_device_batch", "zimg_filter_graph_describe",
+    "zimg_filter_graph_process_device_batch", "zimg_filter_graph_process_device_batch_noalloc", "zimg_filter_graph_describe",
     "zimg_filter_graph_set_resize_precision", "zimg_filter_graph_process_device_profiled",
     "zimg_device_jobs_get_tmp_size", "zimg_device_jobs_create", "zimg_device_jobs_run", "zimg_device_jobs_describe",
     "zimg_device_jobs_free",
@@ -164,6 +164,10 @@ def lib():
         L.zimg_filter_graph_process_device_batch.argtypes = [vp, ctypes.c_uint, ctypes.POINTER(ImageBufferStruct),
                                                              ctypes.POINTER(ctypes.c_ssize_t), ctypes.POINTER(ImageBufferStruct),
                                                              ctypes.POINTER(ctypes.c_ssize_t), vp, vp]
+        L.zimg_filter_graph_process_device_batch_noalloc.restype = ctypes.c_int
+        L.zimg_filter_graph_process_device_batch_noalloc.argtypes = [vp, ctypes.c_uint, ctypes.POINTER(ImageBufferStruct),
+                                                                     ctypes.POINTER(ctypes.c_ssize_t), ctypes.POINTER(ImageBufferStruct),
+                                                                     ctypes.POINTER(ctypes.c_ssize_t), vp]
         L.zimg_filter_graph_describe.restype = sz
         L.zimg_filter_graph_describe.argtypes = [vp, ctypes.c_char_p, sz]
         L.zimg_filter_graph_process_device_profiled.restype = ctypes.c_int
@@ -353,6 +357,16 @@ class FilterGraph:
         ss = (ctypes.c_ssize_t * 4)(*src_frame_stride)
         ds = (ctypes.c_ssize_t * 4)(*dst_frame_stride)
         check(lib().zimg_filter_graph_process_device_batch(self._h, count, ctypes.byref(src), ss, ctypes.byref(dst), ds, tmp, stream))
+
+
+NOALLOC_SCRATCH_BYTES = 128 << 20
+
+
+def process_device_batch_noalloc(graph, count, src, src_frame_stride, dst, dst_frame_stride, stream=0):
+    """zimg_filter_graph_process_device_batch_noalloc: no tmp, no allocation."""
+    ss = (ctypes.c_ssize_t * 4)(*src_frame_stride)
+    ds = (ctypes.c_ssize_t * 4)(*dst_frame_stride)
+    check(lib().zimg_filter_graph_process_device_batch_noalloc(graph.handle, count, ctypes.byref(src), ss, ctypes.byref(dst), ds, stream))
 
 
 def device_jobs_tmp_size(graphs):

@@ -55,6 +55,42 @@ struct HostStaging {
 
 } // namespace
 
+// Device scratch of the allocation-free entry point
+// (zimg_filter_graph_process_device_batch_noalloc): the intermediate planes a
+// graph keeps in its device tmp, for callers that cannot allocate. Every GPU
+// the library runs on holds one copy until the process ends; in host code
+// the array's name is not a device address, so launches use the address
+// hipGetSymbolAddress returns for the current device.
+constexpr size_t kNoAllocScratchBytes = size_t{ 128 } << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mutex;
+std::vector<void *> g_scratch_ptr; // by HIP device
+
+// Address of this device's copy of `scratch`, looked up once per device with
+// that device current.
+void *scratch_for_current_device()
+{
+	int device = 0;
+	if (hipGetDevice(&device) != hipSuccess)
+		zamd::fail(ZIMG_ERROR_UNKNOWN, "no HIP device");
+	std::lock_guard<std::mutex> lock(g_scratch_mutex);
+	if (static_cast<size_t>(device) >= g_scratch_ptr.size())
+		g_scratch_ptr.resize(static_cast<size_t>(device) + 1, nullptr);
+	if (!g_scratch_ptr[device]) {
+		void *p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !p)
+			zamd::fail(ZIMG_ERROR_UNKNOWN, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+		g_scratch_ptr[device] = p;
+	}
+	return g_scratch_ptr[device];
+}
+
+} // namespace
+
 struct zimg_filter_graph {
 	std::unique_ptr<zamd::DeviceProgram> prog;
 	unsigned src_subsample_h = 0, dst_subsample_h = 0;
@@ -709,6 +745,32 @@ zimg_error_code_e zimg_filter_graph_process_device_batch(const zimg_filter_graph
 		DeviceBuffers ds = import_device_buffer(src, src_frame_stride, plan.src_w, true);
 		DeviceBuffers dd = import_device_buffer(dst, dst_frame_stride, plan.dst_w, false);
 		prog.run(ds, dd, tmp, count, stream);
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
+zimg_error_code_e zimg_filter_graph_process_device_batch_noalloc(const zimg_filter_graph *ptr, unsigned count,
+                                                                 const zimg_image_buffer_const *src, const ptrdiff_t src_frame_stride[4],
+                                                                 const zimg_image_buffer *dst, const ptrdiff_t dst_frame_stride[4],
+                                                                 void *stream)
+{
+	try {
+		const DeviceProgram &prog = program(ptr);
+		const Plan &plan = prog.plan();
+		const size_t need = prog.tmp_size(count ? count : 1);
+		if (need > kNoAllocScratchBytes)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "the graph needs " + std::to_string(need) + " bytes of device scratch for " + std::to_string(count) +
+			                                      " frame(s); the allocation-free entry point holds " + std::to_string(kNoAllocScratchBytes));
+		if (!prog.resident_on_current_device())
+			fail(ZIMG_ERROR_UNSUPPORTED_OPERATION,
+			     "coefficient tables are not resident on the current device: build the graph (or set its precision) with this device current");
+		DeviceBuffers ds = import_device_buffer(src, src_frame_stride, plan.src_w, true);
+		DeviceBuffers dd = import_device_buffer(dst, dst_frame_stride, plan.dst_w, false);
+		// Every intermediate plane is written in full before it is read, so
+		// nothing in the scratch needs clearing between calls.
+		prog.run(ds, dd, need ? scratch_for_current_device() : nullptr, count, stream);
 		return ZIMG_ERROR_SUCCESS;
 	} catch (...) {
 		return handle_current_exception();
