@@ -113,8 +113,9 @@ def time_steps(torch, launch, steps, warmup):
     return start.elapsed_time(end) / 1000.0
 
 
-def build_workload(name, frames):
-    """(graph, input stacks, output shapes, algorithmic bytes per frame, dtype)."""
+def build_workload(name, frames, approx=False):
+    """(graph, input stacks, output shapes, algorithmic bytes per frame, dtype).
+    approx: allow_approximate_gamma (the reference's lookup-table gamma)."""
     import zimg_amd as z
     from zimg_amd import workloads as W
     if name in ("fp32_down", "fp32_up"):
@@ -132,6 +133,8 @@ def build_workload(name, frames):
         return g, planes, [(2160, 3840, np.uint16)] * 3 + [None], 3840 * 2160 * 3 * (4 + 2), "f32 in, u16 (12-bit) out"
     if name == "fhd_709_to_2020":
         s, d, p = W.fhd_709_to_2020_formats()
+        if approx:
+            p.allow_approximate_gamma = b"\x01"
         g = z.FilterGraph(s, d, p)
         Ys, Us, Vs = [], [], []
         for k in range(frames):
@@ -163,8 +166,10 @@ def counters_of(key):
     return out
 
 
-def measure(torch, name, frames, steps, warmup, precision=0):
-    g, planes, outs, algo_bytes, dtype = build_workload(name, frames)
+def measure(torch, name, frames, steps, warmup, precision=0, approx=False):
+    g, planes, outs, algo_bytes, dtype = build_workload(name, frames, approx)
+    if approx:
+        dtype += " (approximate gamma: lookup tables)"
     if precision:
         g.set_resize_precision(precision)
         dtype += f" (resize on MFMA, {'fp16' if precision == 1 else 'bf16'} hi+lo, FP32 accumulate)"
@@ -179,6 +184,8 @@ def measure(torch, name, frames, steps, warmup, precision=0):
     stage_ms = [ms / steps for ms in acc]
     k = int(np.argmax(stage_ms))
     key = name if not precision else f"{name}_{'fp16' if precision == 1 else 'bf16'}"
+    if approx:
+        key += "_approx"
     ctr = counters_of(key)
     dom_ms = stage_ms[k]
     achieved = algo_bytes * frames / (dom_ms * 1e-3) / 1e9
@@ -343,6 +350,8 @@ def main():
             results[name] = measure(torch, name, frames, args.steps, args.warmup)
             if name in ("fp32_down", "fp32_up"):
                 results[name]["mfma_fp16"] = measure(torch, name, frames, args.steps, args.warmup, precision=1)
+            if name == "fhd_709_to_2020":
+                results[name]["approximate_gamma"] = measure(torch, name, frames, args.steps, args.warmup, approx=True)
         if not args.no_cpu:
             results[name]["cpu_baseline"] = cpu_baseline(name, args.cpu_runs, full=name in ("fhd_709_to_2020", "uhd_lanczos"))
         print(f"# {name}: {results[name]['fps']} {results[name].get('unit', 'frames/s')}", file=sys.stderr, flush=True)
@@ -364,6 +373,7 @@ def main():
         "config": {"workload": primary, "frames_per_step": m["frames"], "graph_launches": m["launches"], "precision": "exact"},
         "roofline": m["roofline"],
         "cpu_baseline": m.get("cpu_baseline"),
+        "variants": {k: m[k] for k in ("approximate_gamma", "mfma_fp16") if k in m},
         "configs": {k: v for k, v in results.items() if k != primary},
     }
     print(json.dumps(line))

@@ -457,7 +457,9 @@ zimg_error_code_e zimg_filter_graph_process_device_profiled(const zimg_filter_gr
  * are planned once into one persistent launch over the resize tiles of all
  * jobs whose stages are all tiled integer resizes (the others run their own
  * launch sequences first), captured into a HIP graph at creation and
- * replayed by zimg_device_jobs_run. Buffers are fixed at creation. tmp holds
+ * replayed by zimg_device_jobs_run. Buffers are fixed at creation, and the
+ * graphs must outlive the jobs object (the captured launches read their
+ * coefficient tables). tmp holds
  * zimg_device_jobs_get_tmp_size bytes (may be NULL when that is 0).
  * ZIMG_DEVICE_JOBS_AUTOTUNE times the cost model's best tile shapes for each
  * distinct stage shape on the given planes at creation (results are cached

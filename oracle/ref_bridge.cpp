@@ -4,8 +4,10 @@
 // restatement with the reference bit for bit. Nothing here is shipped.
 //
 // Sources compiled (all as-is, no stand-in headers): resize/filter.cpp,
-// common/matrix.cpp, common/libm_wrapper.cpp, colorspace/{gamma,
-// colorspace_param,matrix3,operation,operation_impl,colorspace_graph}.cpp,
+// common/matrix.cpp, common/libm_wrapper.cpp, common/cpuinfo.cpp,
+// common/x86/cpuinfo_x86.cpp, colorspace/{gamma, colorspace_param,matrix3,
+// operation,operation_impl,colorspace_graph}.cpp, colorspace/x86/{operation_impl_x86,
+// operation_impl_avx2,operation_impl_avx512,gamma_constants_avx512}.cpp,
 // depth/{quantize,blue}.cpp and the vendored musl libm of test/extra/musl-libm.
 #include <cmath>
 #include <cstdint>
@@ -14,6 +16,7 @@
 #include <vector>
 
 #include "common/cpuinfo.h"
+#include "common/x86/cpuinfo_x86.h"
 #include "common/libm_wrapper.h"
 #include "common/pixel.h"
 #include "colorspace/colorspace.h"
@@ -128,9 +131,34 @@ void ref_transfer_scales(int transfer, double peak, int scene_referred, float *t
 // float planes in place, mirroring ColorspaceConversion::create and
 // ColorspaceConversionImpl::process (colorspace.cpp:45-122) with CPUClass::NONE.
 // Returns the number of operations, 0 for identity, -1 if no path.
+static int colorspace_apply(int m_in, int t_in, int p_in, int m_out, int t_out, int p_out, double peak, int scene_referred,
+                            int approximate_gamma, int chromatic_adaptation, zimg::CPUClass cpu, float *p0, float *p1, float *p2,
+                            size_t n);
+
 int ref_colorspace_apply(int m_in, int t_in, int p_in, int m_out, int t_out, int p_out,
                          double peak, int scene_referred, int approximate_gamma, int chromatic_adaptation,
                          float *p0, float *p1, float *p2, size_t n)
+{
+	return colorspace_apply(m_in, t_in, p_in, m_out, t_out, p_out, peak, scene_referred, approximate_gamma, chromatic_adaptation,
+	                        zimg::CPUClass::NONE, p0, p1, p2, n);
+}
+
+// The same with the reference's CPU dispatch (cpu AUTO): on an x86 host with
+// AVX2, FMA and F16C its AVX2 operations run. Returns -2 when the host lacks them.
+int ref_colorspace_apply_auto(int m_in, int t_in, int p_in, int m_out, int t_out, int p_out,
+                              double peak, int scene_referred, int approximate_gamma, int chromatic_adaptation,
+                              float *p0, float *p1, float *p2, size_t n)
+{
+	const zimg::X86Capabilities caps = zimg::query_x86_capabilities();
+	if (!caps.avx2 || !caps.fma || !caps.f16c)
+		return -2;
+	return colorspace_apply(m_in, t_in, p_in, m_out, t_out, p_out, peak, scene_referred, approximate_gamma, chromatic_adaptation,
+	                        zimg::CPUClass::AUTO, p0, p1, p2, n);
+}
+
+static int colorspace_apply(int m_in, int t_in, int p_in, int m_out, int t_out, int p_out, double peak, int scene_referred,
+                            int approximate_gamma, int chromatic_adaptation, zimg::CPUClass cpu, float *p0, float *p1, float *p2,
+                            size_t n)
 {
 	using namespace zimg::colorspace;
 	try {
@@ -148,7 +176,7 @@ int ref_colorspace_apply(int m_in, int t_in, int p_in, int m_out, int t_out, int
 		auto path = get_operation_path(in, out);
 		std::vector<std::unique_ptr<Operation>> ops;
 		for (auto &f : path)
-			ops.push_back(f(params, zimg::CPUClass::NONE));
+			ops.push_back(f(params, cpu));
 		float *planes[3] = { p0, p1, p2 };
 		for (auto &op : ops)
 			op->process(planes, planes, 0, static_cast<unsigned>(n));

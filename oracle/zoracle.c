@@ -1428,7 +1428,14 @@ void orc_gamut_matrix(int p_in, int p_out, int adapt, double out[9])
 
 /* ---- operations (operation_impl.cpp) and path search (colorspace_graph.cpp) ---- */
 
-enum { OP_MATRIX, OP_GAMMA, OP_B67, OP_B67_INV, OP_CL_TO_RGB, OP_CL_TO_YUV };
+enum { OP_MATRIX, OP_GAMMA, OP_B67, OP_B67_INV, OP_CL_TO_RGB, OP_CL_TO_YUV, OP_MATRIX_FMA, OP_LUT_LINEAR, OP_LUT_GAMMA };
+
+/* Nonzero while building the chain the reference runs with
+ * allow_approximate_gamma on an x86 CPU with AVX2, FMA and F16C (cpu AUTO,
+ * operation_impl_x86.cpp:12-77): matrices with FMA
+ * (operation_impl_avx2.cpp:43-68), gamma and inverse gamma through lookup
+ * tables (:258-382), and HLG taken scene-referred (operation.cpp:14-17). */
+static int g_csp_approx;
 enum { E_NCL_RGB_TO_YUV, E_NCL_YUV_TO_RGB, E_LIN_TO_GAMMA, E_GAMMA_TO_LIN, E_CL_RGB_TO_YUV, E_CL_YUV_TO_RGB,
        E_GAMUT, E_LMS_TO_ICTCP, E_ICTCP_TO_LMS };
 
@@ -1506,7 +1513,7 @@ static int neighbours(csp_t c, edge_t *e)
 
 static void set_matrix_op(orc_csp_ops *ops, int k, mat3 m)
 {
-	ops->kind[k] = OP_MATRIX;
+	ops->kind[k] = g_csp_approx ? OP_MATRIX_FMA : OP_MATRIX;
 	for (int i = 0; i < 3; ++i)
 		for (int j = 0; j < 3; ++j)
 			ops->m[k][i][j] = (float)m.m[i][j];
@@ -1544,7 +1551,7 @@ static int make_op_inner(orc_csp_ops *ops, int k, int kind, csp_t in, csp_t out,
 		set_matrix_op(ops, k, lms_to_ictcp(in.t));
 		return 0;
 	case E_GAMMA_TO_LIN:
-		display_b67 = in.p != P_UNSPEC && !scene;
+		display_b67 = in.p != P_UNSPEC && !scene && !g_csp_approx;
 		if (in.t == T_B67 && display_b67) {
 			m = ncl_rgb_to_yuv_from_primaries(in.p);
 			orc_transfer_scales(T_B67, peak, 0, &sl, &sg);
@@ -1556,13 +1563,13 @@ static int make_op_inner(orc_csp_ops *ops, int k, int kind, csp_t in, csp_t out,
 		if (select_tf(in.t, scene, &fl, &fg))
 			return 1;
 		orc_transfer_scales(in.t, peak, scene, &sl, &sg);
-		ops->kind[k] = OP_GAMMA;
+		ops->kind[k] = g_csp_approx ? OP_LUT_LINEAR : OP_GAMMA;
 		ops->func[k] = fl;
 		ops->pre[k] = 1.0f;
 		ops->post[k] = sl;
 		return 0;
 	case E_LIN_TO_GAMMA:
-		display_b67 = out.p != P_UNSPEC && !scene;
+		display_b67 = out.p != P_UNSPEC && !scene && !g_csp_approx;
 		if (out.t == T_B67 && display_b67) {
 			m = ncl_rgb_to_yuv_from_primaries(out.p);
 			orc_transfer_scales(T_B67, peak, 0, &sl, &sg);
@@ -1574,7 +1581,7 @@ static int make_op_inner(orc_csp_ops *ops, int k, int kind, csp_t in, csp_t out,
 		if (select_tf(out.t, scene, &fl, &fg))
 			return 1;
 		orc_transfer_scales(out.t, peak, scene, &sl, &sg);
-		ops->kind[k] = OP_GAMMA;
+		ops->kind[k] = g_csp_approx ? OP_LUT_GAMMA : OP_GAMMA;
 		ops->func[k] = fg;
 		ops->pre[k] = sg;
 		ops->post[k] = 1.0f;
@@ -1680,6 +1687,38 @@ int orc_colorspace_build(int m_in, int t_in, int p_in, int m_out, int t_out, int
 	return nk;
 }
 
+int orc_colorspace_build_approx(int m_in, int t_in, int p_in, int m_out, int t_out, int p_out,
+                                double peak_luminance, int scene_referred, int chromatic_adaptation,
+                                orc_csp_ops *ops, char *trace, size_t trace_len)
+{
+	g_csp_approx = 1;
+	int rc = orc_colorspace_build(m_in, t_in, p_in, m_out, t_out, p_out, peak_luminance, scene_referred, chromatic_adaptation,
+	                              ops, trace, trace_len);
+	g_csp_approx = 0;
+	return rc;
+}
+
+/* Lookup-table gamma of the AVX2 path. Entries are computed where they are
+ * used, by the table's own formula. To linear (operation_impl_avx2.cpp:
+ * 122-156, 258-274): 2^16 + 1 entries, entry i = f(i / 65536 * 2 - 0.5) * post,
+ * index = cvtps2dq(fma(x, 32768, 16384)) (round to nearest even; NaN and
+ * values outside int32 give INT32_MIN) clamped to [0, 65536]. To gamma
+ * (:194-240, 304-320): 2^16 entries indexed by the binary16 bits of x
+ * (round to nearest even), entry h = f(half(h) * pre). */
+static float lut_to_linear(int func, float post, float x)
+{
+	float v = fmaf(x, 32768.0f, 16384.0f);
+	int32_t idx = (v >= -2147483648.0f && v < 2147483648.0f) ? (int32_t)rintf(v) : INT32_MIN;
+	idx = idx < 0 ? 0 : idx > 65536 ? 65536 : idx;
+	float g = (float)idx / 65536.0f * 2.0f - 0.5f;
+	return tf_eval(func, g) * post;
+}
+
+static float lut_to_gamma(int func, float pre, float x)
+{
+	return tf_eval(func, orc_half_to_float(orc_float_to_half(x)) * pre);
+}
+
 void orc_colorspace_apply(const orc_csp_ops *ops, float *p0, float *p1, float *p2, size_t n)
 {
 	for (int k = 0; k < ops->n; ++k) {
@@ -1694,6 +1733,27 @@ void orc_colorspace_apply(const orc_csp_ops *ops, float *p0, float *p1, float *p
 				float z = m[2][0] * a + m[2][1] * b + m[2][2] * c;
 				p0[i] = x; p1[i] = y; p2[i] = z;
 			}
+			break;
+		}
+		case OP_MATRIX_FMA: {
+			/* operation_impl_avx2.cpp:43-68: a product, then two fused multiply-adds */
+			const float (*m)[3] = ops->m[k];
+			for (size_t i = 0; i < n; ++i) {
+				float a = p0[i], b = p1[i], c = p2[i];
+				float x = fmaf(m[0][2], c, fmaf(m[0][1], b, m[0][0] * a));
+				float y = fmaf(m[1][2], c, fmaf(m[1][1], b, m[1][0] * a));
+				float z = fmaf(m[2][2], c, fmaf(m[2][1], b, m[2][0] * a));
+				p0[i] = x; p1[i] = y; p2[i] = z;
+			}
+			break;
+		}
+		case OP_LUT_LINEAR:
+		case OP_LUT_GAMMA: {
+			float *pl[3] = { p0, p1, p2 };
+			for (int p = 0; p < 3; ++p)
+				for (size_t i = 0; i < n; ++i)
+					pl[p][i] = ops->kind[k] == OP_LUT_LINEAR ? lut_to_linear(ops->func[k], ops->post[k], pl[p][i])
+					                                         : lut_to_gamma(ops->func[k], ops->pre[k], pl[p][i]);
 			break;
 		}
 		case OP_GAMMA: {

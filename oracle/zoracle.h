@@ -142,6 +142,12 @@ typedef struct orc_csp_ops {
 int orc_colorspace_build(int m_in, int t_in, int p_in, int m_out, int t_out, int p_out,
                          double peak_luminance, int scene_referred, int chromatic_adaptation,
                          orc_csp_ops *ops, char *trace, size_t trace_len);
+/* The chain the reference runs with allow_approximate_gamma on an x86 CPU
+ * with AVX2, FMA and F16C (cpu AUTO): FMA matrices, lookup-table gamma,
+ * scene-referred HLG. Same arguments and result as orc_colorspace_build. */
+int orc_colorspace_build_approx(int m_in, int t_in, int p_in, int m_out, int t_out, int p_out,
+                                double peak_luminance, int scene_referred, int chromatic_adaptation,
+                                orc_csp_ops *ops, char *trace, size_t trace_len);
 void orc_colorspace_apply(const orc_csp_ops *ops, float *p0, float *p1, float *p2, size_t n);
 /* Double-precision matrix helpers exposed for tests. */
 void orc_ncl_rgb_to_yuv(int matrix, double out[9]);

@@ -121,6 +121,8 @@ def oracle():
         lib.orc_colorspace_build.restype = ctypes.c_int
         lib.orc_colorspace_build.argtypes = [ctypes.c_int] * 6 + [ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                                                   ctypes.POINTER(CspOps), ctypes.c_char_p, ctypes.c_size_t]
+        lib.orc_colorspace_build_approx.restype = ctypes.c_int
+        lib.orc_colorspace_build_approx.argtypes = lib.orc_colorspace_build.argtypes
         lib.orc_colorspace_apply.restype = None
         lib.orc_colorspace_apply.argtypes = [ctypes.POINTER(CspOps), c_float_p, c_float_p, c_float_p, ctypes.c_size_t]
         lib.orc_ncl_rgb_to_yuv.restype = None
@@ -154,6 +156,8 @@ def ref():
         lib.ref_colorspace_apply.restype = ctypes.c_int
         lib.ref_colorspace_apply.argtypes = [ctypes.c_int] * 6 + [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                                   c_float_p, c_float_p, c_float_p, ctypes.c_size_t]
+        lib.ref_colorspace_apply_auto.restype = ctypes.c_int
+        lib.ref_colorspace_apply_auto.argtypes = lib.ref_colorspace_apply.argtypes
         lib.ref_ncl_rgb_to_yuv.argtypes = [ctypes.c_int, c_double_p]
         lib.ref_half_to_float.restype = ctypes.c_float
         lib.ref_half_to_float.argtypes = [ctypes.c_uint16]
@@ -343,10 +347,13 @@ def left_shift(img, type_in, type_out, shift):
     return out
 
 
-def colorspace_ops(csp_in, csp_out, peak=100.0, scene=0, adapt=0):
+def colorspace_ops(csp_in, csp_out, peak=100.0, scene=0, adapt=0, approx=0):
+    """Operation chain; approx=1 gives the chain of the reference's AVX2 path with
+    allow_approximate_gamma (FMA matrices, lookup-table gamma)."""
     ops = CspOps()
     buf = ctypes.create_string_buffer(512)
-    n = oracle().orc_colorspace_build(*csp_in, *csp_out, peak, scene, adapt, ctypes.byref(ops), buf, 512)
+    build = oracle().orc_colorspace_build_approx if approx else oracle().orc_colorspace_build
+    n = build(*csp_in, *csp_out, peak, scene, adapt, ctypes.byref(ops), buf, 512)
     return n, ops, buf.value.decode()
 
 
@@ -358,10 +365,19 @@ def colorspace_apply(ops, p0, p1, p2):
     return p0, p1, p2
 
 
-def ref_colorspace_apply(csp_in, csp_out, p0, p1, p2, peak=100.0, scene=0, approx=0, adapt=0):
-    p0 = np.ascontiguousarray(p0, np.float32).copy()
-    p1 = np.ascontiguousarray(p1, np.float32).copy()
-    p2 = np.ascontiguousarray(p2, np.float32).copy()
-    n = ref().ref_colorspace_apply(*csp_in, *csp_out, peak, scene, approx, adapt, ptr(p0, c_float_p), ptr(p1, c_float_p),
-                                   ptr(p2, c_float_p), p0.size)
+def _aligned_copy(a, align=64):
+    buf = np.empty(a.size + align // a.itemsize, a.dtype)
+    off = (-buf.ctypes.data % align) // a.itemsize
+    out = buf[off:off + a.size].reshape(a.shape)
+    out[...] = a
+    return out
+
+
+def ref_colorspace_apply(csp_in, csp_out, p0, p1, p2, peak=100.0, scene=0, approx=0, adapt=0, cpu_auto=False):
+    """The compiled reference's chain: its C path, or with cpu_auto its x86
+    dispatch (AVX2 operations; -2 when the host lacks AVX2/FMA/F16C)."""
+    # rows 64-byte aligned, as the reference's executor provides them (its SIMD loads require it)
+    p0, p1, p2 = (_aligned_copy(np.ascontiguousarray(p, np.float32)) for p in (p0, p1, p2))
+    fn = ref().ref_colorspace_apply_auto if cpu_auto else ref().ref_colorspace_apply
+    n = fn(*csp_in, *csp_out, peak, scene, approx, adapt, ptr(p0, c_float_p), ptr(p1, c_float_p), ptr(p2, c_float_p), p0.size)
     return n, p0, p1, p2

@@ -24,9 +24,10 @@ def float_resize(src, dst_w, dst_h, ftype, a, b):
     return resize_plane(src, ftype, a, b, dst_w, dst_h)
 
 
-def yuv420_10_709_to_2020(Y, U, V):
+def yuv420_10_709_to_2020(Y, U, V, approx=0):
     """FHD 4:2:0 10-bit limited BT.709 -> BT.2020 NCL (transfer BT.709), chroma
-    siting LEFT (API default), bicubic b=0 c=0.5 chroma filter.
+    siting LEFT (API default), bicubic b=0 c=0.5 chroma filter. approx=1: the
+    colourspace chain of allow_approximate_gamma (the reference's AVX2 path).
 
     Chain (graphbuilder.cpp:967-1019): luma WORD->FLOAT; chroma WORD->FLOAT at
     half size, resize to 4:4:4 (shift_w +0.25); colourspace; luma FLOAT->WORD;
@@ -43,7 +44,7 @@ def yuv420_10_709_to_2020(Y, U, V):
     Vf = to_float(V, WORD, s, o)
     up = lambda c: resize_plane(c, BICUBIC, 0.0, 0.5, w, h, shift_w=0.25, shift_h=0.0, subwidth=w / 2, subheight=h / 2)
     Uf, Vf = up(Uf), up(Vf)
-    n, ops, _ = colorspace_ops((M_709, T_709, P_709), (M_2020, T_709, P_2020), 100.0, 0, 0)
+    n, ops, _ = colorspace_ops((M_709, T_709, P_709), (M_2020, T_709, P_2020), 100.0, 0, 0, approx=approx)
     assert n == 5
     Yc, Uc, Vc = colorspace_apply(ops, Yf, Uf, Vf)
     s, o = scale_offset(luma_f, luma_in)

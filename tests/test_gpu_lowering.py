@@ -135,6 +135,55 @@ def test_colorspace_every_family_on_device(peak, scene, adapt):
     assert built > 700 and nopath > 10, (built, nopath)
 
 
+@pytest.mark.parametrize("peak,scene,adapt", PARAM_SETS)
+def test_colorspace_approximate_gamma_on_device(peak, scene, adapt):
+    """allow_approximate_gamma: the device runs the reference's AVX2 chain
+    (FMA matrices, lookup-table gamma, scene-referred HLG), bit for bit
+    against the oracle's restatement of it (itself pinned against the
+    compiled AVX2 sources), over every transfer pair, the matrix and gamut
+    pairs and the random triples, on values that reach the table edges."""
+    print(f"case: approximate gamma peak {peak} scene {scene} adapt {adapt}", flush=True)
+    w, h = 67, 13
+    r = np.random.default_rng(8)
+    planes = [r.uniform(-0.7, 1.7, (h, w)).astype(np.float32) for _ in range(3)]
+    planes[0][0, :16] = [-0.5, 1.5, -0.51, 1.51, 0.0, 1.0, 1e-40, 3e9, -3e9, np.inf, -np.inf, np.nan, 65504.0, 65520.0, 0.25, 2.0 ** -14]
+    params = z.builder_params(version=API_2_5, nominal_peak_luminance=peak, scene_referred=scene, chromatic_adaptation=adapt,
+                              allow_approximate_gamma=1)
+    built = 0
+    for cin, cout in _csp_pairs():
+        ii = (MATRIX[cin[0]], TRANSFER[cin[1]], PRIMARIES[cin[2]])
+        oo = (MATRIX[cout[0]], TRANSFER[cout[1]], PRIMARIES[cout[2]])
+        n, ops, _ = colorspace_ops(ii, oo, peak, scene, adapt, approx=1)
+        try:
+            g = z.FilterGraph(_float_fmt(w, h, *cin), _float_fmt(w, h, *cout), params)
+        except z.ZimgError as e:
+            assert n < 0 and e.code in (z.ERROR_NO_COLORSPACE_CONVERSION, z.ERROR_UNKNOWN), (cin, cout, e)
+            continue
+        assert n >= 0, (cin, cout)
+        outs = gpu_util.run_graph(g, planes + [None], [(h, w, np.float32)] * 3 + [None])
+        want = colorspace_apply(ops, *planes) if n else planes
+        for k in range(3):
+            _assert_bits(outs[k], want[k], f"{cin}->{cout} plane {k}")
+        built += 1
+    assert built > 700, built
+
+
+def test_fhd_709_to_2020_approximate_gamma():
+    """The 4:2:0 10-bit BT.709 -> BT.2020 chain with allow_approximate_gamma:
+    lookup-table gamma in the colourspace stage, bit for bit."""
+    w, h = 640, 360
+    print(f"case: 4:2:0 10-bit 709->2020 approximate gamma {w}x{h}", flush=True)
+    from zimg_amd import workloads as W
+    Y, U, V = W.yuv420_10bit(w, h, seed=21)
+    s, d, p = W.fhd_709_to_2020_formats(w, h)
+    p.allow_approximate_gamma = b"\x01"
+    g = z.FilterGraph(s, d, p)
+    assert "lut_gamma" in g.describe() and "lut_linear" in g.describe()
+    outs = _run420(g, [Y, U, V], w, h)
+    for got, ref, name in zip(outs[:3], P.yuv420_10_709_to_2020(Y, U, V, approx=1), "YUV"):
+        _assert_bits(got, ref, f"plane {name}")
+
+
 def _word420_fmt(w, h, depth=10, **kw):
     d = dict(width=w, height=h, pixel_type=z.PIXEL_WORD, depth=depth, subsample_w=1, subsample_h=1, color_family=z.COLOR_YUV,
              matrix_coefficients=z.MATRIX_BT709, transfer_characteristics=z.TRANSFER_BT709, color_primaries=z.PRIMARIES_BT709)

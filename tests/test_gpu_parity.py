@@ -91,6 +91,24 @@ def test_word_resize_integer_matrix_cores(sw, sh, dw, dh, ft, a, b, depth, frame
         _assert_equal(out[k], P.word_resize(src[k], dw, dh, ft, a, b, depth), f"frame {k}")
 
 
+@pytest.mark.parametrize("sw,sh,dw,dh,frames", [(333, 200, 700, 419, 2), (16, 20, 61, 47, 1)])
+def test_byte_to_word_upscale_integer_matrix_cores(sw, sh, dw, dh, frames):
+    """BYTE 8-bit promoted to WORD 16-bit by the left shift folded into the
+    matrix-core kernel's staging (depth_convert.cpp:23-30), then resized."""
+    print(f"case: BYTE->WORD int8 matrix cores {sw}x{sh}->{dw}x{dh} frames {frames}", flush=True)
+    g = z.FilterGraph(z.image_format(width=sw, height=sh, pixel_type=z.PIXEL_BYTE, depth=8),
+                      z.image_format(width=dw, height=dh, pixel_type=z.PIXEL_WORD, depth=16),
+                      z.builder_params(resample_filter=z.RESIZE_SPLINE36))
+    assert "integer matrix cores" in g.describe()
+    r = np.random.default_rng(sw)
+    src = r.integers(0, 256, (frames, sh, sw)).astype(np.uint8)
+    out, *_ = gpu_util.run_graph(g, [src, None, None, None], [(dh, dw, np.uint16), None, None, None], pitch_align=16)
+    out = out if frames > 1 else out[None]
+    for k in range(frames):
+        want = P.word_resize(left_shift(src[k], BYTE, WORD, 8), dw, dh, SPLINE36, 0.0, 0.0, 16)
+        _assert_equal(out[k], want, f"frame {k}")
+
+
 def _alpha_fmt(w, h, alpha):
     return z.image_format(width=w, height=h, pixel_type=z.PIXEL_FLOAT, color_family=z.COLOR_RGB,
                           matrix_coefficients=z.MATRIX_RGB, transfer_characteristics=z.TRANSFER_BT709,

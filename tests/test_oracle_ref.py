@@ -105,6 +105,50 @@ def test_colorspace_chains_bit_exact():
     assert checked > 1000
 
 
+def test_approximate_gamma_chains_bit_exact():
+    """allow_approximate_gamma: the oracle's restatement of the AVX2 path
+    (FMA matrices, 65537-entry to-linear and 65536-entry to-gamma tables,
+    scene-referred HLG) against the reference's own AVX2 sources compiled
+    here and run through its cpu AUTO dispatch, over every transfer pair and
+    a seeded sample of whole colourspace pairs, on values that reach every
+    table edge: the domain ends, out-of-range and huge values, infinities,
+    NaN, denormals and exact table points."""
+    ref().ref_set_libm(1)
+    rng = np.random.default_rng(41)
+    edge = np.array([-0.5, 1.5, -0.51, 1.51, 0.0, -0.0, 1.0, 1e-40, -1e-40, 3e9, -3e9, 1e30, np.inf, -np.inf, np.nan,
+                     2.0 ** -14, 65504.0, 65520.0, 0.25, 0.5 + 1.0 / 65536], np.float32)
+    grid = (np.arange(0, 65537, 97, dtype=np.float32) / 65536 * 2 - 0.5).astype(np.float32)
+    base = np.concatenate([edge, grid, rng.uniform(-0.7, 1.7, 3000).astype(np.float32),
+                           rng.uniform(0, 1, 1000).astype(np.float32) ** 4])
+    pl = [np.roll(base, 7 * k) for k in range(3)]
+    csps = [(m, t, p) for m in range(13) for t in range(14) for p in range(11)]
+    pairs = [((1, ti, 4), (1, to, 4)) for ti in range(1, 14) for to in range(1, 14) if ti != to]
+    pairs += [(csps[i], csps[j]) for i, j in np.random.default_rng(43).integers(0, len(csps), size=(600, 2))]
+    pairs += [((3, 4, 4), (7, 4, 6)), ((1, 11, 6), (1, 13, 6)), ((12, 11, 6), (1, 1, 6)), ((8, 4, 6), (1, 1, 6))]
+    checked = 0
+    for cin, cout in pairs:
+        for peak, scene, adapt in [(100.0, 0, 0), (203.0, 0, 1), (1000.0, 1, 0)]:
+            n_o, ops, _ = colorspace_ops(cin, cout, peak, scene, adapt, approx=1)
+            n_r, a0, a1, a2 = ref_colorspace_apply(cin, cout, *pl, peak, scene, 1, adapt, cpu_auto=True)
+            if n_r == -2:
+                pytest.skip("host CPU lacks AVX2/FMA/F16C")
+            assert (n_o < 0) == (n_r < 0), (cin, cout)
+            if n_o < 0:
+                continue
+            assert n_o == n_r
+            b = colorspace_apply(ops, *pl)
+            for x, y in zip((a0, a1, a2), b):
+                assert np.array_equal(x, y, equal_nan=True), (cin, cout, peak, scene, adapt)
+            checked += 1
+    assert checked > 400
+    # the mode changes results: the tables are approximations of the exact chain
+    n_e, ops_e, _ = colorspace_ops((3, 4, 4), (7, 4, 6))
+    n_a, ops_a, _ = colorspace_ops((3, 4, 4), (7, 4, 6), approx=1)
+    assert n_e == n_a == 5
+    ex, ap = colorspace_apply(ops_e, *pl), colorspace_apply(ops_a, *pl)
+    assert not np.array_equal(ex[0], ap[0], equal_nan=True)
+
+
 def test_half_conversions_bit_exact():
     """Every half value, and floats over the whole exponent range incl. ties,
     denormal results, overflow, infinities and quiet NaNs with payloads; bits

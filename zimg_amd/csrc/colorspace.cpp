@@ -446,6 +446,14 @@ CspOp make_op(Edge e, const Colorspace &in, const Colorspace &out, const CspPara
 		break;
 	}
 	}
+	if (p.approximate_gamma) {
+		// The reference's AVX2 operations (operation_impl_x86.cpp:12-77): the
+		// constant-luminance and HLG operations have none and stay exact.
+		if (op.kind == CSP_MATRIX)
+			op.kind = CSP_MATRIX_FMA;
+		else if (op.kind == CSP_GAMMA)
+			op.kind = e == E_GAMMA_TO_LIN ? CSP_LUT_LINEAR : CSP_LUT_GAMMA;
+	}
 	return op;
 }
 
@@ -453,6 +461,9 @@ const char *op_name(int k)
 {
 	switch (k) {
 	case CSP_MATRIX: return "matrix";
+	case CSP_MATRIX_FMA: return "matrix_fma";
+	case CSP_LUT_LINEAR: return "lut_linear";
+	case CSP_LUT_GAMMA: return "lut_gamma";
 	case CSP_GAMMA: return "gamma";
 	case CSP_B67: return "hlg_oetf_ootf";
 	case CSP_B67_INV: return "hlg_eotf";
@@ -463,6 +474,27 @@ const char *op_name(int k)
 }
 
 } // namespace
+
+std::vector<float> build_gamma_lut(const CspOp &op)
+{
+	std::vector<float> lut;
+	if (op.kind == CSP_LUT_LINEAR) {
+		// operation_impl_avx2.cpp:258-274: entry i = f(i / 2^16 * 2 - 0.5) * post
+		lut.resize(kLutLinearSize);
+		for (unsigned i = 0; i < kLutLinearSize; ++i) {
+			const float x = static_cast<float>(i) / 65536.0f * 2.0f - 0.5f;
+			lut[i] = zm::transfer(op.func, x) * op.post;
+		}
+	} else if (op.kind == CSP_LUT_GAMMA) {
+		// :304-320: entry h = f(half(h) * pre)
+		lut.resize(kLutGammaSize);
+		for (unsigned h = 0; h < kLutGammaSize; ++h) {
+			const float x = zm::half_to_float(static_cast<uint16_t>(h));
+			lut[h] = zm::transfer(op.func, x * op.pre);
+		}
+	}
+	return lut;
+}
 
 std::string CspProgram::describe() const
 {

@@ -5,12 +5,23 @@
 #pragma once
 
 #include <string>
+#include <vector>
 
 #include "common.h"
 
 namespace zamd {
 
-enum CspOpKind : int { CSP_MATRIX = 0, CSP_GAMMA = 1, CSP_B67 = 2, CSP_B67_INV = 3, CSP_CL_TO_RGB = 4, CSP_CL_TO_YUV = 5 };
+// CSP_MATRIX_FMA, CSP_LUT_LINEAR and CSP_LUT_GAMMA are the operations the
+// reference runs with allow_approximate_gamma on an x86 CPU with AVX2, FMA
+// and F16C (operation_impl_avx2.cpp:43-68, 122-240, 258-382): a matrix as a
+// product and two fused multiply-adds; inverse gamma from a 2^16 + 1 entry
+// table over [-0.5, 1.5]; gamma from a 2^16 entry table indexed by the
+// binary16 bits of the value.
+enum CspOpKind : int { CSP_MATRIX = 0, CSP_GAMMA = 1, CSP_B67 = 2, CSP_B67_INV = 3, CSP_CL_TO_RGB = 4, CSP_CL_TO_YUV = 5,
+                       CSP_MATRIX_FMA = 6, CSP_LUT_LINEAR = 7, CSP_LUT_GAMMA = 8 };
+
+// Entries of the lookup tables of CSP_LUT_LINEAR and CSP_LUT_GAMMA.
+constexpr unsigned kLutLinearSize = 65537, kLutGammaSize = 65536;
 
 // One in-place operation on three float planes (plain old data, copied to
 // kernel arguments as is).
@@ -22,7 +33,11 @@ struct CspOp {
 	float post;   // output scale (gamma, inverse B67, CL->RGB)
 	float kr, kg, kb;
 	float nb, pb, nr, pr;
+	const float *lut; // CSP_LUT_*: the table on the device running the kernel (set at launch)
 };
+
+// The table of a CSP_LUT_* operation, entry by entry as the reference builds it.
+std::vector<float> build_gamma_lut(const CspOp &op);
 
 struct CspProgram {
 	int n = 0;

@@ -107,6 +107,8 @@ public:
 	struct Tables {
 		void *mem = nullptr;
 		std::map<int, dev::PassArg> pass; // by node id
+		void *lut_mem = nullptr;
+		std::map<int, const float *> lut; // gamma lookup tables, by step index * 8 + operation index
 	};
 
 	// Uploads every table the program's launches read on the current HIP

@@ -62,9 +62,12 @@ DeviceProgram::DeviceProgram(const Plan &plan) : m_plan(plan)
 
 DeviceProgram::~DeviceProgram()
 {
-	for (auto &kv : m_tables)
+	for (auto &kv : m_tables) {
 		if (kv.second && kv.second->mem)
 			(void)hipFree(kv.second->mem);
+		if (kv.second && kv.second->lut_mem)
+			(void)hipFree(kv.second->lut_mem);
+	}
 	for (auto &kv : m_mfma_tables)
 		if (kv.second && kv.second->mem)
 			(void)hipFree(kv.second->mem);
@@ -779,7 +782,10 @@ ImmaPlan DeviceProgram::imma_plan(const Step &s) const
 	p.in_cols = in_cols;
 	p.in_pitch = odd_stride(in_cols);
 	p.lds = 2 * dev::kImmaTileW * p.mid_pitch + std::max(2 * 16 * mb * p.in_pitch, 2 * 4 * 16 * dev::kImmaOutPitch);
-	p.ok = p.lds <= 64 * 1024;
+	// A source row holds at least one whole 16-byte chunk (the kernel's
+	// clamped loads stay inside the row).
+	const unsigned esz = s.ix[0].type == 0 ? 1 : 2;
+	p.ok = p.lds <= 64 * 1024 && s.in_w * esz >= 16;
 	if (p.ok) {
 		// Every LDS index the kernel forms, re-derived with its formulas: the
 		// horizontal windows (2 x 4 dwords from (kb - cb) / 2 + 2 (g % 2)) lie in the
@@ -1142,6 +1148,28 @@ const DeviceProgram::Tables &DeviceProgram::tables_for_current_device() const
 			                                  reinterpret_cast<const int32_t *>(base + o_pc), static_cast<int>(pair_count(t)) };
 		}
 	}
+	// Gamma lookup tables of the approximate-gamma operations, one allocation.
+	std::vector<float> luts;
+	std::vector<std::pair<int, size_t>> lut_at;
+	for (size_t si = 0; si < m_steps.size(); ++si) {
+		const Step &st = m_steps[si];
+		if (st.dead || st.kind != Step::CSP)
+			continue;
+		for (int k = 0; k < st.prog.n; ++k) {
+			std::vector<float> l = build_gamma_lut(st.prog.op[k]);
+			if (l.empty())
+				continue;
+			lut_at.emplace_back(static_cast<int>(si) * 8 + k, luts.size());
+			luts.insert(luts.end(), l.begin(), l.end());
+			luts.resize(align_up(luts.size(), 64), 0.0f);
+		}
+	}
+	if (!luts.empty()) {
+		check_hip(hipMalloc(&tab->lut_mem, luts.size() * 4), "hipMalloc(gamma tables)");
+		check_hip(hipMemcpy(tab->lut_mem, luts.data(), luts.size() * 4, hipMemcpyHostToDevice), "hipMemcpy(gamma tables)");
+		for (auto &e : lut_at)
+			tab->lut[e.first] = static_cast<const float *>(tab->lut_mem) + e.second;
+	}
 	m_tables[device] = tab;
 	return *tab;
 }
@@ -1369,6 +1397,9 @@ void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void
 			a.width = s.width;
 			a.height = s.height;
 			a.prog = s.prog;
+			for (int k = 0; k < a.prog.n; ++k)
+				if (a.prog.op[k].kind == CSP_LUT_LINEAR || a.prog.op[k].kind == CSP_LUT_GAMMA)
+					a.prog.op[k].lut = tabs.lut.at(static_cast<int>(si) * 8 + k);
 			rc = dev::launch_colorspace(a, stream);
 			break;
 		}

@@ -110,6 +110,9 @@ struct ImmaArg {
 	unsigned lds;
 };
 constexpr unsigned kImmaMaxMb = 6;
+// 16-byte source chunks each thread prefetches for the next tile (the rest
+// of a wider window is loaded when the tile is staged).
+constexpr int kImmaStageU = 4;
 // Row pitch (samples) of a wave's LDS copy of 16 output rows of its tile;
 // the staged source area holds at least 4 such copies.
 constexpr unsigned kImmaOutPitch = kImmaTileW + 8;

@@ -1163,6 +1163,22 @@ __device__ __forceinline__ int32_t imma_pack(int32_t hh, int32_t mid, int32_t ll
 	return med3_i32(static_cast<int32_t>(s) >> 14, 0, pixel_max);
 }
 
+// The kernel is persistent: gridDim.x workgroups (a multiple of 8) walk the
+// tiles, the workgroups of XCD x (blockIdx.x % 8 == x) interleaved over one
+// contiguous range of tiles, so neighbouring tiles, which share source rows
+// and columns, are in flight together in one L2. While a tile is multiplied
+// and stored, the next tile's source chunks and coefficient records are
+// already on their way into registers, so each tile's memory round trip
+// overlaps the previous tile's work.
+struct ImmaPrefetch {
+	uint4 raw[kImmaStageU]; // source chunks of the staged window (16 bytes each)
+	uint4 hc[2];            // coefficient records of the wave's two column blocks
+	int32_t hinit[2];
+	int hkb[2];
+	int rbase, cb;          // first staged source row and column
+	unsigned bx, by, f;
+};
+
 template <int MB>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) k_resize_imma(ImmaArg a)
 {
@@ -1170,43 +1186,115 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 	const unsigned lane = threadIdx.x & 63;
 	const unsigned w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const unsigned half = (lane >> 4) & 1;
-	// Workgroups go to the 8 XCDs in turn; each XCD takes a contiguous run of
-	// tiles (row-major over the frame batch), so tiles that share source rows
-	// and columns meet in one L2.
-	const unsigned nx = gridDim.x, ny = gridDim.y, n = nx * ny * gridDim.z;
-	const unsigned lin = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
-	const unsigned xcd = lin % 8, q8 = n / 8, r8 = n % 8;
-	const unsigned t = (xcd < r8 ? xcd * (q8 + 1) : r8 * (q8 + 1) + (xcd - r8) * q8) + lin / 8;
-	const unsigned bx = t % nx, by = (t / nx) % ny, f = t / (nx * ny);
-	const unsigned j0 = bx * kImmaTileW, i0 = by * 16 * a.rb;
-	const unsigned hb0 = bx * (kImmaTileW / 16), vb0 = by * a.rb;
-	const int rbase = a.v[vb0].kb; // first intermediate row of the tile
+	const unsigned nx = (a.dst_w + kImmaTileW - 1) / kImmaTileW, ny = (a.dst_h + 16 * a.rb - 1) / (16 * a.rb);
+	const unsigned nt = nx * ny * a.frames;
+	const unsigned xcd = blockIdx.x % 8, rank = blockIdx.x / 8, g8 = gridDim.x / 8;
+	const unsigned q8 = nt / 8, r8 = nt % 8;
+	const unsigned tbeg = xcd < r8 ? xcd * (q8 + 1) : r8 * (q8 + 1) + (xcd - r8) * q8;
+	const unsigned tend = tbeg + q8 + (xcd < r8 ? 1u : 0u);
 	unsigned char *midh = smem;
 	unsigned char *midl = midh + kImmaTileW * a.mid_pitch;
+	uint16_t *xin = reinterpret_cast<uint16_t *>(smem + 2 * kImmaTileW * a.mid_pitch);
 
-	// Horizontal coefficients of the wave's two column blocks (blocks past the
-	// plane repeat the last one; their columns are not stored).
-	uint4 hc[2];
-	int32_t hinit[2];
-	int hkb[2];
+	// Staging geometry, the same for every tile: prefetched chunk u of this
+	// thread is chunk e = u kThreads + thread of the window (16 MB rows x
+	// in_cols samples); threads past the end repeat the last chunk (same
+	// value, same place), chunks past kImmaStageU kThreads are loaded when
+	// the tile is staged.
+	const unsigned esz = a.in.type == 0 ? 1u : 2u;
+	const unsigned ce = 16 / esz;
+	const unsigned cpr = a.in_cols / ce;
+	const unsigned total = 16 * MB * cpr;
+	const unsigned last_full = (a.src_w - ce) / ce * ce; // a.src_w >= ce (host-checked)
+	unsigned srow[kImmaStageU], scol[kImmaStageU];
 #pragma unroll
-	for (int q = 0; q < 2; ++q) {
-		const ImmaBlock &b = a.h[min(hb0 + 2 * w + q, a.nh - 1)];
-		hc[q] = *reinterpret_cast<const uint4 *>(b.c[lane & 31]);
-		hinit[q] = b.init[lane & 15];
-		hkb[q] = b.kb;
+	for (int u = 0; u < kImmaStageU; ++u) {
+		const unsigned e = min(u * kThreads + threadIdx.x, total - 1);
+		srow[u] = e / cpr;
+		scol[u] = (e - srow[u] * cpr) * ce;
 	}
-	// Vertical coefficients of the wave's row blocks w and w + 4.
-	uint4 vc[2];
-	int32_t vinit[2];
-	int vkb[2];
+
+	// Blocks past the plane repeat the last one; their outputs are not stored.
+	auto load_h = [&](ImmaPrefetch &p) {
+		const unsigned hb0 = p.bx * (kImmaTileW / 16);
 #pragma unroll
-	for (int k = 0; k < 2; ++k) {
-		const ImmaBlock &b = a.v[min(vb0 + w + 4 * k, a.nv - 1)];
-		vc[k] = *reinterpret_cast<const uint4 *>(b.c[lane & 31]);
-		vinit[k] = b.init[lane & 15];
-		vkb[k] = b.kb;
-	}
+		for (int q = 0; q < 2; ++q) {
+			const ImmaBlock &b = a.h[min(hb0 + 2 * w + q, a.nh - 1)];
+			p.hc[q] = *reinterpret_cast<const uint4 *>(b.c[lane & 31]);
+			p.hinit[q] = b.init[lane & 15];
+			p.hkb[q] = b.kb;
+		}
+	};
+	// The next tile's window goes into registers right after the current
+	// tile is staged; its horizontal records once the current horizontal
+	// pass is done; a tile's vertical records are loaded as the tile starts.
+	auto prefetch = [&](unsigned t, ImmaPrefetch &p) {
+		p.bx = t % nx;
+		p.by = (t / nx) % ny;
+		p.f = t / (nx * ny);
+		const unsigned hb0 = p.bx * (kImmaTileW / 16), vb0 = p.by * a.rb;
+		p.rbase = a.v[vb0].kb;
+		p.cb = a.h[hb0].kb & ~15;
+		const char *src = frame_ptr(a.src, p.f);
+#pragma unroll
+		for (int u = 0; u < kImmaStageU; ++u) {
+			// clamped into the plane; chunks that are not whole are rebuilt when stored
+			const unsigned r = min(static_cast<unsigned>(p.rbase) + srow[u], a.src_h - 1);
+			const unsigned c = min(static_cast<unsigned>(p.cb) + scol[u], last_full);
+			p.raw[u] = *reinterpret_cast<const uint4 *>(row_ptr(src, a.src.stride, r) + static_cast<size_t>(c) * esz);
+		}
+	};
+	// One chunk of the window as biased samples in LDS; rows and columns past
+	// the plane are zero samples (they meet only zero taps), so a chunk that
+	// is not whole inside the plane is rebuilt sample by sample.
+	auto store_chunk = [&](const ImmaPrefetch &p, const char *src, unsigned sr, unsigned sc, uint4 v) {
+		const unsigned r = static_cast<unsigned>(p.rbase) + sr, c = static_cast<unsigned>(p.cb) + sc;
+		if (r >= a.src_h || c + ce > a.src_w) {
+			uint32_t q[4] = { 0, 0, 0, 0 };
+			if (r < a.src_h) {
+				const char *row = row_ptr(src, a.src.stride, r);
+#pragma unroll
+				for (unsigned k = 0; k < 16; ++k)
+					if (k < ce && c + k < a.src_w) {
+						const uint32_t x = load_as_uint(row, c + k, a.in.type);
+						if (esz == 2)
+							q[(k >> 1) & 3] |= x << (16 * (k & 1));
+						else
+							q[k >> 2] |= x << (8 * (k & 3));
+					}
+			}
+			v = make_uint4(q[0], q[1], q[2], q[3]);
+		}
+		const unsigned sh = a.in.op == 1 ? a.in.shift : 0;
+		uint16_t *d = xin + static_cast<size_t>(sr) * a.in_pitch + sc;
+		const uint32_t wv[4] = { v.x, v.y, v.z, v.w };
+		if (esz == 2) {
+			*reinterpret_cast<uint4 *>(d) = make_uint4(shift_bias2(wv[0], sh), shift_bias2(wv[1], sh), shift_bias2(wv[2], sh), shift_bias2(wv[3], sh));
+		} else {
+			uint32_t o[8];
+#pragma unroll
+			for (int k = 0; k < 4; ++k) {
+				o[2 * k] = shift_bias2((wv[k] & 0xFFu) | ((wv[k] & 0xFF00u) << 8), sh);
+				o[2 * k + 1] = shift_bias2(((wv[k] >> 16) & 0xFFu) | ((wv[k] >> 8) & 0xFF0000u), sh);
+			}
+			reinterpret_cast<uint4 *>(d)[0] = make_uint4(o[0], o[1], o[2], o[3]);
+			reinterpret_cast<uint4 *>(d)[1] = make_uint4(o[4], o[5], o[6], o[7]);
+		}
+	};
+	// The prefetched chunks, then (wide windows only) the rest of the window
+	// loaded here.
+	auto stage = [&](const ImmaPrefetch &p) {
+		const char *src = frame_ptr(a.src, p.f);
+#pragma unroll
+		for (int u = 0; u < kImmaStageU; ++u)
+			store_chunk(p, src, srow[u], scol[u], p.raw[u]);
+		for (unsigned e = kImmaStageU * kThreads + threadIdx.x; e < total; e += kThreads) {
+			const unsigned sr = e / cpr, sc = (e - sr * cpr) * ce;
+			const unsigned r = min(static_cast<unsigned>(p.rbase) + sr, a.src_h - 1);
+			const unsigned c = min(static_cast<unsigned>(p.cb) + sc, last_full);
+			store_chunk(p, src, sr, sc, *reinterpret_cast<const uint4 *>(row_ptr(src, a.src.stride, r) + static_cast<size_t>(c) * esz));
+		}
+	};
 	// The three B operands from a record's (ch, cl): [ch | 0], [cl | ch], [0 | cl].
 	const bool lo_half = lane < 32;
 	auto operands = [&](const uint4 &c, long &Bh, long &Bm, long &Bl) {
@@ -1215,114 +1303,133 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 		Bm = static_cast<long>(lo_half ? cl : ch);
 		Bl = static_cast<long>(lo_half ? 0 : cl);
 	};
-	// Stage the source window (16 MB rows from the first vertical window, the
-	// columns of the tile's 8 horizontal windows) as biased samples with
-	// 16-byte loads of whole row segments; rows and columns past the plane are
-	// stored as zero samples (they meet only zero taps).
-	const char *src = frame_ptr(a.src, f);
-	const int cb = a.h[hb0].kb & ~15; // first staged source column
-	uint16_t *xin = reinterpret_cast<uint16_t *>(smem + 2 * kImmaTileW * a.mid_pitch);
-	stage_tile<true, 4>(xin, a.in_pitch, src, a.src.stride, static_cast<unsigned>(rbase), 16 * MB, a.src_h,
-	                    static_cast<unsigned>(cb), a.in_cols, a.src_w, a.in);
-	__syncthreads();
 
-	// Horizontal pass.
-	{
-		const uint32_t sel = lane < 32 ? 0x07050301u : 0x06040200u; // high or low bytes of 4 samples
-		const uint32_t lx = lane < 32 ? 0u : 0x80808080u;           // low byte - 128 = low byte ^ 0x80
-		const uint32_t *xw = reinterpret_cast<const uint32_t *>(xin);
-		const unsigned pw = a.in_pitch / 2;
-#pragma unroll
-		for (int m = 0; m < MB; ++m) {
-			const uint32_t *row = xw + (16 * m + (lane & 15)) * pw;
-#pragma unroll
-			for (int q = 0; q < 2; ++q) {
-				// lane group g holds window samples 4 (g % 2) + 0..3 and + 8..11:
-				// the two groups of a half-wave read different banks of a row
-				const uint32_t *x = row + (hkb[q] - cb) / 2 + 2 * half;
-				const uint32_t d0 = x[0], d1 = x[1], d4 = x[4], d5 = x[5];
-				const uint32_t ax = __builtin_amdgcn_perm(d1, d0, sel) ^ lx, ay = __builtin_amdgcn_perm(d5, d4, sel) ^ lx;
-				const long A = static_cast<long>((static_cast<uint64_t>(ay) << 32) | ax);
-				long Bh, Bm, Bl;
-				operands(hc[q], Bh, Bm, Bl);
-				const imma_v4i zero = { 0, 0, 0, 0 };
-				const imma_v4i init = { hinit[q], hinit[q], hinit[q], hinit[q] };
-				const imma_v4i hh = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bh, zero, 0, 0, 0);
-				const imma_v4i md = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bm, zero, 0, 0, 0);
-				const imma_v4i ll = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bl, init, 0, 0, 0);
-				const uint32_t y0 = imma_pack(hh[0], md[0], ll[0], a.pixel_max), y1 = imma_pack(hh[1], md[1], ll[1], a.pixel_max);
-				const uint32_t y2 = imma_pack(hh[2], md[2], ll[2], a.pixel_max), y3 = imma_pack(hh[3], md[3], ll[3], a.pixel_max);
-				const uint32_t p01 = __builtin_amdgcn_perm(y1, y0, 0x05010400u); // y0.b0 y1.b0 y0.b1 y1.b1
-				const uint32_t p23 = __builtin_amdgcn_perm(y3, y2, 0x05010400u);
-				const uint32_t lo = __builtin_amdgcn_perm(p23, p01, 0x05040100u) ^ 0x80808080u;
-				const uint32_t hi = __builtin_amdgcn_perm(p23, p01, 0x07060302u) ^ 0x80808080u;
-				const unsigned off = (16 * (2 * w + q) + (lane & 15)) * a.mid_pitch + 16 * m + 4 * (lane >> 4);
-				*reinterpret_cast<uint32_t *>(midh + off) = hi;
-				*reinterpret_cast<uint32_t *>(midl + off) = lo;
-			}
-		}
+	ImmaPrefetch nxt;
+	unsigned t = tbeg + rank;
+	if (t < tend) {
+		prefetch(t, nxt);
+		load_h(nxt);
 	}
-	__syncthreads();
-
-	// Vertical pass. The staged source is no longer read: each wave copies its
-	// output rows there before storing them.
-	char *dst = frame_ptr_w(a.dst, f);
-	uint16_t *ot = reinterpret_cast<uint16_t *>(xin) + w * 16 * kImmaOutPitch;
-	const unsigned char *plane = (lane < 32 ? midh : midl) + (lane & 15) * a.mid_pitch;
+	for (; t < tend; t += g8) {
+		const ImmaPrefetch cur = nxt;
+		uint4 vc[2];
+		int32_t vinit[2];
+		int vkb[2];
 #pragma unroll
-	for (int k = 0; k < 2; ++k) {
-		const unsigned vb = w + 4 * k;
-		if (vb >= a.rb)
-			break;
-		const unsigned o = static_cast<unsigned>(vkb[k] - rbase) + 8 * half;
-		const unsigned sh = o & 3;
-		long Bh, Bm, Bl;
-		operands(vc[k], Bh, Bm, Bl);
-		const imma_v4i zero = { 0, 0, 0, 0 };
-		const imma_v4i init = { vinit[k], vinit[k], vinit[k], vinit[k] };
-		// Column block c + 1 is multiplied before block c is packed into the
-		// wave's LDS copy of its 16 output rows.
-		imma_v4i acc[2][3];
-		auto mul = [&](unsigned c, imma_v4i *r) {
-			const uint32_t *p = reinterpret_cast<const uint32_t *>(plane + 16 * c * a.mid_pitch + (o & ~3u));
-			const uint32_t d0 = p[0], d1 = p[1], d2 = p[2];
-			const uint32_t ax = __builtin_amdgcn_alignbyte(d1, d0, sh), ay = __builtin_amdgcn_alignbyte(d2, d1, sh);
-			const long A = static_cast<long>((static_cast<uint64_t>(ay) << 32) | ax);
-			r[0] = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bh, zero, 0, 0, 0);
-			r[1] = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bm, zero, 0, 0, 0);
-			r[2] = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bl, init, 0, 0, 0);
-		};
-		mul(0, acc[0]);
-#pragma unroll
-		for (unsigned c = 0; c < kImmaTileW / 16; ++c) {
-			if (c + 1 < kImmaTileW / 16)
-				mul(c + 1, acc[(c + 1) & 1]);
-			const imma_v4i hh = acc[c & 1][0], md = acc[c & 1][1], ll = acc[c & 1][2];
-			const uint32_t y0 = imma_pack(hh[0], md[0], ll[0], a.pixel_max), y1 = imma_pack(hh[1], md[1], ll[1], a.pixel_max);
-			const uint32_t y2 = imma_pack(hh[2], md[2], ll[2], a.pixel_max), y3 = imma_pack(hh[3], md[3], ll[3], a.pixel_max);
-			*reinterpret_cast<uint2 *>(ot + (lane & 15) * kImmaOutPitch + 16 * c + 4 * (lane >> 4)) = make_uint2(pack16x2(y0, y1), pack16x2(y2, y3));
+		for (int k = 0; k < 2; ++k) {
+			const ImmaBlock &b = a.v[min(cur.by * a.rb + w + 4 * k, a.nv - 1)];
+			vc[k] = *reinterpret_cast<const uint4 *>(b.c[lane & 31]);
+			vinit[k] = b.init[lane & 15];
+			vkb[k] = b.kb;
 		}
-		__builtin_amdgcn_wave_barrier();
-		// Whole row segments out: 16 lanes per row, 16 bytes each.
+		stage(cur);
+		__syncthreads();
+		const bool more = t + g8 < tend;
+		if (more)
+			prefetch(t + g8, nxt);
+		const unsigned j0 = cur.bx * kImmaTileW, i0 = cur.by * 16 * a.rb;
+
+		// Horizontal pass.
+		{
+			const uint32_t sel = lane < 32 ? 0x07050301u : 0x06040200u; // high or low bytes of 4 samples
+			const uint32_t lx = lane < 32 ? 0u : 0x80808080u;           // low byte - 128 = low byte ^ 0x80
+			const uint32_t *xw = reinterpret_cast<const uint32_t *>(xin);
+			const unsigned pw = a.in_pitch / 2;
 #pragma unroll
-		for (unsigned u = 0; u < 4; ++u) {
-			const unsigned e = 64 * u + lane, r = e / 16, chunk = e % 16;
-			const unsigned row = i0 + 16 * vb + r, col = j0 + 8 * chunk;
-			const uint4 v = *reinterpret_cast<const uint4 *>(ot + r * kImmaOutPitch + 8 * chunk);
-			if (row < a.dst_h) {
-				uint16_t *d = reinterpret_cast<uint16_t *>(row_ptr(dst, a.dst.stride, row)) + col;
-				if (col + 8 <= a.dst_w) {
-					*reinterpret_cast<uint4 *>(d) = v;
-				} else {
-					const uint32_t vv[4] = { v.x, v.y, v.z, v.w };
+			for (int m = 0; m < MB; ++m) {
+				const uint32_t *row = xw + (16 * m + (lane & 15)) * pw;
 #pragma unroll
-					for (unsigned i = 0; i < 7; ++i)
-						if (col + i < a.dst_w)
-							d[i] = static_cast<uint16_t>(vv[i / 2] >> (16 * (i & 1)));
+				for (int q = 0; q < 2; ++q) {
+					// lane group g holds window samples 4 (g % 2) + 0..3 and + 8..11:
+					// the two groups of a half-wave read different banks of a row
+					const uint32_t *x = row + (cur.hkb[q] - cur.cb) / 2 + 2 * half;
+					const uint32_t d0 = x[0], d1 = x[1], d4 = x[4], d5 = x[5];
+					const uint32_t ax = __builtin_amdgcn_perm(d1, d0, sel) ^ lx, ay = __builtin_amdgcn_perm(d5, d4, sel) ^ lx;
+					const long A = static_cast<long>((static_cast<uint64_t>(ay) << 32) | ax);
+					long Bh, Bm, Bl;
+					operands(cur.hc[q], Bh, Bm, Bl);
+					const imma_v4i zero = { 0, 0, 0, 0 };
+					const imma_v4i init = { cur.hinit[q], cur.hinit[q], cur.hinit[q], cur.hinit[q] };
+					const imma_v4i hh = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bh, zero, 0, 0, 0);
+					const imma_v4i md = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bm, zero, 0, 0, 0);
+					const imma_v4i ll = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bl, init, 0, 0, 0);
+					const uint32_t y0 = imma_pack(hh[0], md[0], ll[0], a.pixel_max), y1 = imma_pack(hh[1], md[1], ll[1], a.pixel_max);
+					const uint32_t y2 = imma_pack(hh[2], md[2], ll[2], a.pixel_max), y3 = imma_pack(hh[3], md[3], ll[3], a.pixel_max);
+					const uint32_t p01 = __builtin_amdgcn_perm(y1, y0, 0x05010400u); // y0.b0 y1.b0 y0.b1 y1.b1
+					const uint32_t p23 = __builtin_amdgcn_perm(y3, y2, 0x05010400u);
+					const uint32_t lo = __builtin_amdgcn_perm(p23, p01, 0x05040100u) ^ 0x80808080u;
+					const uint32_t hi = __builtin_amdgcn_perm(p23, p01, 0x07060302u) ^ 0x80808080u;
+					const unsigned off = (16 * (2 * w + q) + (lane & 15)) * a.mid_pitch + 16 * m + 4 * (lane >> 4);
+					*reinterpret_cast<uint32_t *>(midh + off) = hi;
+					*reinterpret_cast<uint32_t *>(midl + off) = lo;
 				}
 			}
 		}
-		__builtin_amdgcn_wave_barrier();
+		__syncthreads();
+		if (more)
+			load_h(nxt);
+
+		// Vertical pass. The staged source is no longer read: each wave copies
+		// its output rows there before storing them.
+		char *dst = frame_ptr_w(a.dst, cur.f);
+		uint16_t *ot = xin + w * 16 * kImmaOutPitch;
+		const unsigned char *plane = (lane < 32 ? midh : midl) + (lane & 15) * a.mid_pitch;
+#pragma unroll
+		for (int k = 0; k < 2; ++k) {
+			const unsigned vb = w + 4 * k;
+			if (vb >= a.rb)
+				break;
+			const unsigned o = static_cast<unsigned>(vkb[k] - cur.rbase) + 8 * half;
+			const unsigned sh = o & 3;
+			long Bh, Bm, Bl;
+			operands(vc[k], Bh, Bm, Bl);
+			const imma_v4i zero = { 0, 0, 0, 0 };
+			const imma_v4i init = { vinit[k], vinit[k], vinit[k], vinit[k] };
+			// Column block c + 1 is multiplied before block c is packed into the
+			// wave's LDS copy of its 16 output rows.
+			imma_v4i acc[2][3];
+			auto mul = [&](unsigned c, imma_v4i *r) {
+				const uint32_t *p = reinterpret_cast<const uint32_t *>(plane + 16 * c * a.mid_pitch + (o & ~3u));
+				const uint32_t d0 = p[0], d1 = p[1], d2 = p[2];
+				const uint32_t ax = __builtin_amdgcn_alignbyte(d1, d0, sh), ay = __builtin_amdgcn_alignbyte(d2, d1, sh);
+				const long A = static_cast<long>((static_cast<uint64_t>(ay) << 32) | ax);
+				r[0] = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bh, zero, 0, 0, 0);
+				r[1] = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bm, zero, 0, 0, 0);
+				r[2] = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bl, init, 0, 0, 0);
+			};
+			mul(0, acc[0]);
+#pragma unroll
+			for (unsigned c = 0; c < kImmaTileW / 16; ++c) {
+				if (c + 1 < kImmaTileW / 16)
+					mul(c + 1, acc[(c + 1) & 1]);
+				const imma_v4i hh = acc[c & 1][0], md = acc[c & 1][1], ll = acc[c & 1][2];
+				const uint32_t y0 = imma_pack(hh[0], md[0], ll[0], a.pixel_max), y1 = imma_pack(hh[1], md[1], ll[1], a.pixel_max);
+				const uint32_t y2 = imma_pack(hh[2], md[2], ll[2], a.pixel_max), y3 = imma_pack(hh[3], md[3], ll[3], a.pixel_max);
+				*reinterpret_cast<uint2 *>(ot + (lane & 15) * kImmaOutPitch + 16 * c + 4 * (lane >> 4)) = make_uint2(pack16x2(y0, y1), pack16x2(y2, y3));
+			}
+			__builtin_amdgcn_wave_barrier();
+			// Whole row segments out: 16 lanes per row, 16 bytes each.
+#pragma unroll
+			for (unsigned u = 0; u < 4; ++u) {
+				const unsigned e = 64 * u + lane, r = e / 16, chunk = e % 16;
+				const unsigned row = i0 + 16 * vb + r, col = j0 + 8 * chunk;
+				const uint4 v = *reinterpret_cast<const uint4 *>(ot + r * kImmaOutPitch + 8 * chunk);
+				if (row < a.dst_h) {
+					uint16_t *d = reinterpret_cast<uint16_t *>(row_ptr(dst, a.dst.stride, row)) + col;
+					if (col + 8 <= a.dst_w) {
+						*reinterpret_cast<uint4 *>(d) = v;
+					} else {
+						const uint32_t vv[4] = { v.x, v.y, v.z, v.w };
+#pragma unroll
+						for (unsigned i = 0; i < 7; ++i)
+							if (col + i < a.dst_w)
+								d[i] = static_cast<uint16_t>(vv[i / 2] >> (16 * (i & 1)));
+					}
+				}
+			}
+			__builtin_amdgcn_wave_barrier();
+		}
+		__syncthreads();
 	}
 }
 
@@ -1403,6 +1510,39 @@ template <int NP>
 __device__ __forceinline__ void apply_op(const CspOp &op, float *a, float *b, float *c)
 {
 	switch (op.kind) {
+	case CSP_MATRIX_FMA:
+#pragma unroll
+		for (int p = 0; p < NP; ++p) {
+			const float x = __builtin_fmaf(op.m[2], c[p], __builtin_fmaf(op.m[1], b[p], op.m[0] * a[p]));
+			const float y = __builtin_fmaf(op.m[5], c[p], __builtin_fmaf(op.m[4], b[p], op.m[3] * a[p]));
+			const float z = __builtin_fmaf(op.m[8], c[p], __builtin_fmaf(op.m[7], b[p], op.m[6] * a[p]));
+			a[p] = x; b[p] = y; c[p] = z;
+		}
+		break;
+	case CSP_LUT_LINEAR: {
+		// index = cvtps2dq(fma(x, 2^15, 2^14)): round to nearest even, NaN and
+		// values outside int32 to INT32_MIN; clamped to the table
+		auto idx = [](float x) {
+			const float v = __builtin_fmaf(x, 32768.0f, 16384.0f);
+			const int i = (v >= -2147483648.0f && v < 2147483648.0f) ? static_cast<int>(__builtin_rintf(v)) : INT_MIN;
+			return static_cast<unsigned>(min(max(i, 0), static_cast<int>(kLutLinearSize) - 1));
+		};
+#pragma unroll
+		for (int p = 0; p < NP; ++p) {
+			a[p] = op.lut[idx(a[p])];
+			b[p] = op.lut[idx(b[p])];
+			c[p] = op.lut[idx(c[p])];
+		}
+		break;
+	}
+	case CSP_LUT_GAMMA:
+#pragma unroll
+		for (int p = 0; p < NP; ++p) {
+			a[p] = op.lut[float_to_half(a[p])];
+			b[p] = op.lut[float_to_half(b[p])];
+			c[p] = op.lut[float_to_half(c[p])];
+		}
+		break;
 	case CSP_MATRIX:
 #pragma unroll
 		for (int p = 0; p < NP; ++p) {
@@ -1777,12 +1917,20 @@ int launch_resize_imma(const ImmaArg &a, void *stream)
 {
 	if (!a.frames || !a.dst_w || !a.dst_h)
 		return 0;
-	dim3 grid((a.dst_w + kImmaTileW - 1) / kImmaTileW, (a.dst_h + 16 * a.rb - 1) / (16 * a.rb), a.frames);
+	const size_t tiles = static_cast<size_t>((a.dst_w + kImmaTileW - 1) / kImmaTileW) * ((a.dst_h + 16 * a.rb - 1) / (16 * a.rb)) * a.frames;
+	// Persistent grid: as many workgroups as the LDS lets every CU hold at
+	// once (a multiple of 8, one share per XCD), or fewer for small batches.
+	int device = 0, cus = 0;
+	if (hipGetDevice(&device) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+		return static_cast<int>(hipErrorInvalidDevice);
+	const size_t per_cu = std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / std::max<unsigned>(a.lds, 1)));
+	size_t grid = std::min<size_t>(static_cast<size_t>(cus) * per_cu, (tiles + 7) / 8 * 8);
+	grid = std::max<size_t>(8, grid / 8 * 8);
 	hipStream_t s = static_cast<hipStream_t>(stream);
 	switch (a.mb) {
 #define ZAMD_IMMA(n) \
 	case n: \
-		hipLaunchKernelGGL((k_resize_imma<n>), grid, dim3(kThreads), a.lds, s, a); \
+		hipLaunchKernelGGL((k_resize_imma<n>), dim3(static_cast<unsigned>(grid)), dim3(kThreads), a.lds, s, a); \
 		break;
 	ZAMD_IMMA(1) ZAMD_IMMA(2) ZAMD_IMMA(3) ZAMD_IMMA(4) ZAMD_IMMA(5) ZAMD_IMMA(6)
 #undef ZAMD_IMMA

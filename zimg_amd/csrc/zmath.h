@@ -42,6 +42,21 @@ ZHD float bitsf(uint32_t u) { float x; std::memcpy(&x, &u, 4); return x; }
 ZHD uint64_t dbits(double x) { uint64_t u; std::memcpy(&u, &x, 8); return u; }
 ZHD double bitsd(uint64_t u) { double x; std::memcpy(&x, &u, 8); return x; }
 
+// IEEE binary16 bits to binary32, exactly (a NaN is made quiet and keeps its
+// payload), as quantize.cpp:24-77 widens a half.
+ZHD float half_to_float(uint16_t h)
+{
+	const uint32_t sign = static_cast<uint32_t>(h & 0x8000u) << 16;
+	const uint32_t e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+	if (e == 0x1F)
+		return bitsf(sign | 0x7F800000u | (m ? 0x00400000u | (m << 13) : 0u));
+	if (e == 0) {
+		const float f = static_cast<float>(m) * 0x1p-24f; // exact: m < 2^10
+		return sign ? -f : f;
+	}
+	return bitsf(sign | ((e + 112u) << 23) | (m << 13));
+}
+
 // 2^(i/32) (double bits) minus i<<47.
 ZCONST static const uint64_t kExp2Tab[32] = {
 	0x3ff0000000000000ULL, 0x3fefd9b0d3158574ULL, 0x3fefb5586cf9890fULL, 0x3fef9301d0125b51ULL,
