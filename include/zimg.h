@@ -440,6 +440,41 @@ zimg_error_code_e zimg_filter_graph_process_device_batch_noalloc(const zimg_filt
                                                                  const zimg_image_buffer *dst, const ptrdiff_t dst_frame_stride[4],
                                                                  void *stream);
 
+/* device extension: packed layouts in HBM, the device counterpart of the
+ * reference's pack / unpack callbacks (zimg.h:455-484, the YUY2 and planar
+ * layouts of src/testapp/frame.cpp:30-100). A buffer is planar
+ * (ZIMG_PACKING_PLANAR: data[p] as zimg_image_buffer planes), semi-planar
+ * 4:2:0 (ZIMG_PACKING_NV12: data[0] luma, data[1] U and V samples
+ * alternating; BYTE is NV12, WORD with shift 6 is P010, shift 0 is P016) or
+ * packed 4:2:2 BYTE in data[0] (ZIMG_PACKING_YUY2: Y0 U Y1 V per pixel pair;
+ * ZIMG_PACKING_UYVY: U Y0 V Y1). WORD samples sit in each 16-bit word
+ * shifted left by `shift` bits (unpacking shifts right, dropping the low
+ * bits). The graph's format must match the layout (YUV, 4:2:0 for NV12, 4:2:2
+ * BYTE for YUY2 / UYVY, no alpha). Unpacking and packing run as launches
+ * around the graph's own, through tmp (zimg_filter_graph_get_device_packed_tmp_size
+ * bytes; a luma plane stored contiguously with shift 0 is used in place). */
+#define ZIMG_PACKING_PLANAR 0
+#define ZIMG_PACKING_NV12 1
+#define ZIMG_PACKING_YUY2 2
+#define ZIMG_PACKING_UYVY 3
+
+typedef struct zimg_device_packed_buffer {
+	unsigned version;         /* ZIMG_API_VERSION */
+	int packing;              /* ZIMG_PACKING_* */
+	unsigned shift;           /* WORD samples: bits below the value in each 16-bit word */
+	void *data[4];            /* device pointers */
+	ptrdiff_t stride[4];      /* bytes per row */
+	ptrdiff_t frame_stride[4];/* bytes between frames of a batch */
+} zimg_device_packed_buffer;
+
+ZIMG_VISIBILITY
+zimg_error_code_e zimg_filter_graph_get_device_packed_tmp_size(const zimg_filter_graph *ptr, unsigned count, int src_packing,
+                                                               int dst_packing, size_t *out);
+
+ZIMG_VISIBILITY
+zimg_error_code_e zimg_filter_graph_process_device_packed(const zimg_filter_graph *ptr, unsigned count, const zimg_device_packed_buffer *src,
+                                                          const zimg_device_packed_buffer *dst, void *tmp, void *stream);
+
 /* device extension: zimg_filter_graph_process_device_batch with HIP events
  * recorded around each device stage on `stream`; waits for the stream, then
  * writes up to max_stages stage durations (ms) and algorithmic bytes per

@@ -8,7 +8,8 @@
 // common/x86/cpuinfo_x86.cpp, colorspace/{gamma, colorspace_param,matrix3,
 // operation,operation_impl,colorspace_graph}.cpp, colorspace/x86/{operation_impl_x86,
 // operation_impl_avx2,operation_impl_avx512,gamma_constants_avx512}.cpp,
-// depth/{quantize,blue}.cpp and the vendored musl libm of test/extra/musl-libm.
+// depth/{quantize,blue}.cpp, unresize/bilinear.cpp and the vendored musl libm
+// of test/extra/musl-libm.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -27,6 +28,7 @@
 #include "depth/blue.h"
 #include "depth/quantize.h"
 #include "resize/filter.h"
+#include "unresize/bilinear.h"
 #include "musl-libm/mymath.h"
 
 namespace {
@@ -181,6 +183,30 @@ static int colorspace_apply(int m_in, int t_in, int p_in, int m_out, int t_out, 
 		for (auto &op : ops)
 			op->process(planes, planes, 0, static_cast<unsigned>(n));
 		return static_cast<int>(ops.size());
+	} catch (...) {
+		return -1;
+	}
+}
+
+// create_bilinear_context (unresize/bilinear.cpp:112-172) of unresizing
+// up_dim samples to orig_dim: with coef == NULL only *row_size is written.
+int ref_unresize_context(unsigned up_dim, unsigned orig_dim, double shift, unsigned *row_size, float *coef, unsigned *offset,
+                         float *lu_c, float *lu_l, float *lu_u)
+{
+	try {
+		zimg::unresize::BilinearContext ctx = zimg::unresize::create_bilinear_context(orig_dim, up_dim, shift);
+		*row_size = ctx.matrix_row_size;
+		if (coef) {
+			for (unsigned i = 0; i < ctx.output_width; ++i) {
+				for (unsigned k = 0; k < ctx.matrix_row_size; ++k)
+					coef[static_cast<size_t>(i) * ctx.matrix_row_size + k] = ctx.matrix_coefficients[static_cast<size_t>(i) * ctx.matrix_row_stride + k];
+				offset[i] = ctx.matrix_row_offsets[i];
+				lu_c[i] = ctx.lu_c[i];
+				lu_l[i] = ctx.lu_l[i];
+				lu_u[i] = ctx.lu_u[i];
+			}
+		}
+		return 0;
 	} catch (...) {
 		return -1;
 	}

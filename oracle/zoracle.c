@@ -1820,3 +1820,137 @@ void orc_colorspace_apply(const orc_csp_ops *ops, float *p0, float *p1, float *p
 		}
 	}
 }
+
+/* ======================================================================
+ * Unresize — unresize/bilinear.cpp:31-172 (context), unresize_impl.cpp:27-85
+ * (line kernels). The least-squares inverse of a bilinear upscale.
+ * ====================================================================== */
+
+/* Entry j of row i of the bilinear matrix A (out x in): rows hold one or two
+ * consecutive stored entries from *left (bilinear.cpp:64-108). */
+typedef struct { unsigned left, n; double v[2]; } orc_brow;
+
+static double brow_at(const orc_brow *r, size_t j)
+{
+	return (j < r->left || j >= r->left + r->n) ? 0.0 : r->v[j - r->left];
+}
+
+int orc_unresize_context(unsigned up_dim, unsigned orig_dim, double shift, unsigned *row_size,
+                         float *coef, unsigned *offset, float *lu_c, float *lu_l, float *lu_u)
+{
+	const unsigned in = orig_dim, out = up_dim;
+	if (in > out || !in)
+		return -1;
+	const double sh = -shift * in / out;
+	orc_brow *a = calloc(out, sizeof(orc_brow));
+	size_t *tl = calloc(in, sizeof(size_t)), *tr = calloc(in, sizeof(size_t));
+	double *c = calloc(in, sizeof(double)), *l = calloc(in, sizeof(double)), *u = calloc(in, sizeof(double));
+	double leftmost = 0.5 + sh, rightmost = in - 0.5 + sh;
+	double lf = floor(leftmost), rf = floor(rightmost);
+	unsigned li = (unsigned)(lf < 0 ? 0 : lf > in - 1.0 ? in - 1.0 : lf);
+	unsigned ri = (unsigned)(rf < 0 ? 0 : rf > in - 1.0 ? in - 1.0 : rf);
+	for (unsigned i = 0; i < out; ++i) {
+		double position = (i + 0.5) * in / out;
+		if (position <= leftmost) {
+			a[i].left = li; a[i].n = 1; a[i].v[0] = 1.0;
+		} else if (position >= rightmost) {
+			a[i].left = ri; a[i].n = 1; a[i].v[0] = 1.0;
+		} else {
+			unsigned left_idx = (unsigned)floor(position - leftmost);
+			double distance = position - left_idx - leftmost;
+			a[i].left = left_idx; a[i].n = 2;
+			a[i].v[0] = 1.0 - distance; a[i].v[1] = distance;
+		}
+	}
+	/* rows of A' (transpose) trimmed to their nonzero ends (matrix.cpp:154-200) */
+	for (unsigned col = 0; col < in; ++col) {
+		size_t first = out, last = 0;
+		for (unsigned i = 0; i < out; ++i)
+			if (brow_at(&a[i], col) != 0) { if (first == out) first = i; last = i; }
+		if (first < out) { tl[col] = first; tr[col] = last + 1; } else { tl[col] = tr[col] = 0; }
+	}
+	/* tridiagonal of A'A, each entry summed over the A' row's range in order (matrix.cpp:202-219) */
+#define ORC_PROD(i, j) ({ double acc_ = 0; for (size_t k_ = tl[i]; k_ < tr[i]; ++k_) acc_ += brow_at(&a[k_], (i)) * brow_at(&a[k_], (j)); acc_; })
+	const double eps = DBL_EPSILON;
+	if (in == 1) {
+		l[0] = ORC_PROD(0, 0);
+	} else {
+		c[0] = 0;
+		l[0] = ORC_PROD(0, 0);
+		u[0] = ORC_PROD(0, 1) / (l[0] + eps);
+		for (size_t i = 1; i + 1 < in; ++i) {
+			c[i] = ORC_PROD(i, i - 1);
+			l[i] = ORC_PROD(i, i) - c[i] * u[i - 1];
+			u[i] = ORC_PROD(i, i + 1) / (l[i] + eps);
+		}
+		c[in - 1] = ORC_PROD(in - 1, in - 2);
+		l[in - 1] = ORC_PROD(in - 1, in - 1) - c[in - 1] * u[in - 2];
+		u[in - 1] = 0;
+	}
+#undef ORC_PROD
+	size_t rs = 0;
+	for (unsigned i = 0; i < in; ++i)
+		if (tr[i] - tl[i] > rs) rs = tr[i] - tl[i];
+	*row_size = (unsigned)rs;
+	if (coef) {
+		for (unsigned i = 0; i < in; ++i) {
+			size_t left = tl[i] < out - rs ? tl[i] : out - rs;
+			for (size_t j = 0; j < rs; ++j)
+				coef[i * rs + j] = j < tr[i] - left ? (float)brow_at(&a[left + j], i) : 0.0f;
+			offset[i] = (unsigned)left;
+			lu_c[i] = (float)c[i];
+			lu_l[i] = (float)(1.0 / (l[i] + eps));
+			lu_u[i] = (float)u[i];
+		}
+	}
+	free(a); free(tl); free(tr); free(c); free(l); free(u);
+	return 0;
+}
+
+void orc_unresize_h_f32(const float *coef, const unsigned *offset, unsigned row_size, const float *lu_c, const float *lu_l,
+                        const float *lu_u, unsigned out_w, const float *src, ptrdiff_t src_stride, float *dst,
+                        ptrdiff_t dst_stride, unsigned height)
+{
+	/* unresize_impl.cpp:27-56 */
+	for (unsigned r = 0; r < height; ++r) {
+		const float *s = CROWP(float, src, src_stride, r);
+		float *d = ROWP(float, dst, dst_stride, r);
+		float z = 0.0f, w = 0.0f;
+		for (unsigned j = 0; j < out_w; ++j) {
+			float accum = 0.0f;
+			for (unsigned k = 0; k < row_size; ++k)
+				accum += coef[(size_t)j * row_size + k] * s[offset[j] + k];
+			z = (accum - lu_c[j] * z) * lu_l[j];
+			d[j] = z;
+		}
+		for (unsigned j = out_w; j != 0; --j) {
+			w = d[j - 1] - lu_u[j - 1] * w;
+			d[j - 1] = w;
+		}
+	}
+}
+
+void orc_unresize_v_f32(const float *coef, const unsigned *offset, unsigned row_size, const float *lu_c, const float *lu_l,
+                        const float *lu_u, unsigned out_h, const float *src, ptrdiff_t src_stride, float *dst,
+                        ptrdiff_t dst_stride, unsigned width)
+{
+	/* unresize_impl.cpp:58-85, 150-156: forward over every row, then back */
+	for (unsigned i = 0; i < out_h; ++i) {
+		float *d = ROWP(float, dst, dst_stride, i);
+		for (unsigned j = 0; j < width; ++j) {
+			float z = i ? CROWP(float, dst, dst_stride, i - 1)[j] : 0.0f;
+			float accum = 0.0f;
+			for (unsigned k = 0; k < row_size; ++k)
+				accum += coef[(size_t)i * row_size + k] * CROWP(float, src, src_stride, offset[i] + k)[j];
+			z = (accum - lu_c[i] * z) * lu_l[i];
+			d[j] = z;
+		}
+	}
+	for (unsigned i = out_h; i != 0; --i) {
+		for (unsigned j = 0; j < width; ++j) {
+			float w = i < out_h ? CROWP(float, dst, dst_stride, i)[j] : 0.0f;
+			w = CROWP(float, dst, dst_stride, i - 1)[j] - lu_u[i - 1] * w;
+			ROWP(float, dst, dst_stride, i - 1)[j] = w;
+		}
+	}
+}

@@ -149,6 +149,19 @@ int orc_colorspace_build_approx(int m_in, int t_in, int p_in, int m_out, int t_o
                                 double peak_luminance, int scene_referred, int chromatic_adaptation,
                                 orc_csp_ops *ops, char *trace, size_t trace_len);
 void orc_colorspace_apply(const orc_csp_ops *ops, float *p0, float *p1, float *p2, size_t n);
+/* ---- unresize (unresize/bilinear.cpp, unresize_impl.cpp) ----
+ * Context of unresizing up_dim samples to orig_dim (call first with coef ==
+ * NULL for row_size): A' band rows of row_size taps from offset[i] (stride
+ * row_size), and the LU factors of A'A (lu_l inverted). */
+int orc_unresize_context(unsigned up_dim, unsigned orig_dim, double shift, unsigned *row_size,
+                         float *coef, unsigned *offset, float *lu_c, float *lu_l, float *lu_u);
+void orc_unresize_h_f32(const float *coef, const unsigned *offset, unsigned row_size, const float *lu_c, const float *lu_l,
+                        const float *lu_u, unsigned out_w, const float *src, ptrdiff_t src_stride, float *dst,
+                        ptrdiff_t dst_stride, unsigned height);
+void orc_unresize_v_f32(const float *coef, const unsigned *offset, unsigned row_size, const float *lu_c, const float *lu_l,
+                        const float *lu_u, unsigned out_h, const float *src, ptrdiff_t src_stride, float *dst,
+                        ptrdiff_t dst_stride, unsigned width);
+
 /* Double-precision matrix helpers exposed for tests. */
 void orc_ncl_rgb_to_yuv(int matrix, double out[9]);
 void orc_gamut_matrix(int p_in, int p_out, int adapt, double out[9]);

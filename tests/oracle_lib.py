@@ -125,6 +125,14 @@ def oracle():
         lib.orc_colorspace_build_approx.argtypes = lib.orc_colorspace_build.argtypes
         lib.orc_colorspace_apply.restype = None
         lib.orc_colorspace_apply.argtypes = [ctypes.POINTER(CspOps), c_float_p, c_float_p, c_float_p, ctypes.c_size_t]
+        lib.orc_unresize_context.restype = ctypes.c_int
+        lib.orc_unresize_context.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_double, c_u32_p, c_float_p, c_u32_p, c_float_p,
+                                             c_float_p, c_float_p]
+        for name in ("orc_unresize_h_f32", "orc_unresize_v_f32"):
+            f = getattr(lib, name)
+            f.restype = None
+            f.argtypes = [c_float_p, c_u32_p, ctypes.c_uint, c_float_p, c_float_p, c_float_p, ctypes.c_uint, c_float_p,
+                          ctypes.c_ssize_t, c_float_p, ctypes.c_ssize_t, ctypes.c_uint]
         lib.orc_ncl_rgb_to_yuv.restype = None
         lib.orc_ncl_rgb_to_yuv.argtypes = [ctypes.c_int, c_double_p]
         lib.orc_gamut_matrix.restype = None
@@ -158,6 +166,9 @@ def ref():
                                                                   c_float_p, c_float_p, c_float_p, ctypes.c_size_t]
         lib.ref_colorspace_apply_auto.restype = ctypes.c_int
         lib.ref_colorspace_apply_auto.argtypes = lib.ref_colorspace_apply.argtypes
+        lib.ref_unresize_context.restype = ctypes.c_int
+        lib.ref_unresize_context.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_double, c_u32_p, c_float_p, c_u32_p, c_float_p,
+                                             c_float_p, c_float_p]
         lib.ref_ncl_rgb_to_yuv.argtypes = [ctypes.c_int, c_double_p]
         lib.ref_half_to_float.restype = ctypes.c_float
         lib.ref_half_to_float.argtypes = [ctypes.c_uint16]
@@ -381,3 +392,62 @@ def ref_colorspace_apply(csp_in, csp_out, p0, p1, p2, peak=100.0, scene=0, appro
     fn = ref().ref_colorspace_apply_auto if cpu_auto else ref().ref_colorspace_apply
     n = fn(*csp_in, *csp_out, peak, scene, approx, adapt, ptr(p0, c_float_p), ptr(p1, c_float_p), ptr(p2, c_float_p), p0.size)
     return n, p0, p1, p2
+
+
+# ---------------------------------------------------------------------------
+# unresize
+
+
+def unresize_context(up_dim, orig_dim, shift=0.0, use_ref=False):
+    """(row_size, coef[orig, row_size], offset, lu_c, lu_l, lu_u) of unresizing
+    up_dim samples to orig_dim (None when the reference refuses it)."""
+    fn = ref().ref_unresize_context if use_ref else oracle().orc_unresize_context
+    rs = ctypes.c_uint32(0)
+    if fn(up_dim, orig_dim, shift, ctypes.byref(rs), None, None, None, None, None) != 0:
+        return None
+    rs = rs.value
+    coef = np.zeros(orig_dim * rs, np.float32)
+    off = np.zeros(orig_dim, np.uint32)
+    c, l, u = (np.zeros(orig_dim, np.float32) for _ in range(3))
+    assert fn(up_dim, orig_dim, shift, ctypes.byref(ctypes.c_uint32(0)), ptr(coef, c_float_p), ptr(off, c_u32_p), ptr(c, c_float_p),
+              ptr(l, c_float_p), ptr(u, c_float_p)) == 0
+    return rs, coef.reshape(orig_dim, rs), off, c, l, u
+
+
+def unresize_h(img, ctx):
+    rs, coef, off, c, l, u = ctx
+    img = np.ascontiguousarray(img, np.float32)
+    out = np.zeros((img.shape[0], off.shape[0]), np.float32)
+    oracle().orc_unresize_h_f32(ptr(np.ascontiguousarray(coef), c_float_p), ptr(off, c_u32_p), rs, ptr(c, c_float_p), ptr(l, c_float_p),
+                                ptr(u, c_float_p), off.shape[0], ptr(img, c_float_p), img.strides[0], ptr(out, c_float_p),
+                                out.strides[0], img.shape[0])
+    return out
+
+
+def unresize_v(img, ctx):
+    rs, coef, off, c, l, u = ctx
+    img = np.ascontiguousarray(img, np.float32)
+    out = np.zeros((off.shape[0], img.shape[1]), np.float32)
+    oracle().orc_unresize_v_f32(ptr(np.ascontiguousarray(coef), c_float_p), ptr(off, c_u32_p), rs, ptr(c, c_float_p), ptr(l, c_float_p),
+                                ptr(u, c_float_p), off.shape[0], ptr(img, c_float_p), img.strides[0], ptr(out, c_float_p),
+                                out.strides[0], img.shape[1])
+    return out
+
+
+def unresize_plane(img, dst_w, dst_h, shift_w=0.0, shift_h=0.0):
+    """UnresizeConversion::create (unresize/unresize.cpp:36-88) over the oracle's kernels."""
+    up_h, up_w = img.shape
+    skip_h = up_w == dst_w and shift_w == 0
+    skip_v = up_h == dst_h and shift_h == 0
+    if skip_h and skip_v:
+        return np.array(img, np.float32)
+    if skip_h:
+        return unresize_v(img, unresize_context(up_h, dst_h, shift_h))
+    if skip_v:
+        return unresize_h(img, unresize_context(up_w, dst_w, shift_w))
+    xs, ys = dst_w / up_w, dst_h / up_h
+    if max(xs, 1.0) * 2.0 + xs * max(ys, 1.0) < max(ys, 1.0) + ys * max(xs, 1.0) * 2.0:
+        t = unresize_h(img, unresize_context(up_w, dst_w, shift_w))
+        return unresize_v(t, unresize_context(up_h, dst_h, shift_h))
+    t = unresize_v(img, unresize_context(up_h, dst_h, shift_h))
+    return unresize_h(t, unresize_context(up_w, dst_w, shift_w))

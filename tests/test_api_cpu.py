@@ -334,3 +334,40 @@ def test_geometry_oracle_matches_reference_traces(name):
         want = tuple(float(m.group(k)) for k in range(6, 10))
         got = P.resize_params(*pair)
         assert got is not None and np.allclose(got, want, atol=5e-7), (name, q, got, want)
+
+
+def test_unresize_lowering_and_limits():
+    """resample_filter ZIMG_RESIZE_UNRESIZE (zimg.cpp:21,450): planes convert
+    to FLOAT and unresize (graphbuilder.cpp:721-724,761-796), pass order by
+    the same cost rule as resize (unresize.cpp:12-18); upscaling, subregions
+    and unresize named only for chroma are refused."""
+    U = -1
+    s = z.image_format(width=640, height=360, pixel_type=z.PIXEL_FLOAT)
+    g = z.FilterGraph(s, z.image_format(width=333, height=200, pixel_type=z.PIXEL_FLOAT), z.builder_params(resample_filter=U))
+    assert g.plan_trace() == ["unresize[0]: [640, 360] => [333, 200] (0.000000, 0.000000)"]
+    assert [l.split(":")[1].strip() for l in g.device_stages()] == ["unresize_v[0]", "unresize_h[0]"]
+    w = z.image_format(width=640, height=360, pixel_type=z.PIXEL_WORD, depth=12)
+    g = z.FilterGraph(w, z.image_format(width=640, height=181, pixel_type=z.PIXEL_WORD, depth=12), z.builder_params(resample_filter=U))
+    assert [t.split(":")[0] for t in g.plan_trace()] == ["depth[0]", "unresize[0]", "depth[0]"]
+    assert _err_of(z.image_format(width=64, height=48, pixel_type=z.PIXEL_FLOAT), z.image_format(width=128, height=48, pixel_type=z.PIXEL_FLOAT),
+                   z.builder_params(resample_filter=U)) == z.ERROR_RESAMPLING_NOT_AVAILABLE
+    assert _err_of(z.image_format(width=64, height=48, pixel_type=z.PIXEL_FLOAT, active_width=60.0),
+                   z.image_format(width=32, height=48, pixel_type=z.PIXEL_FLOAT), z.builder_params(resample_filter=U)) == \
+        z.ERROR_RESAMPLING_NOT_AVAILABLE
+    assert _err_of(s, z.image_format(width=320, height=360, pixel_type=z.PIXEL_FLOAT),
+                   z.builder_params(resample_filter_uv=U)) == z.ERROR_ILLEGAL_ARGUMENT
+
+
+def test_packed_layout_checks():
+    """Packed device layouts must match the graph's format (include/zimg.h):
+    YUY2 needs 4:2:2 BYTE, NV12 4:2:0 BYTE or WORD; staging covers every
+    plane that cannot be used in place."""
+    f420 = z.image_format(width=64, height=48, pixel_type=z.PIXEL_BYTE, subsample_w=1, subsample_h=1, color_family=z.COLOR_YUV,
+                          matrix_coefficients=z.MATRIX_BT709, transfer_characteristics=z.TRANSFER_BT709, color_primaries=z.PRIMARIES_BT709)
+    g = z.FilterGraph(f420, f420)
+    with pytest.raises(z.ZimgError) as ei:
+        g.device_packed_tmp_size(1, z.PACKING_YUY2, z.PACKING_PLANAR)
+    assert ei.value.code == z.ERROR_ILLEGAL_ARGUMENT
+    planar = g.device_packed_tmp_size(2, z.PACKING_PLANAR, z.PACKING_PLANAR)
+    nv12 = g.device_packed_tmp_size(2, z.PACKING_NV12, z.PACKING_NV12)
+    assert planar == g.device_tmp_size(2) and nv12 >= planar + 2 * 2 * (256 * 48 + 2 * 256 * 24)

@@ -345,3 +345,37 @@ def test_cl_and_ictcp_420_to_rgb(matrix, transfer):
     want = colorspace_apply(ops, Yf, Uf, Vf)
     for q in range(3):
         _assert_bits(outs[q], want[q], f"plane {q}")
+
+
+@pytest.mark.parametrize("sw,sh,dw,dh", [(640, 360, 333, 200), (500, 300, 500, 143), (777, 101, 300, 101), (64, 64, 64, 64)])
+def test_unresize_float_bit_exact(sw, sh, dw, dh):
+    """Unresize (resample_filter ZIMG_RESIZE_UNRESIZE): FLOAT planes, both
+    passes or one, against the oracle's restatement of unresize_impl.cpp."""
+    from oracle_lib import unresize_plane
+    print(f"case: unresize FLOAT {sw}x{sh}->{dw}x{dh}", flush=True)
+    img = np.random.default_rng(sw + dh).random((2, sh, sw), dtype=np.float32)
+    g = z.FilterGraph(z.image_format(width=sw, height=sh, pixel_type=z.PIXEL_FLOAT),
+                      z.image_format(width=dw, height=dh, pixel_type=z.PIXEL_FLOAT), z.builder_params(resample_filter=-1))
+    (out, *_) = gpu_util.run_graph(g, [img, None, None, None], [(dh, dw, np.float32), None, None, None], pad_bytes=12)
+    for k in range(2):
+        _assert_bits(out[k], unresize_plane(img[k], dw, dh), f"frame {k}")
+
+
+def test_unresize_yuv420_word_chain():
+    """4:2:0 10-bit WORD unresized horizontally: to FLOAT, unresize with the
+    LEFT-siting chroma shift, back to WORD (graphbuilder.cpp:721-724,761-796)."""
+    from oracle_lib import unresize_plane, dither as odither
+    w, h, dw = 640, 360, 320
+    print(f"case: unresize 4:2:0 WORD {w}x{h}->{dw}x{h}", flush=True)
+    planes = _yuv420_planes(w, h, 31)
+    g = z.FilterGraph(_word420_fmt(w, h), _word420_fmt(dw, h), z.builder_params(resample_filter=-1))
+    outs = gpu_util.run_graph(g, planes + [None], [(h, dw, np.uint16), (h // 2, dw // 2, np.uint16), (h // 2, dw // 2, np.uint16), None])
+    gs, gd = P.plane_geometry(w, h, ssw=1, ssh=1), P.plane_geometry(dw, h, ssw=1, ssh=1)
+    for q in range(3):
+        chroma = q > 0
+        s_, o_ = scale_offset((WORD, 10, 0, int(chroma), 0), (FLOAT, 32, 0, 0, 0))
+        f = to_float(planes[q], WORD, s_, o_)
+        shift_w = P.resize_params(gs[q], gd[q])[0]
+        r = unresize_plane(f, gd[q][0], gd[q][1], shift_w=shift_w)
+        s2, o2 = scale_offset((FLOAT, 32, 0, 0, 0), (WORD, 10, 0, int(chroma), 0))
+        _assert_bits(outs[q], odither(r, FLOAT, WORD, s2, o2, 10, plane=q), f"plane {q}")

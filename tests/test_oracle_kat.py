@@ -446,3 +446,69 @@ def test_fp64_matrix_reproduces_pinned_f32_table():
         assert np.array_equal(want.ravel().view(np.uint32), d32.view(np.uint32))
         m = oracle_filter64(ftype, a, b, src, dst)
         assert np.max(np.abs(m.sum(axis=1) - 1.0)) < 1e-12
+
+
+def _bilinear_matrix(up, orig, shift):
+    """The bilinear upscale A (up x orig) that unresize inverts, from its
+    definition: output sample i sits at (i + 0.5) orig / up on the input grid,
+    interpolated linearly between the two nearest input samples, the outermost
+    input sample repeated beyond the ends (bilinear.cpp:64-108)."""
+    A = np.zeros((up, orig))
+    sh = -shift * orig / up
+    lo, hi = 0.5 + sh, orig - 0.5 + sh
+    for i in range(up):
+        x = (i + 0.5) * orig / up
+        if x <= lo:
+            A[i, int(min(max(math.floor(lo), 0), orig - 1))] = 1
+        elif x >= hi:
+            A[i, int(min(max(math.floor(hi), 0), orig - 1))] = 1
+        else:
+            k = int(math.floor(x - lo))
+            d = x - k - lo
+            A[i, k] += 1 - d
+            A[i, k + 1] += d
+    return A
+
+
+@pytest.mark.parametrize("up,orig,shift", [(40, 17, 0.0), (64, 32, 0.25), (33, 33, 0.5), (90, 31, -0.125)])
+def test_unresize_solves_least_squares(up, orig, shift):
+    """Unresize is the least-squares inverse of the bilinear upscale: its
+    output solves (A'A) x = A'y for the A built above (FP64 numpy solve; float
+    error only), and unresizing A x gives back x."""
+    from oracle_lib import unresize_context, unresize_h, unresize_v
+    A = _bilinear_matrix(up, orig, shift)
+    ctx = unresize_context(up, orig, shift)
+    rng = np.random.default_rng(up + orig)
+    y = rng.random((5, up)).astype(np.float32)
+    want = np.linalg.solve(A.T @ A, A.T @ y.astype(np.float64).T).T
+    got = unresize_h(y, ctx)
+    assert np.max(np.abs(got - want)) < 2e-4 * max(1.0, np.max(np.abs(want)))
+    x = rng.random((orig, 6))
+    back = unresize_v((A @ x).astype(np.float32), ctx)
+    assert np.max(np.abs(back - x)) < 2e-4
+
+
+def test_unresize_kernels_sequential_order():
+    """Both line kernels in the reference's float order (unresize_impl.cpp:
+    27-85): the band accumulated from 0 in tap order, forward substitution
+    z = (accum - c z) l, then back substitution w = d - u w."""
+    from oracle_lib import unresize_context, unresize_h, unresize_v
+    rs, coef, off, c, l, u = ctx = unresize_context(23, 9, 0.25)
+    rng = np.random.default_rng(5)
+    y = (rng.random((3, 23)) * np.where(np.arange(23) % 2, 1e3, 1e-3)).astype(np.float32)
+    want = np.zeros((3, 9), np.float32)
+    f = np.float32
+    for r in range(3):
+        z = f(0)
+        for j in range(9):
+            acc = f(0)
+            for k in range(rs):
+                acc = f(acc + f(coef[j, k] * y[r, off[j] + k]))
+            z = f(f(acc - f(c[j] * z)) * l[j])
+            want[r, j] = z
+        w = f(0)
+        for j in range(9, 0, -1):
+            w = f(want[r, j - 1] - f(u[j - 1] * w))
+            want[r, j - 1] = w
+    assert np.array_equal(unresize_h(y, ctx).view(np.uint32), want.view(np.uint32))
+    assert np.array_equal(unresize_v(y.T.copy(), ctx).view(np.uint32), want.T.copy().view(np.uint32))

@@ -6,7 +6,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from oracle_lib import (BICUBIC, BILINEAR, LANCZOS, POINT, SPLINE16, SPLINE36, SPLINE64, blue_noise, c_float_p, c_u8_p, colorspace_apply,
+from oracle_lib import (unresize_context, BICUBIC, BILINEAR, LANCZOS, POINT, SPLINE16, SPLINE36, SPLINE64, blue_noise, c_float_p, c_u8_p, colorspace_apply,
                         colorspace_ops, oracle, oracle_filter, ptr, ref, ref_colorspace_apply, ref_filter)
 
 pytestmark = pytest.mark.ref
@@ -188,3 +188,18 @@ def test_scale_offset_bit_exact():
             o.orc_scale_offset(*fi, *fo, ctypes.byref(so[0]), ctypes.byref(so[1]))
             r.ref_scale_offset(*fi, *fo, ctypes.byref(sr[0]), ctypes.byref(sr[1]))
             assert (so[0].value, so[1].value) == (sr[0].value, sr[1].value)
+
+
+@pytest.mark.parametrize("up,orig,shift", [(13, 7, 0.0), (100, 37, 0.25), (3840, 1920, 0.0), (2160, 1080, -0.25), (64, 64, 0.5),
+                                           (17, 1, 0.0), (1999, 1000, 0.125), (9, 8, -1.5)])
+def test_unresize_context_bit_exact(up, orig, shift):
+    """create_bilinear_context (unresize/bilinear.cpp:112-172), compiled from
+    the reference: the A' band, its offsets and the LU factors of A'A, bit for bit."""
+    a = unresize_context(up, orig, shift)
+    b = unresize_context(up, orig, shift, use_ref=True)
+    assert b is not None and a is not None
+    assert a[0] == b[0]
+    for x, y in zip(a[1:], b[1:]):
+        assert np.array_equal(np.asarray(x).view(np.uint32), np.asarray(y).view(np.uint32))
+    # upscaling by unresize is refused by both
+    assert unresize_context(7, 13) is None and unresize_context(7, 13, use_ref=True) is None

@@ -66,6 +66,7 @@ EXPORTS = [
     "zimg_image_format_default", "zimg_graph_builder_params_default", "zimg_filter_graph_build",
     "zimg_filter_graph_get_device_tmp_size", "zimg_filter_graph_process_device",
     "zimg_filter_graph_process_device_batch", "zimg_filter_graph_process_device_batch_noalloc", "zimg_filter_graph_describe",
+    "zimg_filter_graph_get_device_packed_tmp_size", "zimg_filter_graph_process_device_packed",
     "zimg_filter_graph_set_resize_precision", "zimg_filter_graph_process_device_profiled",
     "zimg_device_jobs_get_tmp_size", "zimg_device_jobs_create", "zimg_device_jobs_run", "zimg_device_jobs_describe",
     "zimg_device_jobs_free",
@@ -112,6 +113,30 @@ class _Plane(ctypes.Structure):
 class ImageBufferStruct(ctypes.Structure):
     """zimg_image_buffer / zimg_image_buffer_const (zimg.h:409-430)."""
     _fields_ = [("version", ctypes.c_uint), ("plane", _Plane * 4)]
+
+
+PACKING_PLANAR, PACKING_NV12, PACKING_YUY2, PACKING_UYVY = 0, 1, 2, 3
+
+
+class PackedBufferStruct(ctypes.Structure):
+    """zimg_device_packed_buffer (include/zimg.h)."""
+    _fields_ = [("version", ctypes.c_uint), ("packing", ctypes.c_int), ("shift", ctypes.c_uint),
+                ("data", ctypes.c_void_p * 4), ("stride", ctypes.c_ssize_t * 4), ("frame_stride", ctypes.c_ssize_t * 4)]
+
+
+def packed_buffer(packing, planes, shift=0, version=None):
+    """planes: up to 4 (device ptr, row stride, frame stride) or None."""
+    b = PackedBufferStruct()
+    b.version = API_VERSION if version is None else version
+    b.packing = packing
+    b.shift = shift
+    for i, pl in enumerate(planes):
+        if pl is None:
+            continue
+        b.data[i] = pl[0]
+        b.stride[i] = pl[1]
+        b.frame_stride[i] = pl[2] if len(pl) > 2 else 0
+    return b
 
 
 CALLBACK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint)
@@ -168,6 +193,11 @@ def lib():
         L.zimg_filter_graph_process_device_batch_noalloc.argtypes = [vp, ctypes.c_uint, ctypes.POINTER(ImageBufferStruct),
                                                                      ctypes.POINTER(ctypes.c_ssize_t), ctypes.POINTER(ImageBufferStruct),
                                                                      ctypes.POINTER(ctypes.c_ssize_t), vp]
+        L.zimg_filter_graph_get_device_packed_tmp_size.restype = ctypes.c_int
+        L.zimg_filter_graph_get_device_packed_tmp_size.argtypes = [vp, ctypes.c_uint, ctypes.c_int, ctypes.c_int, ctypes.POINTER(sz)]
+        L.zimg_filter_graph_process_device_packed.restype = ctypes.c_int
+        L.zimg_filter_graph_process_device_packed.argtypes = [vp, ctypes.c_uint, ctypes.POINTER(PackedBufferStruct),
+                                                              ctypes.POINTER(PackedBufferStruct), vp, vp]
         L.zimg_filter_graph_describe.restype = sz
         L.zimg_filter_graph_describe.argtypes = [vp, ctypes.c_char_p, sz]
         L.zimg_filter_graph_process_device_profiled.restype = ctypes.c_int
@@ -316,6 +346,14 @@ class FilterGraph:
         check(lib().zimg_filter_graph_process_device_profiled(self._h, count, ctypes.byref(src), sfs, ctypes.byref(dst), dfs, tmp, stream,
                                                               64, ms, by, ctypes.byref(n)))
         return [(ms[k], by[k]) for k in range(min(n.value, 64))]
+
+    def device_packed_tmp_size(self, count, src_packing, dst_packing):
+        out = ctypes.c_size_t()
+        check(lib().zimg_filter_graph_get_device_packed_tmp_size(self._h, count, src_packing, dst_packing, ctypes.byref(out)))
+        return out.value
+
+    def process_device_packed(self, count, src, dst, tmp=0, stream=0):
+        check(lib().zimg_filter_graph_process_device_packed(self._h, count, ctypes.byref(src), ctypes.byref(dst), tmp, stream))
 
     def set_resize_precision(self, mode):
         check(lib().zimg_filter_graph_set_resize_precision(self._h, mode))

@@ -328,8 +328,10 @@ ResampleFilter tr_filter(zimg_resample_filter_e f, double a, double b)
 		return r;
 	}
 	default:
+		// ZIMG_RESIZE_UNRESIZE (zimg.cpp:21,320-323): no filter object; the
+		// builder unresizes instead (graphbuilder.cpp:761-796).
 		if (static_cast<int>(f) == -1)
-			fail(ZIMG_ERROR_UNSUPPORTED_OPERATION, "unresize not supported");
+			return r;
 		enum_error("unrecognized resampling filter");
 	}
 }
@@ -391,6 +393,9 @@ BuildParams import_params(const zimg_graph_builder_params &p)
 		r.filter = tr_filter(p.resample_filter, p.filter_param_a, p.filter_param_b);
 		r.filter_uv = tr_filter(p.resample_filter_uv, p.filter_param_a_uv, p.filter_param_b_uv);
 		r.dither = tr_dither(p.dither_type);
+		r.unresize = static_cast<int>(p.resample_filter) == -1;
+		if (!r.unresize && static_cast<int>(p.resample_filter_uv) == -1)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "unresize is selected by resample_filter");
 		tr_cpu(p.cpu_type);
 	}
 	if (p.version >= API_2_2) {
@@ -432,6 +437,61 @@ DeviceBuffers import_device_buffer(const Buf *b, const ptrdiff_t *fstride, const
 			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, is_src ? "missing source plane" : "missing destination plane");
 	}
 	return d;
+}
+
+// Where each planar sample of a graph-side plane lives in a packed layout
+// (zimg_device_packed_buffer): packed plane index, element step and position.
+struct PackMap {
+	int packed = -1; // -1: the plane is absent
+	unsigned step = 1, pos = 0;
+};
+
+void pack_maps(int packing, const Plan &plan, bool src, PixelType type, PackMap map[4])
+{
+	const unsigned *w = src ? plan.src_w : plan.dst_w;
+	const unsigned *h = src ? plan.src_h : plan.dst_h;
+	for (int q = 0; q < 4; ++q)
+		map[q] = PackMap{};
+	if (packing == ZIMG_PACKING_PLANAR) {
+		for (int q = 0; q < 4; ++q)
+			if (w[q])
+				map[q] = PackMap{ q, 1, 0 };
+		return;
+	}
+	if (!w[1] || !w[2] || w[3])
+		fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "packed layouts carry YUV without alpha");
+	if (packing == ZIMG_PACKING_NV12) {
+		if (w[1] * 2 != w[0] || h[1] * 2 != h[0] || (type != PixelType::BYTE && type != PixelType::WORD))
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "NV12 / P010 layout needs 4:2:0 BYTE or WORD");
+		map[0] = PackMap{ 0, 1, 0 };
+		map[1] = PackMap{ 1, 2, 0 };
+		map[2] = PackMap{ 1, 2, 1 };
+	} else if (packing == ZIMG_PACKING_YUY2 || packing == ZIMG_PACKING_UYVY) {
+		if (w[1] * 2 != w[0] || h[1] != h[0] || type != PixelType::BYTE)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "YUY2 / UYVY layout needs 4:2:2 BYTE");
+		const bool yuy2 = packing == ZIMG_PACKING_YUY2;
+		map[0] = PackMap{ 0, 2, yuy2 ? 0u : 1u };
+		map[1] = PackMap{ 0, 4, yuy2 ? 1u : 0u };
+		map[2] = PackMap{ 0, 4, yuy2 ? 3u : 2u };
+	} else {
+		fail(ZIMG_ERROR_ENUM_OUT_OF_RANGE, "unrecognized packing");
+	}
+}
+
+// Bytes of planar staging one side of a packed call needs per frame (planes
+// used in place need none), and each staged plane's offset and pitch.
+size_t packed_staging(const PackMap map[4], const unsigned *w, const unsigned *h, unsigned bpp, unsigned shift, bool stage[4],
+                      size_t off[4], size_t pitch[4], unsigned frames, size_t base)
+{
+	size_t o = base;
+	for (int q = 0; q < 4; ++q) {
+		stage[q] = map[q].packed >= 0 && (map[q].step != 1 || shift != 0);
+		pitch[q] = (static_cast<size_t>(w[q]) * bpp + 255) / 256 * 256;
+		off[q] = o;
+		if (stage[q])
+			o += pitch[q] * h[q] * frames;
+	}
+	return o - base;
 }
 
 } // namespace
@@ -771,6 +831,99 @@ zimg_error_code_e zimg_filter_graph_process_device_batch_noalloc(const zimg_filt
 		// Every intermediate plane is written in full before it is read, so
 		// nothing in the scratch needs clearing between calls.
 		prog.run(ds, dd, need ? scratch_for_current_device() : nullptr, count, stream);
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
+zimg_error_code_e zimg_filter_graph_get_device_packed_tmp_size(const zimg_filter_graph *ptr, unsigned count, int src_packing,
+                                                               int dst_packing, size_t *out)
+{
+	try {
+		const DeviceProgram &prog = program(ptr);
+		const Plan &plan = prog.plan();
+		const unsigned n = count ? count : 1;
+		PackMap ms[4], md[4];
+		pack_maps(src_packing, plan, true, plan.src_type, ms);
+		pack_maps(dst_packing, plan, false, plan.dst_type, md);
+		bool st[4];
+		size_t off[4], pitch[4];
+		// shift 1 stands for "any nonzero": the luma plane is then staged too
+		size_t total = packed_staging(ms, plan.src_w, plan.src_h, ptr->src_bpp, src_packing == ZIMG_PACKING_PLANAR ? 0 : 1, st, off, pitch, n, 0);
+		total += packed_staging(md, plan.dst_w, plan.dst_h, ptr->dst_bpp, dst_packing == ZIMG_PACKING_PLANAR ? 0 : 1, st, off, pitch, n, 0);
+		*out = (total + 255) / 256 * 256 + prog.tmp_size(n);
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
+zimg_error_code_e zimg_filter_graph_process_device_packed(const zimg_filter_graph *ptr, unsigned count, const zimg_device_packed_buffer *src,
+                                                          const zimg_device_packed_buffer *dst, void *tmp, void *stream)
+{
+	try {
+		const DeviceProgram &prog = program(ptr);
+		const Plan &plan = prog.plan();
+		if (!src || !dst)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null buffer");
+		if (src->version < API_2_0 || dst->version < API_2_0)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "API version invalid");
+		const unsigned n = count ? count : 1;
+		if (src->shift >= 8 * ptr->src_bpp || dst->shift >= 8 * ptr->dst_bpp ||
+		    (ptr->src_bpp != 2 && src->shift) || (ptr->dst_bpp != 2 && dst->shift))
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "sample shift only for WORD samples, below 16");
+		PackMap ms[4], md[4];
+		pack_maps(src->packing, plan, true, plan.src_type, ms);
+		pack_maps(dst->packing, plan, false, plan.dst_type, md);
+		bool sst[4], dst_st[4];
+		size_t soff[4], spitch[4], doff[4], dpitch[4];
+		const unsigned sshift = src->packing == ZIMG_PACKING_PLANAR ? 0 : src->shift;
+		const unsigned dshift = dst->packing == ZIMG_PACKING_PLANAR ? 0 : dst->shift;
+		size_t used = packed_staging(ms, plan.src_w, plan.src_h, ptr->src_bpp, sshift, sst, soff, spitch, n, 0);
+		used += packed_staging(md, plan.dst_w, plan.dst_h, ptr->dst_bpp, dshift, dst_st, doff, dpitch, n, used);
+		const size_t prog_off = (used + 255) / 256 * 256;
+		if ((used || prog.tmp_size(n)) && !tmp)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "device tmp buffer required");
+		char *t = static_cast<char *>(tmp);
+
+		// graph-side planar views and the pack launches around the program
+		DeviceBuffers ds, dd;
+		dev::PackArg unpack{}, pack{};
+		unpack.frames = pack.frames = n;
+		unpack.pack = 0;
+		pack.pack = 1;
+		auto side = [&](const zimg_device_packed_buffer *b, const PackMap *m, const bool *st, const size_t *off, const size_t *pitch,
+		                const unsigned *w, const unsigned *h, unsigned bpp, unsigned shift, DeviceBuffers &out, dev::PackArg &pa) {
+			for (int q = 0; q < 4; ++q) {
+				if (m[q].packed < 0)
+					continue;
+				const int pp = m[q].packed;
+				if (!b->data[pp])
+					fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "missing packed plane");
+				const dev::PlaneArg packed{ static_cast<char *>(b->data[pp]), b->stride[pp], b->frame_stride[pp] };
+				if (!st[q]) {
+					out.ptr[q] = packed.ptr;
+					out.stride[q] = packed.stride;
+					out.fstride[q] = packed.fstride;
+					continue;
+				}
+				const dev::PlaneArg planar{ t + off[q], static_cast<long long>(pitch[q]), static_cast<long long>(pitch[q] * h[q]) };
+				out.ptr[q] = planar.ptr;
+				out.stride[q] = planar.stride;
+				out.fstride[q] = planar.fstride;
+				pa.p[pa.planes++] = dev::PackPlane{ planar, packed, w[q], h[q], bpp, m[q].step, m[q].pos, shift };
+			}
+		};
+		side(src, ms, sst, soff, spitch, plan.src_w, plan.src_h, ptr->src_bpp, sshift, ds, unpack);
+		side(dst, md, dst_st, doff, dpitch, plan.dst_w, plan.dst_h, ptr->dst_bpp, dshift, dd, pack);
+		int rc = dev::launch_pack(unpack, stream);
+		if (rc)
+			fail(ZIMG_ERROR_UNKNOWN, std::string("unpack launch failed: ") + hipGetErrorString(static_cast<hipError_t>(rc)));
+		prog.run(ds, dd, t ? t + prog_off : nullptr, n, stream);
+		rc = dev::launch_pack(pack, stream);
+		if (rc)
+			fail(ZIMG_ERROR_UNKNOWN, std::string("pack launch failed: ") + hipGetErrorString(static_cast<hipError_t>(rc)));
 		return ZIMG_ERROR_SUCCESS;
 	} catch (...) {
 		return handle_current_exception();

@@ -44,7 +44,7 @@ public:
 	};
 
 	struct Step {
-		enum Kind : int { RESIZE, POINT, CSP, ED } kind = POINT;
+		enum Kind : int { RESIZE, POINT, CSP, ED, UNRESIZE } kind = POINT;
 		Buf in[3], out[3];
 		dev::InXform ix[3]{};
 		dev::OutXform ox[3]{};
@@ -109,6 +109,8 @@ public:
 		std::map<int, dev::PassArg> pass; // by node id
 		void *lut_mem = nullptr;
 		std::map<int, const float *> lut; // gamma lookup tables, by step index * 8 + operation index
+		void *unresize_mem = nullptr;
+		std::map<int, dev::UnresizeArg> unresize; // table pointers of unresize passes, by node id
 	};
 
 	// Uploads every table the program's launches read on the current HIP

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstdio>
+#include <array>
 #include <cstring>
 
 #include "device.h"
@@ -67,6 +68,8 @@ DeviceProgram::~DeviceProgram()
 			(void)hipFree(kv.second->mem);
 		if (kv.second && kv.second->lut_mem)
 			(void)hipFree(kv.second->lut_mem);
+		if (kv.second && kv.second->unresize_mem)
+			(void)hipFree(kv.second->unresize_mem);
 	}
 	for (auto &kv : m_mfma_tables)
 		if (kv.second && kv.second->mem)
@@ -205,6 +208,18 @@ void DeviceProgram::lower()
 			s.mid_w = s.in_w;
 			s.mid_h = s.in_h;
 			std::snprintf(label, sizeof(label), "resize_%c(fw=%u)[%d]", s.h_first ? 'h' : 'v', n.table->filter_width, n.plane);
+			s.label = label;
+			break;
+		case NodeKind::UNRESIZE_H:
+		case NodeKind::UNRESIZE_V:
+			if (in_type != PixelType::FLOAT)
+				fail(ZIMG_ERROR_UNKNOWN, "unresize of a non-FLOAT plane");
+			s.kind = Step::UNRESIZE;
+			s.h_first = n.kind == NodeKind::UNRESIZE_H;
+			s.tab[0] = id;
+			s.in_w = dep->width;
+			s.in_h = dep->height;
+			std::snprintf(label, sizeof(label), "unresize_%c[%d]", s.h_first ? 'h' : 'v', n.plane);
 			s.label = label;
 			break;
 		case NodeKind::COPY_RECT:
@@ -1170,6 +1185,43 @@ const DeviceProgram::Tables &DeviceProgram::tables_for_current_device() const
 		for (auto &e : lut_at)
 			tab->lut[e.first] = static_cast<const float *>(tab->lut_mem) + e.second;
 	}
+	// Unresize tables: band, offsets and LU factors of each pass, one allocation.
+	std::vector<char> ubuf;
+	std::vector<std::pair<int, std::array<size_t, 5>>> uat;
+	for (const Step &st : m_steps) {
+		if (st.dead || st.kind != Step::UNRESIZE)
+			continue;
+		const UnresizeTable &t = *m_plan.nodes[st.tab[0]].utable;
+		std::array<size_t, 5> o{};
+		auto put = [&](const void *p, size_t n) {
+			const size_t at = align_up(ubuf.size(), 256);
+			ubuf.resize(at + n);
+			std::memcpy(ubuf.data() + at, p, n);
+			return at;
+		};
+		o[0] = put(t.coef.data(), t.coef.size() * 4);
+		o[1] = put(t.offset.data(), t.offset.size() * 4);
+		o[2] = put(t.lu_c.data(), t.lu_c.size() * 4);
+		o[3] = put(t.lu_l.data(), t.lu_l.size() * 4);
+		o[4] = put(t.lu_u.data(), t.lu_u.size() * 4);
+		uat.emplace_back(st.tab[0], o);
+	}
+	if (!ubuf.empty()) {
+		check_hip(hipMalloc(&tab->unresize_mem, ubuf.size()), "hipMalloc(unresize tables)");
+		check_hip(hipMemcpy(tab->unresize_mem, ubuf.data(), ubuf.size(), hipMemcpyHostToDevice), "hipMemcpy(unresize tables)");
+		const char *base = static_cast<const char *>(tab->unresize_mem);
+		for (auto &e : uat) {
+			const UnresizeTable &t = *m_plan.nodes[e.first].utable;
+			dev::UnresizeArg a{};
+			a.row_size = t.row_size;
+			a.coef = reinterpret_cast<const float *>(base + e.second[0]);
+			a.offset = reinterpret_cast<const unsigned *>(base + e.second[1]);
+			a.c = reinterpret_cast<const float *>(base + e.second[2]);
+			a.l = reinterpret_cast<const float *>(base + e.second[3]);
+			a.u = reinterpret_cast<const float *>(base + e.second[4]);
+			tab->unresize[e.first] = a;
+		}
+	}
 	m_tables[device] = tab;
 	return *tab;
 }
@@ -1269,6 +1321,9 @@ std::vector<double> DeviceProgram::step_bytes() const
 			break;
 		case Step::ED:
 			b = s.ed_planes * px * (elem(s.ix[0].type) + elem(s.ox[0].type));
+			break;
+		case Step::UNRESIZE:
+			b = static_cast<double>(s.in_w) * s.in_h * 4 + px * 4;
 			break;
 		}
 		out.push_back(b);
@@ -1403,6 +1458,19 @@ void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void
 			rc = dev::launch_colorspace(a, stream);
 			break;
 		}
+		case Step::UNRESIZE: {
+			dev::UnresizeArg a = tabs.unresize.at(s.tab[0]);
+			a.src = resolve(s.in[0]);
+			a.dst = resolve(s.out[0]);
+			a.frames = frames;
+			a.src_w = s.in_w;
+			a.src_h = s.in_h;
+			a.dst_w = s.width;
+			a.dst_h = s.height;
+			a.horizontal = s.h_first ? 1 : 0;
+			rc = dev::launch_unresize(a, stream);
+			break;
+		}
 		case Step::ED: {
 			dev::EdArg a{};
 			for (int p = 0; p < s.ed_planes; ++p) {
@@ -1433,7 +1501,7 @@ std::string DeviceProgram::describe() const
 	std::string r;
 	for (auto &s : m_steps) {
 		char buf[256];
-		static const char *kinds[] = { "resize", "point", "colorspace", "error_diffusion" };
+		static const char *kinds[] = { "resize", "point", "colorspace", "error_diffusion", "unresize" };
 		if (s.kind == Step::RESIZE)
 			std::snprintf(buf, sizeof(buf), "%s %ux%u->%ux%u passes=%d %s tile=%ux%u lds=%zu tiled=%d(%ux%u lds=%zu): %s\n", kinds[s.kind],
 			              s.in_w, s.in_h, s.width, s.height, s.passes, s.h_first ? "h_first" : "v_first", s.tile_w, s.tile_h,

@@ -9,6 +9,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdlib>
+#include <limits>
 
 #include "common.h"
 
@@ -235,6 +236,119 @@ FilterTable compute_filter(const ResampleFilter &f, unsigned src_dim, unsigned d
 		t.left[i] = static_cast<unsigned>(left);
 	}
 	return t;
+}
+
+namespace {
+
+// One row of a sparse matrix as the reference's RowMatrix keeps it: the
+// stored range [left, left + v.size()) and its values (zeros outside).
+struct BandRow {
+	size_t left = 0;
+	std::vector<double> v;
+	double at(size_t j) const { return j < left || j >= left + v.size() ? 0.0 : v[j - left]; }
+	size_t right() const { return left + v.size(); }
+};
+
+} // namespace
+
+UnresizeTable compute_unresize(unsigned up_dim, unsigned orig_dim, double shift)
+{
+	// create_bilinear_context(in = orig_dim, out = up_dim, shift) (bilinear.cpp:112-172)
+	const unsigned in = orig_dim, out = up_dim;
+	if (in > out)
+		fail(ZIMG_ERROR_RESAMPLING_NOT_AVAILABLE, "unresize can not upscale");
+	const double sh = -shift * in / out;
+
+	// bilinear_weights (bilinear.cpp:64-108): out rows of one or two entries.
+	std::vector<BandRow> m(out);
+	const double leftmost = 0.5 + sh, rightmost = in - 0.5 + sh;
+	const unsigned leftmost_idx = static_cast<unsigned>(std::clamp(std::floor(leftmost), 0.0, in - 1.0));
+	const unsigned rightmost_idx = static_cast<unsigned>(std::clamp(std::floor(rightmost), 0.0, in - 1.0));
+	for (unsigned i = 0; i < out; ++i) {
+		const double position = (i + 0.5) * in / out;
+		if (position <= leftmost) {
+			m[i].left = leftmost_idx;
+			m[i].v = { 1.0 };
+		} else if (position >= rightmost) {
+			m[i].left = rightmost_idx;
+			m[i].v = { 1.0 };
+		} else {
+			const unsigned left_idx = static_cast<unsigned>(std::floor(position - leftmost));
+			const double distance = position - left_idx - leftmost;
+			m[i].left = left_idx;
+			m[i].v = { 1.0 - distance, distance };
+		}
+	}
+
+	// Transpose, its rows trimmed to their first and last nonzero entries
+	// (RowMatrix operator~ and compress, common/matrix.cpp:154-200).
+	std::vector<BandRow> t(in);
+	for (unsigned c = 0; c < in; ++c) {
+		size_t first = out, last = 0;
+		for (unsigned i = 0; i < out; ++i) {
+			if (m[i].at(c) != 0) {
+				first = std::min<size_t>(first, i);
+				last = i;
+			}
+		}
+		if (first <= last && first < out) {
+			t[c].left = first;
+			for (size_t i = first; i <= last; ++i)
+				t[c].v.push_back(m[i].at(c));
+		}
+	}
+
+	// The tridiagonal of A'A (operator*, matrix.cpp:202-219: each entry summed
+	// over the lhs row's stored range in order), then its LU factors
+	// (tridiagonal_decompose, bilinear.cpp:31-59).
+	auto prod = [&](size_t i, size_t j) {
+		double acc = 0;
+		for (size_t k = t[i].left; k < t[i].right(); ++k)
+			acc += t[i].at(k) * m[k].at(j);
+		return acc;
+	};
+	const double eps = std::numeric_limits<double>::epsilon();
+	std::vector<double> lc(in, 0.0), ll(in, 0.0), lu(in, 0.0);
+	if (in == 1) {
+		ll[0] = prod(0, 0);
+	} else {
+		lc[0] = 0;
+		ll[0] = prod(0, 0);
+		lu[0] = prod(0, 1) / (ll[0] + eps);
+		for (size_t i = 1; i + 1 < in; ++i) {
+			lc[i] = prod(i, i - 1);
+			ll[i] = prod(i, i) - lc[i] * lu[i - 1];
+			lu[i] = prod(i, i + 1) / (ll[i] + eps);
+		}
+		lc[in - 1] = prod(in - 1, in - 2);
+		ll[in - 1] = prod(in - 1, in - 1) - lc[in - 1] * lu[in - 2];
+		lu[in - 1] = 0;
+	}
+
+	UnresizeTable r;
+	r.up_dim = out;
+	r.orig_dim = in;
+	size_t rowsize = 0;
+	for (unsigned i = 0; i < in; ++i)
+		rowsize = std::max(rowsize, t[i].right() - t[i].left);
+	if (!rowsize)
+		fail(ZIMG_ERROR_UNKNOWN, "unresize: empty matrix");
+	r.row_size = static_cast<unsigned>(rowsize);
+	r.coef.assign(static_cast<size_t>(in) * rowsize, 0.0f);
+	r.offset.resize(in);
+	r.lu_c.resize(in);
+	r.lu_l.resize(in);
+	r.lu_u.resize(in);
+	for (unsigned i = 0; i < in; ++i) {
+		const size_t left = std::min(t[i].left, static_cast<size_t>(out) - rowsize);
+		for (size_t j = 0; j < t[i].right() - left; ++j)
+			r.coef[i * rowsize + j] = static_cast<float>(t[i].at(left + j));
+		r.offset[i] = static_cast<unsigned>(left);
+		r.lu_c[i] = static_cast<float>(lc[i]);
+		r.lu_l[i] = static_cast<float>(1.0 / (ll[i] + eps));
+		r.lu_u[i] = static_cast<float>(lu[i]);
+	}
+	return r;
 }
 
 } // namespace zamd

@@ -36,4 +36,21 @@ struct FilterTable {
 // compute_filter (filter.cpp:249-306) followed by matrix_to_filter (:43-127).
 FilterTable compute_filter(const ResampleFilter &f, unsigned src_dim, unsigned dst_dim, double shift, double width);
 
+// Unresize of one dimension (BilinearContext, unresize/bilinear.h:12-72): the
+// least-squares inverse of a bilinear upscale from orig_dim to up_dim samples.
+// Output sample j is the solution of (A'A) x = A'y with A the bilinear
+// matrix: a band of A' (row_size taps from offset[j]) then a tridiagonal LU
+// solve, forward with c and the inverted diagonal l, backward with u.
+struct UnresizeTable {
+	unsigned up_dim = 0, orig_dim = 0;
+	unsigned row_size = 0;
+	std::vector<float> coef;       // [orig_dim * row_size]
+	std::vector<unsigned> offset;  // [orig_dim]
+	std::vector<float> lu_c, lu_l, lu_u; // [orig_dim]
+};
+
+// create_bilinear_context (unresize/bilinear.cpp:112-172). shift is the
+// conversion's shift of the output grid (graphbuilder.cpp:773-774).
+UnresizeTable compute_unresize(unsigned up_dim, unsigned orig_dim, double shift);
+
 } // namespace zamd

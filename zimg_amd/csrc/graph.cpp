@@ -331,6 +331,8 @@ class Planner {
 	// graphbuilder.cpp:721-751
 	PixelFormat choose_resize_format(const State &t, int q)
 	{
+		if (m_params.unresize)
+			return PixelType::FLOAT;
 		auto supported = [](PixelType ty) { return ty == PixelType::WORD || ty == PixelType::FLOAT; };
 		const double src_pels = static_cast<double>(m_state.p[q].width) * m_state.p[q].height;
 		const double dst_pels = static_cast<double>(t.p[q].width) * t.p[q].height;
@@ -363,11 +365,32 @@ class Planner {
 		});
 	}
 
+	void add_unresize_pass(const Mask &mask, bool horizontal, unsigned up_w, unsigned up_h, unsigned orig_dim, double shift)
+	{
+		auto table = std::make_shared<UnresizeTable>(compute_unresize(horizontal ? up_w : up_h, orig_dim, shift));
+		each(mask, [&](int q) {
+			Node n;
+			n.kind = horizontal ? NodeKind::UNRESIZE_H : NodeKind::UNRESIZE_V;
+			n.dep[0] = m_ids[q].node;
+			n.dep_out[0] = m_ids[q].out;
+			n.width = horizontal ? orig_dim : up_w;
+			n.height = horizontal ? up_h : orig_dim;
+			n.out = m_state.p[q].format;
+			n.plane = q;
+			n.utable = table;
+			m_ids[q] = PlaneRef{ add_node(n), 0 };
+		});
+	}
+
 	// graphbuilder.cpp:753-824 and ResizeConversion::create (resize.cpp:38-102)
 	void resize_plane(const State &t, const Mask &mask, int q)
 	{
 		if (!needs_resize_plane(t, q))
 			return;
+		if (m_params.unresize) {
+			unresize_plane(t, mask, q);
+			return;
+		}
 		const PlaneState &sp = m_state.p[q], &dp = t.p[q];
 		const double scale_w = static_cast<double>(dp.aw) / sp.aw;
 		const double scale_h = static_cast<double>(dp.ah) / sp.ah;
@@ -408,6 +431,50 @@ class Planner {
 					add_resize_pass(mask, false, sw, sh, dh, shift_h, subheight, f, fmt);
 					add_resize_pass(mask, true, sw, dh, dw, shift_w, subwidth, f, fmt);
 				}
+			}
+		}
+		each(mask, [&](int r) {
+			PixelFormat keep = m_state.p[r].format;
+			m_state.p[r] = t.p[r];
+			m_state.p[r].format = keep;
+		});
+	}
+
+	// graphbuilder.cpp:761-796 with UnresizeConversion::create (unresize/unresize.cpp:36-88).
+	void unresize_plane(const State &t, const Mask &mask, int q)
+	{
+		const PlaneState &sp = m_state.p[q], &dp = t.p[q];
+		if (sp.width != sp.aw || sp.height != sp.ah || dp.width != dp.aw || dp.height != dp.ah)
+			fail(ZIMG_ERROR_RESAMPLING_NOT_AVAILABLE, "unresize not supported for for given subregion");
+		const double scale_w = static_cast<double>(dp.aw) / sp.aw;
+		const double scale_h = static_cast<double>(dp.ah) / sp.ah;
+		const double shift_w = sp.left - dp.left / scale_w;
+		const double shift_h = sp.top - dp.top / scale_h;
+		if (sp.format.type != PixelType::FLOAT)
+			internal("unresize format");
+
+		char buf[160];
+		std::snprintf(buf, sizeof(buf), "unresize[%d]: [%u, %u] => [%u, %u] (%f, %f)", q, sp.width, sp.height, dp.width, dp.height,
+		              shift_w, shift_h);
+		trace(buf);
+
+		const unsigned uw = sp.width, uh = sp.height, ow = dp.width, oh = dp.height;
+		const bool skip_h = uw == ow && shift_w == 0;
+		const bool skip_v = uh == oh && shift_h == 0;
+		if (skip_h && !skip_v) {
+			add_unresize_pass(mask, false, uw, uh, oh, shift_h);
+		} else if (skip_v && !skip_h) {
+			add_unresize_pass(mask, true, uw, uh, ow, shift_w);
+		} else if (!skip_h && !skip_v) {
+			const double xs = static_cast<double>(ow) / uw, ys = static_cast<double>(oh) / uh;
+			const double h_first_cost = std::max(xs, 1.0) * 2.0 + xs * std::max(ys, 1.0);
+			const double v_first_cost = std::max(ys, 1.0) + ys * std::max(xs, 1.0) * 2.0;
+			if (h_first_cost < v_first_cost) {
+				add_unresize_pass(mask, true, uw, uh, ow, shift_w);
+				add_unresize_pass(mask, false, ow, uh, oh, shift_h);
+			} else {
+				add_unresize_pass(mask, false, uw, uh, oh, shift_h);
+				add_unresize_pass(mask, true, uw, oh, ow, shift_w);
 			}
 		}
 		each(mask, [&](int r) {

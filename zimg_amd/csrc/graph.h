@@ -46,11 +46,14 @@ struct BuildParams {
 	bool approximate_gamma = false;
 	bool scene_referred = false;
 	bool chromatic_adaptation = false;
+	bool unresize = false; // resample_filter ZIMG_RESIZE_UNRESIZE (zimg.cpp:21,322,450)
 };
 
 // PREMUL / UNPREMUL: FLOAT colour plane (dep 0) times / divided by the FLOAT
 // alpha plane (dep 1), PremultiplyFilter / UnpremultiplyFilter (simple_filters.cpp:76-110).
-enum class NodeKind : int { SOURCE, DEPTH, RESIZE_H, RESIZE_V, COLORSPACE, COPY_RECT, FILL, PREMUL, UNPREMUL };
+// UNRESIZE_H / UNRESIZE_V: FLOAT unresize of one dimension (UnresizeImplH / V,
+// unresize/unresize_impl.cpp:27-156).
+enum class NodeKind : int { SOURCE, DEPTH, RESIZE_H, RESIZE_V, COLORSPACE, COPY_RECT, FILL, PREMUL, UNPREMUL, UNRESIZE_H, UNRESIZE_V };
 
 // A node produces one plane (COLORSPACE produces three; consumers name the output index).
 struct Node {
@@ -70,6 +73,8 @@ struct Node {
 
 	// RESIZE_H / RESIZE_V
 	std::shared_ptr<FilterTable> table;
+	// UNRESIZE_H / UNRESIZE_V
+	std::shared_ptr<UnresizeTable> utable;
 	unsigned depth_bits = 16;       // WORD pixel maximum = 2^depth_bits - 1
 
 	// COLORSPACE

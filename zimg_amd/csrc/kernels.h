@@ -137,6 +137,39 @@ struct CspArg {
 	CspProgram prog;
 };
 
+// Unresize of one dimension (unresize_impl.cpp:27-156), FLOAT planes:
+// horizontal along rows (one thread per row), vertical down columns (one
+// thread per column). Output sample j: the band coef[j][0..row_size) from
+// input sample offset[j], forward substitution with c and the inverted
+// diagonal l, then back substitution with u.
+struct UnresizeArg {
+	PlaneArg src, dst;
+	unsigned frames;
+	unsigned src_w, src_h, dst_w, dst_h;
+	int horizontal;
+	unsigned row_size;
+	const float *coef;
+	const unsigned *offset;
+	const float *c, *l, *u;
+};
+
+// Device pack / unpack between a planar plane and its samples inside a
+// packed layout (NV12 / P010 semi-planar chroma, YUY2 / UYVY 4:2:2): sample
+// j of row i of the planar plane is element step * j + pos of row i of the
+// packed plane, stored there shifted left by `shift` bits.
+struct PackPlane {
+	PlaneArg planar, packed;
+	unsigned width, height;
+	unsigned elem;      // bytes per sample (1 BYTE, 2 WORD)
+	unsigned step, pos, shift;
+};
+struct PackArg {
+	PackPlane p[4];
+	unsigned planes, frames;
+	int pack;           // 1: planar -> packed, 0: packed -> planar
+};
+int launch_pack(const PackArg &a, void *stream);
+
 // Widest plane the error-diffusion wavefront takes (its LDS ring of rows
 // handed between strips holds at most 1024 floats per wave).
 constexpr unsigned kEdMaxWidth = 16257;
@@ -222,6 +255,7 @@ int launch_point(const PointArg &a, void *stream);
 int launch_colorspace(const CspArg &a, void *stream);
 int launch_error_diffusion(const EdArg &a, void *stream);
 int launch_mfma_resize(const MfmaArg &a, void *stream);
+int launch_unresize(const UnresizeArg &a, void *stream);
 int launch_mfma_fused(const MfmaFusedArg &a, void *stream);
 int launch_resize_imma(const ImmaArg &a, void *stream);
 // Largest register capacity (tap pairs) the persistent batched kernel handles.

@@ -1678,6 +1678,61 @@ __global__ void __launch_bounds__(kThreads) k_colorspace(CspArg a)
 	}
 }
 
+// Unresize (unresize_impl.cpp:27-85): the reference's float order, one row
+// (horizontal) or one column (vertical) per thread; the band coefficients
+// and LU factors of an output sample are the same for every thread, so they
+// are scalar loads.
+__global__ void __launch_bounds__(kThreads) k_unresize_h(UnresizeArg a)
+{
+	const unsigned f = blockIdx.y;
+	const unsigned r = blockIdx.x * kThreads + threadIdx.x;
+	if (r >= a.src_h)
+		return;
+	const float *s = reinterpret_cast<const float *>(row_ptr(frame_ptr(a.src, f), a.src.stride, r));
+	float *d = reinterpret_cast<float *>(row_ptr(frame_ptr_w(a.dst, f), a.dst.stride, r));
+	float z = 0.0f;
+	for (unsigned j = 0; j < a.dst_w; ++j) {
+		const float *cj = a.coef + static_cast<size_t>(j) * a.row_size;
+		const float *x = s + a.offset[j];
+		float accum = 0.0f;
+		for (unsigned k = 0; k < a.row_size; ++k)
+			accum += cj[k] * x[k];
+		z = (accum - a.c[j] * z) * a.l[j];
+		d[j] = z;
+	}
+	float w = 0.0f;
+	for (unsigned j = a.dst_w; j != 0; --j) {
+		w = d[j - 1] - a.u[j - 1] * w;
+		d[j - 1] = w;
+	}
+}
+
+__global__ void __launch_bounds__(kThreads) k_unresize_v(UnresizeArg a)
+{
+	const unsigned f = blockIdx.y;
+	const unsigned j = blockIdx.x * kThreads + threadIdx.x;
+	if (j >= a.src_w)
+		return;
+	const char *src = frame_ptr(a.src, f);
+	char *dst = frame_ptr_w(a.dst, f);
+	float z = 0.0f;
+	for (unsigned i = 0; i < a.dst_h; ++i) {
+		const float *ci = a.coef + static_cast<size_t>(i) * a.row_size;
+		const unsigned top = a.offset[i];
+		float accum = 0.0f;
+		for (unsigned k = 0; k < a.row_size; ++k)
+			accum += ci[k] * reinterpret_cast<const float *>(row_ptr(src, a.src.stride, top + k))[j];
+		z = (accum - a.c[i] * z) * a.l[i];
+		reinterpret_cast<float *>(row_ptr(dst, a.dst.stride, i))[j] = z;
+	}
+	float w = 0.0f;
+	for (unsigned i = a.dst_h; i != 0; --i) {
+		float *d = reinterpret_cast<float *>(row_ptr(dst, a.dst.stride, i - 1));
+		w = d[j] - a.u[i - 1] * w;
+		d[j] = w;
+	}
+}
+
 // Floyd-Steinberg error diffusion (dither.cpp:57-89, 286-349) as a row-skewed
 // wavefront. The plane is cut into strips of 64 rows; strip s belongs to wave
 // s mod kEdWaves and starts at step s D. Lane l of the strip owns row
@@ -1943,6 +1998,58 @@ int launch_point(const PointArg &a, void *stream)
 {
 	dim3 grid((a.width + kThreads - 1) / kThreads, a.height, a.frames);
 	hipLaunchKernelGGL(k_point, grid, dim3(kThreads), 0, static_cast<hipStream_t>(stream), a);
+	return static_cast<int>(hipGetLastError());
+}
+
+// Pack / unpack (grid.y = plane, grid.z = frame): one thread per planar sample.
+__global__ void __launch_bounds__(kThreads) k_pack(PackArg a)
+{
+	const PackPlane &p = a.p[blockIdx.y];
+	const unsigned f = blockIdx.z;
+	const unsigned idx = blockIdx.x * kThreads + threadIdx.x;
+	if (idx >= p.width * p.height)
+		return;
+	const unsigned i = idx / p.width, j = idx - i * p.width;
+	const size_t e = static_cast<size_t>(p.step) * j + p.pos;
+	if (a.pack) {
+		const char *s = row_ptr(frame_ptr(p.planar, f), p.planar.stride, i);
+		char *d = row_ptr(frame_ptr_w(p.packed, f), p.packed.stride, i);
+		if (p.elem == 1)
+			reinterpret_cast<uint8_t *>(d)[e] = static_cast<uint8_t>(reinterpret_cast<const uint8_t *>(s)[j] << p.shift);
+		else
+			reinterpret_cast<uint16_t *>(d)[e] = static_cast<uint16_t>(reinterpret_cast<const uint16_t *>(s)[j] << p.shift);
+	} else {
+		const char *s = row_ptr(frame_ptr(p.packed, f), p.packed.stride, i);
+		char *d = row_ptr(frame_ptr_w(p.planar, f), p.planar.stride, i);
+		if (p.elem == 1)
+			reinterpret_cast<uint8_t *>(d)[j] = static_cast<uint8_t>(reinterpret_cast<const uint8_t *>(s)[e] >> p.shift);
+		else
+			reinterpret_cast<uint16_t *>(d)[j] = static_cast<uint16_t>(reinterpret_cast<const uint16_t *>(s)[e] >> p.shift);
+	}
+}
+
+int launch_pack(const PackArg &a, void *stream)
+{
+	if (!a.frames || !a.planes)
+		return 0;
+	unsigned most = 0;
+	for (unsigned q = 0; q < a.planes; ++q)
+		most = std::max(most, a.p[q].width * a.p[q].height);
+	dim3 grid((most + kThreads - 1) / kThreads, a.planes, a.frames);
+	hipLaunchKernelGGL(k_pack, grid, dim3(kThreads), 0, static_cast<hipStream_t>(stream), a);
+	return static_cast<int>(hipGetLastError());
+}
+
+int launch_unresize(const UnresizeArg &a, void *stream)
+{
+	if (!a.frames)
+		return 0;
+	const unsigned n = a.horizontal ? a.src_h : a.src_w;
+	dim3 grid((n + kThreads - 1) / kThreads, a.frames);
+	if (a.horizontal)
+		hipLaunchKernelGGL(k_unresize_h, grid, dim3(kThreads), 0, static_cast<hipStream_t>(stream), a);
+	else
+		hipLaunchKernelGGL(k_unresize_v, grid, dim3(kThreads), 0, static_cast<hipStream_t>(stream), a);
 	return static_cast<int>(hipGetLastError());
 }
 
