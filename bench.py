@@ -166,10 +166,27 @@ def counters_of(key):
     return out
 
 
-def measure(torch, name, frames, steps, warmup, precision=0, approx=False):
+def train_chain_surrogates():
+    """The PQ EOTF and HLG OETF surrogates of uhd_pq_to_hlg, validated per bin
+    against the tolerances the 12-bit 0.5 LSB budget gives them (trained
+    before, and outside, the timed region)."""
+    import zimg_amd as z
+    t0 = time.perf_counter()
+    tol_pq, tol_hlg = z.pq_to_hlg_surrogate_tolerances(bits=12, lsb=0.5)
+    pq = z.TransferSurrogate(z.TRANSFER_ST2084, True, tol_pq, True, steps=20000, seed=7)
+    hlg = z.TransferSurrogate(z.TRANSFER_ARIB_B67, False, tol_hlg, False, steps=20000, seed=7)
+    info = {"train_s": round(time.perf_counter() - t0, 2), "tol_pq_rel": tol_pq, "tol_hlg_abs": tol_hlg,
+            "pq_coverage": round(pq.info()["coverage"], 4), "hlg_coverage": round(hlg.info()["coverage"], 4)}
+    return [pq, hlg], info
+
+
+def measure(torch, name, frames, steps, warmup, precision=0, approx=False, surrogates=None):
     g, planes, outs, algo_bytes, dtype = build_workload(name, frames, approx)
     if approx:
         dtype += " (approximate gamma: lookup tables)"
+    if surrogates:
+        g.set_transfer_surrogates(surrogates[0])
+        dtype += " (PQ EOTF and HLG OETF on the learned surrogate, exact fallback)"
     if precision:
         g.set_resize_precision(precision)
         dtype += f" (resize on MFMA, {'fp16' if precision == 1 else 'bf16'} hi+lo, FP32 accumulate)"
@@ -186,6 +203,8 @@ def measure(torch, name, frames, steps, warmup, precision=0, approx=False):
     key = name if not precision else f"{name}_{'fp16' if precision == 1 else 'bf16'}"
     if approx:
         key += "_approx"
+    if surrogates:
+        key += "_learned"
     ctr = counters_of(key)
     dom_ms = stage_ms[k]
     achieved = algo_bytes * frames / (dom_ms * 1e-3) / 1e9
@@ -199,8 +218,11 @@ def measure(torch, name, frames, steps, warmup, precision=0, approx=False):
         roof["binding"] = {kk: ctr[kk] for kk in ("binding", "valu_insts_per_px", "valu_issue_frac", "mfma_busy_frac") if kk in ctr}
     del keep
     torch.cuda.empty_cache()
-    return {"fps": round(frames * steps / elapsed, 1), "ms_per_step": round(per_step * 1e3, 4), "frames": frames, "launches": len(stages),
-            "dtype": dtype, "roofline": roof}
+    r = {"fps": round(frames * steps / elapsed, 1), "ms_per_step": round(per_step * 1e3, 4), "frames": frames, "launches": len(stages),
+         "dtype": dtype, "roofline": roof}
+    if surrogates:
+        r["surrogates"] = surrogates[1]
+    return r
 
 
 def measure_batch(torch, steps, warmup):
@@ -352,6 +374,10 @@ def main():
                 results[name]["mfma_fp16"] = measure(torch, name, frames, args.steps, args.warmup, precision=1)
             if name == "fhd_709_to_2020":
                 results[name]["approximate_gamma"] = measure(torch, name, frames, args.steps, args.warmup, approx=True)
+            if name == "uhd_pq_to_hlg":
+                learned = measure(torch, name, frames, args.steps, args.warmup, surrogates=train_chain_surrogates())
+                learned["gain_vs_exact"] = round(learned["fps"] / results[name]["fps"], 3)
+                results[name]["learned"] = learned
         if not args.no_cpu:
             results[name]["cpu_baseline"] = cpu_baseline(name, args.cpu_runs, full=name in ("fhd_709_to_2020", "uhd_lanczos"))
         print(f"# {name}: {results[name]['fps']} {results[name].get('unit', 'frames/s')}", file=sys.stderr, flush=True)
@@ -373,7 +399,7 @@ def main():
         "config": {"workload": primary, "frames_per_step": m["frames"], "graph_launches": m["launches"], "precision": "exact"},
         "roofline": m["roofline"],
         "cpu_baseline": m.get("cpu_baseline"),
-        "variants": {k: m[k] for k in ("approximate_gamma", "mfma_fp16") if k in m},
+        "variants": {k: m[k] for k in ("approximate_gamma", "mfma_fp16", "learned") if k in m},
         "configs": {k: v for k, v in results.items() if k != primary},
     }
     print(json.dumps(line))

@@ -567,6 +567,20 @@ zimg_error_code_e zimg_transfer_surrogate_eval(const float *params, const float 
 ZIMG_VISIBILITY
 void zimg_transfer_surrogate_free(zimg_transfer_surrogate *s);
 
+/* device extension: route a graph's transfer functions through learned
+ * surrogates (count of them, at most one per transfer function and
+ * direction; count 0 restores the exact transfers). Each colourspace
+ * conversion's gamma operation whose function has a surrogate, and the HLG
+ * OETF that follows the inverse OOTF (with to_linear 0 for ARIB_B67), run
+ * through the surrogate kernel with its exact fallback, between colourspace
+ * launches, in three FLOAT planes of the device tmp buffer (size tmp after
+ * this call). The reference has no counterpart: its GammaOperationC
+ * (colorspace/operation_impl.cpp:53-77) evaluates every transfer exactly.
+ * The surrogates must outlive every call that uses them and live on the
+ * device that processes. */
+ZIMG_VISIBILITY
+zimg_error_code_e zimg_filter_graph_set_transfer_surrogates(zimg_filter_graph *ptr, const zimg_transfer_surrogate *const *s, unsigned count);
+
 /* device extension: a short text description of the lowered device stages
  * (the analogue of the reference's FilterObserver trace, graphbuilder.h:45-64).
  * Writes at most n bytes including the terminator; returns the full length. */

@@ -67,7 +67,8 @@ EXPORTS = [
     "zimg_filter_graph_get_device_tmp_size", "zimg_filter_graph_process_device",
     "zimg_filter_graph_process_device_batch", "zimg_filter_graph_process_device_batch_noalloc", "zimg_filter_graph_describe",
     "zimg_filter_graph_get_device_packed_tmp_size", "zimg_filter_graph_process_device_packed",
-    "zimg_filter_graph_set_resize_precision", "zimg_filter_graph_process_device_profiled",
+    "zimg_filter_graph_set_resize_precision", "zimg_filter_graph_set_transfer_surrogates",
+    "zimg_filter_graph_process_device_profiled",
     "zimg_device_jobs_get_tmp_size", "zimg_device_jobs_create", "zimg_device_jobs_run", "zimg_device_jobs_describe",
     "zimg_device_jobs_free",
     "zimg_transfer_surrogate_train", "zimg_transfer_surrogate_apply", "zimg_transfer_exact_apply", "zimg_transfer_surrogate_info",
@@ -208,6 +209,8 @@ def lib():
                                                                 ctypes.POINTER(ctypes.c_uint)]
         L.zimg_filter_graph_set_resize_precision.restype = ctypes.c_int
         L.zimg_filter_graph_set_resize_precision.argtypes = [vp, ctypes.c_int]
+        L.zimg_filter_graph_set_transfer_surrogates.restype = ctypes.c_int
+        L.zimg_filter_graph_set_transfer_surrogates.argtypes = [vp, ctypes.POINTER(vp), ctypes.c_uint]
         L.zimg_device_jobs_get_tmp_size.restype = ctypes.c_int
         L.zimg_device_jobs_get_tmp_size.argtypes = [ctypes.POINTER(vp), ctypes.c_uint, ctypes.POINTER(sz)]
         L.zimg_device_jobs_create.restype = vp
@@ -358,6 +361,15 @@ class FilterGraph:
     def set_resize_precision(self, mode):
         check(lib().zimg_filter_graph_set_resize_precision(self._h, mode))
 
+    def set_transfer_surrogates(self, surrogates):
+        """zimg_filter_graph_set_transfer_surrogates; the graph keeps the
+        TransferSurrogate objects alive while it uses them ([] restores the
+        exact transfers)."""
+        surrogates = list(surrogates)
+        arr = (ctypes.c_void_p * max(len(surrogates), 1))(*[s._h for s in surrogates])
+        check(lib().zimg_filter_graph_set_transfer_surrogates(self._h, arr, len(surrogates)))
+        self._surrogates = surrogates
+
     def describe(self):
         n = lib().zimg_filter_graph_describe(self._h, None, 0)
         buf = ctypes.create_string_buffer(n + 1)
@@ -479,6 +491,24 @@ class TransferSurrogate:
         if h and _lib is not None:
             _lib.zimg_transfer_surrogate_free(h)
             self._h = None
+
+
+def pq_to_hlg_surrogate_tolerances(bits=12, lsb=0.5):
+    """Per-bin tolerances of the two surrogates of an RGB PQ -> HLG conversion
+    (PQ EOTF, relative; HLG OETF, absolute) from an output error budget of
+    `lsb` codes at `bits` bits, split evenly between the two and halved again
+    for the values between a bin's validation points.
+
+    The HLG stage is E' = OETF(E * Y^((1 - g) / g)), g = 1.2, Y a positive
+    combination of E's channels: a relative error e in E moves the scaled
+    value by at most e (1 + (g - 1) / g) = 7e/6 relative, and OETF moves by
+    at most 0.25 per unit of log E (sqrt(3E) / 2 below 1/12, where it reaches
+    0.25; 12aE / (12E - b) above, falling from 0.25 to 0.183 at E = 1). So a
+    relative PQ error e costs at most 0.2917 e (2^bits - 1) codes, and an
+    absolute OETF error d costs d (2^bits - 1) codes."""
+    codes = float((1 << bits) - 1)
+    share = lsb / 2.0 / 2.0
+    return share / (0.25 * 7.0 / 6.0 * codes), share / codes
 
 
 def transfer_exact_apply(transfer, to_linear, ptr, n, pre=1.0, post=1.0, stream=None):

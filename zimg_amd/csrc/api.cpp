@@ -993,6 +993,23 @@ zimg_error_code_e zimg_filter_graph_set_resize_precision(zimg_filter_graph *ptr,
 	}
 }
 
+zimg_error_code_e zimg_filter_graph_set_transfer_surrogates(zimg_filter_graph *ptr, const zimg_transfer_surrogate *const *s, unsigned count)
+{
+	try {
+		if (!ptr || !ptr->prog)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null graph");
+		if (count && !s)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null surrogate list");
+		std::vector<const zamd::learned::Surrogate *> v;
+		for (unsigned k = 0; k < count; ++k)
+			v.push_back(reinterpret_cast<const zamd::learned::Surrogate *>(s[k]));
+		ptr->prog->set_transfer_surrogates(v);
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
 zimg_error_code_e zimg_device_jobs_get_tmp_size(const zimg_filter_graph *const *graphs, unsigned count, size_t *out)
 {
 	try {

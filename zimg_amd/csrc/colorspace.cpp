@@ -466,6 +466,7 @@ const char *op_name(int k)
 	case CSP_LUT_GAMMA: return "lut_gamma";
 	case CSP_GAMMA: return "gamma";
 	case CSP_B67: return "hlg_oetf_ootf";
+	case CSP_B67_SCALE: return "hlg_ootf_scale";
 	case CSP_B67_INV: return "hlg_eotf";
 	case CSP_CL_TO_RGB: return "cl_to_rgb";
 	case CSP_CL_TO_YUV: return "cl_to_yuv";

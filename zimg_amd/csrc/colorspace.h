@@ -17,8 +17,12 @@ namespace zamd {
 // product and two fused multiply-adds; inverse gamma from a 2^16 + 1 entry
 // table over [-0.5, 1.5]; gamma from a 2^16 entry table indexed by the
 // binary16 bits of the value.
+//
+// CSP_B67_SCALE is the first half of CSP_B67 (the inverse OOTF scaling by
+// Ys^((1 - gamma) / gamma), without the OETF): a program splits CSP_B67 into
+// it and a learned OETF when a transfer surrogate takes over the OETF.
 enum CspOpKind : int { CSP_MATRIX = 0, CSP_GAMMA = 1, CSP_B67 = 2, CSP_B67_INV = 3, CSP_CL_TO_RGB = 4, CSP_CL_TO_YUV = 5,
-                       CSP_MATRIX_FMA = 6, CSP_LUT_LINEAR = 7, CSP_LUT_GAMMA = 8 };
+                       CSP_MATRIX_FMA = 6, CSP_LUT_LINEAR = 7, CSP_LUT_GAMMA = 8, CSP_B67_SCALE = 9 };
 
 // Entries of the lookup tables of CSP_LUT_LINEAR and CSP_LUT_GAMMA.
 constexpr unsigned kLutLinearSize = 65537, kLutGammaSize = 65536;

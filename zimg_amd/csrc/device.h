@@ -12,6 +12,7 @@
 
 #include "graph.h"
 #include "kernels.h"
+#include "learned.h"
 
 namespace zamd {
 
@@ -93,6 +94,15 @@ public:
 	// sizing tmp and processing; not thread-safe against concurrent processing.
 	void set_resize_precision(int mode);
 	int resize_precision() const { return m_precision; }
+	// Learned transfer surrogates (at most one per transfer function): a
+	// colourspace step's CSP_GAMMA operations whose transfer has one, and the
+	// HLG OETF of its CSP_B67 operations, run through the surrogate kernel
+	// (exact fallback outside its validated bins) between colourspace
+	// launches, in three FLOAT planes of tmp. Empty (the default): every
+	// transfer exact. The surrogates must outlive the calls that use them and
+	// live on the device that runs them; set before sizing tmp.
+	void set_transfer_surrogates(const std::vector<const learned::Surrogate *> &s);
+	bool has_transfer_surrogates() const { return !m_surrogates.empty(); }
 	// Enqueue the program for `frames` frames. Throws Error on invalid buffers or launch failure.
 	// events (optional, steps().size() + 1 hipEvent_t): recorded before each step and after the last.
 	void run(const DeviceBuffers &src, const DeviceBuffers &dst, void *tmp, unsigned frames, void *stream,
@@ -153,6 +163,20 @@ private:
 	ImmaPlan imma_plan(const Step &s) const;
 	int m_precision = 0;
 	mutable std::set<int> m_prepared; // HIP device * 4 + precision mode
+
+	std::vector<const learned::Surrogate *> m_surrogates;
+	// One launch of a colourspace step split around its learned transfers.
+	struct CspSegment {
+		const learned::Surrogate *sur = nullptr; // null: a colourspace launch of prog
+		CspProgram prog;
+		float pre = 1.0f, post = 1.0f;
+	};
+	// The launches of a CSP step (luts: the step's tables patched in); empty
+	// when no surrogate applies to it.
+	std::vector<CspSegment> csp_segments(const Step &s, const CspProgram &prog) const;
+	size_t surrogate_bytes(const Step &s, unsigned frames) const;
+	void run_learned_csp(const Step &s, const std::vector<CspSegment> &segs, const dev::PlaneArg *src, const dev::PlaneArg *dst,
+	                     char *mid, unsigned frames, void *stream) const;
 
 	bool mfma_step(const Step &s) const { return m_precision && s.kind == Step::RESIZE && !s.word && s.ox[0].dither == 0; }
 	// Launch shape of the one-launch two-pass matrix-core resize (ok == false:

@@ -13,6 +13,7 @@
 #include <cstring>
 
 #include "device.h"
+#include "zmath.h"
 
 namespace zamd {
 
@@ -625,10 +626,128 @@ size_t DeviceProgram::tmp_size(unsigned frames) const
 	size_t total = 0;
 	for (auto &sl : m_slots)
 		total += align_up(sl.pitch * sl.height * frames, kPitchAlign);
-	for (auto &s : m_steps)
+	for (auto &s : m_steps) {
 		if (!s.dead && mfma_step(s))
 			total += mfma_mid_bytes(s, frames);
+		total += surrogate_bytes(s, frames);
+	}
 	return total;
+}
+
+void DeviceProgram::set_transfer_surrogates(const std::vector<const learned::Surrogate *> &s)
+{
+	std::set<int> funcs;
+	for (const learned::Surrogate *m : s) {
+		if (!m)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null transfer surrogate");
+		if (!funcs.insert(m->func()).second)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "two surrogates for one transfer function");
+	}
+	m_surrogates = s;
+}
+
+std::vector<DeviceProgram::CspSegment> DeviceProgram::csp_segments(const Step &s, const CspProgram &prog) const
+{
+	std::vector<CspSegment> segs;
+	if (m_surrogates.empty() || s.dead || s.kind != Step::CSP)
+		return segs;
+	auto find = [&](int func) -> const learned::Surrogate * {
+		for (const learned::Surrogate *m : m_surrogates)
+			if (m->func() == func)
+				return m;
+		return nullptr;
+	};
+	bool any = false;
+	CspSegment cur;
+	auto flush = [&]() {
+		if (cur.prog.n) {
+			segs.push_back(cur);
+			cur = CspSegment{};
+		}
+	};
+	for (int k = 0; k < prog.n; ++k) {
+		const CspOp &op = prog.op[k];
+		const learned::Surrogate *m = op.kind == CSP_GAMMA ? find(op.func) : op.kind == CSP_B67 ? find(zm::F_B67_OETF) : nullptr;
+		if (!m) {
+			cur.prog.op[cur.prog.n++] = op;
+			continue;
+		}
+		any = true;
+		if (op.kind == CSP_B67) {
+			// inverse OOTF scaling stays a colourspace operation; the OETF
+			// applied to its result (unit scales) is the learned part
+			CspOp scale = op;
+			scale.kind = CSP_B67_SCALE;
+			cur.prog.op[cur.prog.n++] = scale;
+			flush();
+			segs.push_back(CspSegment{ m, CspProgram{}, 1.0f, 1.0f });
+		} else {
+			flush();
+			segs.push_back(CspSegment{ m, CspProgram{}, op.pre, op.post });
+		}
+	}
+	flush();
+	if (!any)
+		segs.clear();
+	return segs;
+}
+
+size_t DeviceProgram::surrogate_bytes(const Step &s, unsigned frames) const
+{
+	if (csp_segments(s, s.prog).empty())
+		return 0;
+	const size_t pitch = align_up(static_cast<size_t>(s.width) * 4, kPitchAlign);
+	return 3 * align_up(pitch * s.height * frames, kPitchAlign);
+}
+
+void DeviceProgram::run_learned_csp(const Step &s, const std::vector<CspSegment> &segs, const dev::PlaneArg *src,
+                                    const dev::PlaneArg *dst, char *mid, unsigned frames, void *stream) const
+{
+	// Intermediates: three FLOAT planes; colourspace and surrogate launches
+	// are elementwise, so they run in place there.
+	const size_t pitch = align_up(static_cast<size_t>(s.width) * 4, kPitchAlign);
+	const size_t plane = align_up(pitch * s.height * frames, kPitchAlign);
+	dev::PlaneArg t[3];
+	for (int p = 0; p < 3; ++p)
+		t[p] = dev::PlaneArg{ mid + p * plane, static_cast<long long>(pitch), static_cast<long long>(pitch * s.height) };
+	const dev::InXform plain_in{ 3, 0, 0, 1.0f, 0.0f };
+	const dev::OutXform plain_out{ 3, 0, 0, 0, 32, 1.0f, 0.0f };
+	bool in_plain = true, out_plain = true;
+	for (int p = 0; p < 3; ++p) {
+		in_plain = in_plain && s.ix[p].type == 3 && s.ix[p].op == 0;
+		out_plain = out_plain && s.ox[p].type == 3 && s.ox[p].op == 0;
+	}
+	std::vector<CspSegment> list = segs;
+	if (list.front().sur && !in_plain)
+		list.insert(list.begin(), CspSegment{});
+	if (list.back().sur && !out_plain)
+		list.push_back(CspSegment{});
+	for (size_t i = 0; i < list.size(); ++i) {
+		const bool first = i == 0, last = i + 1 == list.size();
+		const dev::PlaneArg *in = first ? src : t;
+		const dev::PlaneArg *out = last ? dst : t;
+		const CspSegment &g = list[i];
+		if (g.sur) {
+			learned::PlaneView v[3];
+			for (int p = 0; p < 3; ++p)
+				v[p] = learned::PlaneView{ in[p].ptr, out[p].ptr, in[p].stride, in[p].fstride, out[p].stride, out[p].fstride };
+			learned::apply_planes(g.sur->device_model(g.pre, g.post), v, 3, s.width, s.height, frames, stream);
+			continue;
+		}
+		dev::CspArg a{};
+		for (int p = 0; p < 3; ++p) {
+			a.src[p] = in[p];
+			a.dst[p] = out[p];
+			a.in[p] = first ? s.ix[p] : plain_in;
+			a.out[p] = last ? s.ox[p] : plain_out;
+		}
+		a.frames = frames;
+		a.width = s.width;
+		a.height = s.height;
+		a.prog = g.prog;
+		if (const int rc = dev::launch_colorspace(a, stream))
+			fail(ZIMG_ERROR_UNKNOWN, std::string("device launch failed: ") + hipGetErrorString(static_cast<hipError_t>(rc)));
+	}
 }
 
 void DeviceProgram::set_resize_precision(int mode)
@@ -1353,6 +1472,13 @@ void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void
 			off += mfma_mid_bytes(m_steps[k], frames);
 		}
 	}
+	std::vector<size_t> sur_off(m_steps.size(), 0);
+	for (size_t k = 0; k < m_steps.size(); ++k) {
+		if (const size_t b = surrogate_bytes(m_steps[k], frames)) {
+			sur_off[k] = off;
+			off += b;
+		}
+	}
 	if (off && !tmp)
 		fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "device tmp buffer required");
 
@@ -1455,6 +1581,11 @@ void DeviceProgram::run(const DeviceBuffers &src, const DeviceBuffers &dst, void
 			for (int k = 0; k < a.prog.n; ++k)
 				if (a.prog.op[k].kind == CSP_LUT_LINEAR || a.prog.op[k].kind == CSP_LUT_GAMMA)
 					a.prog.op[k].lut = tabs.lut.at(static_cast<int>(si) * 8 + k);
+			const std::vector<CspSegment> segs = csp_segments(s, a.prog);
+			if (!segs.empty()) {
+				run_learned_csp(s, segs, a.src, a.dst, static_cast<char *>(tmp) + sur_off[si], frames, stream);
+				break;
+			}
 			rc = dev::launch_colorspace(a, stream);
 			break;
 		}
@@ -1512,6 +1643,15 @@ std::string DeviceProgram::describe() const
 		else
 			std::snprintf(buf, sizeof(buf), "%s %ux%u: %s\n", kinds[s.kind], s.width, s.height, s.label.c_str());
 		r += buf;
+		if (s.kind == Step::CSP) {
+			const std::vector<CspSegment> segs = csp_segments(s, s.prog);
+			if (!segs.empty()) {
+				std::string t = "  learned transfers, launches:";
+				for (const CspSegment &g : segs)
+					t += g.sur ? " surrogate(f" + std::to_string(g.sur->func()) + ")" : " colorspace[" + g.prog.describe() + "]";
+				r += t + "\n";
+			}
+		}
 		if (s.kind == Step::RESIZE && s.imma.ok) {
 			std::snprintf(buf, sizeof(buf), "  integer matrix cores: one launch, %ux%u tiles, lds=%u\n", dev::kImmaTileW, 16 * s.imma.rb,
 			              s.imma.lds);

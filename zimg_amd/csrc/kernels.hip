@@ -1581,6 +1581,18 @@ __device__ __forceinline__ void apply_op(const CspOp &op, float *a, float *b, fl
 			c[p] = zm::arib_b67_oetf(bl * ys_inv);
 		}
 		break;
+	case CSP_B67_SCALE:
+#pragma unroll
+		for (int p = 0; p < NP; ++p) {
+			const float gamma = 1.2f;
+			const float r = a[p] * op.pre, g = b[p] * op.pre, bl = c[p] * op.pre;
+			const float yd = zm::maxf(op.kr * r + op.kg * g + op.kb * bl, zm::FLT_MIN_);
+			const float ys_inv = zm::powf_(yd, (1.0f - gamma) / gamma);
+			a[p] = r * ys_inv;
+			b[p] = g * ys_inv;
+			c[p] = bl * ys_inv;
+		}
+		break;
 	case CSP_B67_INV:
 #pragma unroll
 		for (int p = 0; p < NP; ++p) {

@@ -64,6 +64,17 @@ private:
 	void validate(void *stream);
 };
 
+// dst[p] = post * f(src[p] * pre) over whole planes (up to 3, FLOAT, frames
+// of a batch frame_stride bytes apart): the surrogate where its bins are
+// validated, the exact transfer elsewhere. src and dst may be the same.
+struct PlaneView {
+	const char *src;
+	char *dst;
+	long long src_stride, src_fstride, dst_stride, dst_fstride;
+};
+void apply_planes(const DeviceModel &m, const PlaneView *planes, unsigned nplanes, unsigned width, unsigned height, unsigned frames,
+                  void *stream);
+
 // Internal transfer id of a public transfer characteristic and direction.
 int function_for(int transfer, int to_linear);
 // FP64 value of an internal transfer function (training targets, judging).

@@ -119,6 +119,21 @@ __device__ __forceinline__ float output_of(const float *p, const WaveLds &w, uns
 	return v;
 }
 
+// Network output of sample lane % kTile after forward_tile, in every lane of
+// the wave (call from wave-uniform control flow): the two halves of the
+// wave each sum 16 of the 32 hidden units of a sample, then exchange.
+__device__ __forceinline__ float infer_output(const float *p, const WaveLds &w, unsigned lane)
+{
+	const unsigned s = lane % kTile, m0 = (lane / kTile) * (H / 2);
+	float v = 0.0f;
+#pragma unroll 8
+	for (unsigned m = 0; m < H / 2; ++m) {
+		const float z = w.z2[(m0 + m) * kPitch + s];
+		v += p[OFF_W3 + m0 + m] * (z * sigmoid(z));
+	}
+	return p[OFF_B3] + (v + __shfl_xor(v, 32));
+}
+
 // One training pass over `tiles` wave tiles per wave: gradient of
 // sum_s (v_s - t_s)^2 written as this workgroup's partial gradient, and the
 // partial loss. Samples of workgroup b, wave w, tile i start at
@@ -313,17 +328,90 @@ __global__ void __launch_bounds__(64 * kWaves) k_surrogate_apply(float *__restri
 		__builtin_amdgcn_wave_barrier();
 		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 		// The network runs only for tiles with at least one validated value.
-		if (__builtin_amdgcn_ballot_w64(use) != 0)
+		float net = 0.0f;
+		if (__builtin_amdgcn_ballot_w64(use) != 0) {
 			forward_tile(p, w, lane);
+			net = infer_output(p, w, lane);
+		}
 		if (lane < kTile && s < n) {
 			float y;
 			if (use) {
-				const float v = output_of(p, w, lane) * m.out_scale + m.out_offset;
+				const float v = net * m.out_scale + m.out_offset;
 				y = m.post * (m.out_log ? __builtin_amdgcn_exp2f(v) : v);
 			} else {
 				y = m.post * zm::transfer(m.func, xin * m.pre);
 			}
 			x[s] = y;
+		}
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	}
+}
+
+// The same over up to three pitched FLOAT planes of a batch (the surrogate
+// form of a colourspace gamma operation): sample s of the flat index
+// (plane, frame, row, column) space.
+struct SurrogatePlanes {
+	learned::PlaneView p[3];
+	unsigned nplanes, width, height, frames;
+	learned::DeviceModel m;
+};
+
+__global__ void __launch_bounds__(64 * kWaves) k_surrogate_planes(SurrogatePlanes a)
+{
+	zm::lds_tables_init();
+	__shared__ float p[NPARAM];
+	__shared__ WaveLds wl[kWaves];
+	const learned::DeviceModel &m = a.m;
+	const unsigned tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+	for (unsigned k = tid; k < NPARAM; k += 64 * kWaves)
+		p[k] = m.params[k];
+	__syncthreads();
+	WaveLds &w = wl[wv];
+	const uint64_t per_row = a.width, per_frame = per_row * a.height, per_plane = per_frame * a.frames;
+	const uint64_t n = per_plane * a.nplanes;
+	const uint64_t waves = static_cast<uint64_t>(gridDim.x) * kWaves;
+	for (uint64_t base = (static_cast<uint64_t>(blockIdx.x) * kWaves + wv) * kTile; base < n; base += waves * kTile) {
+		const uint64_t s = base + lane;
+		float xin = 0.0f, tin = 0.0f;
+		bool use = false;
+		const float *src = nullptr;
+		float *dst = nullptr;
+		if (lane < kTile && s < n) {
+			const unsigned q = static_cast<unsigned>(s / per_plane);
+			const uint64_t r0 = s - q * per_plane;
+			const unsigned f = static_cast<unsigned>(r0 / per_frame);
+			const uint64_t r1 = r0 - f * per_frame;
+			const unsigned row = static_cast<unsigned>(r1 / per_row), col = static_cast<unsigned>(r1 - row * per_row);
+			const learned::PlaneView &v = a.p[q];
+			src = reinterpret_cast<const float *>(v.src + f * v.src_fstride + row * v.src_stride) + col;
+			dst = reinterpret_cast<float *>(v.dst + f * v.dst_fstride + row * v.dst_stride) + col;
+			xin = *src;
+			const float xs = xin * m.pre;
+			tin = m.in_log ? __builtin_amdgcn_logf(xs) : xs;
+			const float pos = (tin - m.lo) * m.inv_width;
+			if (xs > 0.0f && pos >= 0.0f && pos < 1.0f) {
+				const unsigned bin = min(static_cast<unsigned>(pos * m.nbins), m.nbins - 1);
+				use = m.valid[bin] != 0;
+			}
+			w.u[lane] = use ? 2.0f * pos - 1.0f : 0.0f;
+		}
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		float net = 0.0f;
+		if (__builtin_amdgcn_ballot_w64(use) != 0) {
+			forward_tile(p, w, lane);
+			net = infer_output(p, w, lane);
+		}
+		if (dst) {
+			float y;
+			if (use) {
+				const float v = net * m.out_scale + m.out_offset;
+				y = m.post * (m.out_log ? __builtin_amdgcn_exp2f(v) : v);
+			} else {
+				y = m.post * zm::transfer(m.func, xin * m.pre);
+			}
+			*dst = y;
 		}
 		__builtin_amdgcn_wave_barrier();
 		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -357,8 +445,9 @@ __global__ void __launch_bounds__(64 * kWaves) k_mlp_eval(const float *__restric
 		__builtin_amdgcn_wave_barrier();
 		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 		forward_tile(p, w, lane);
+		const float out = infer_output(p, w, lane);
 		if (lane < kTile && s < n)
-			v[s] = output_of(p, w, lane);
+			v[s] = out;
 		__builtin_amdgcn_wave_barrier();
 		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 	}
@@ -612,6 +701,23 @@ void apply_exact(int func, float *x, size_t n, float pre, float post, void *stre
 	hipLaunchKernelGGL(k_transfer_apply, dim3(4096), dim3(256), 0, static_cast<hipStream_t>(stream), x, static_cast<unsigned>(n), func, pre,
 	                   post);
 	check_hip(hipGetLastError(), "transfer launch");
+}
+
+void apply_planes(const DeviceModel &m, const PlaneView *planes, unsigned nplanes, unsigned width, unsigned height, unsigned frames,
+                  void *stream)
+{
+	if (nplanes > 3)
+		fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "at most three planes");
+	SurrogatePlanes a{};
+	for (unsigned q = 0; q < nplanes; ++q)
+		a.p[q] = planes[q];
+	a.nplanes = nplanes;
+	a.width = width;
+	a.height = height;
+	a.frames = frames;
+	a.m = m;
+	hipLaunchKernelGGL(k_surrogate_planes, dim3(4096), dim3(64 * kWaves), 0, static_cast<hipStream_t>(stream), a);
+	check_hip(hipGetLastError(), "surrogate launch");
 }
 
 void Surrogate::get_params(float *out) const
