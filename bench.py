@@ -28,7 +28,7 @@ path) on this host: single-thread median of >= 10 runs, and frame-parallel
 over the usable cores (one frame per thread, the reference harness pattern of
 graphapp.cpp:355-426), with the CPU model.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME [--variant FORM]] [--no-cpu]
 """
 import argparse
 import json
@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--frames", type=int, default=None, help="distinct frames per step (default 64)")
     ap.add_argument("--workload", default=None, choices=WORKLOADS,
                     help="measure one workload only (its line becomes the primary); default: all, fhd_709_to_2020 primary")
+    ap.add_argument("--variant", default="all", choices=["all", "exact", "fp16", "bf16", "approx", "learned"],
+                    help="with --workload: measure only this form (exact path, FP16/BF16 matrix-core resize, "
+                         "approximate gamma, learned transfers); default: the exact path and every form the workload has")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--cpu-runs", type=int, default=10, help="runs per CPU baseline mode (median reported)")
     return ap.parse_args()
@@ -352,6 +355,12 @@ def cpu_baseline(name, runs, full):
     return out
 
 
+def variant_applies(name, variant):
+    """Whether a workload has the form `variant` (the exact path always)."""
+    return variant in ("all", "exact") or (variant in ("fp16", "bf16") and name in ("fp32_down", "fp32_up")) or \
+        (variant == "approx" and name == "fhd_709_to_2020") or (variant == "learned" and name == "uhd_pq_to_hlg")
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -365,11 +374,21 @@ def main():
     frames = args.frames or 64
     todo = [args.workload] if args.workload else WORKLOADS
     results = {}
+    only = args.variant if args.workload else "all"
+    if only != "all" and not variant_applies(args.workload, only):
+        print(json.dumps({"error": f"workload {args.workload} has no {only} form"}))
+        sys.exit(2)
     for name in todo:
-        if name == "batch64":
+        if only != "all" and only != "exact":
+            kw = {"fp16": {"precision": 1}, "bf16": {"precision": 2}, "approx": {"approx": True}}.get(only, {})
+            if only == "learned":
+                kw = {"surrogates": train_chain_surrogates()}
+            results[name] = measure(torch, name, frames, args.steps, args.warmup, **kw)
+        elif name == "batch64":
             results[name] = measure_batch(torch, args.steps, args.warmup)
         else:
             results[name] = measure(torch, name, frames, args.steps, args.warmup)
+        if only == "all" and name != "batch64":
             if name in ("fp32_down", "fp32_up"):
                 results[name]["mfma_fp16"] = measure(torch, name, frames, args.steps, args.warmup, precision=1)
             if name == "fhd_709_to_2020":
@@ -396,7 +415,8 @@ def main():
         "vs_baseline": None,
         "dtype": m["dtype"],
         "data": "synthetic (seeded, zimg_amd/workloads.py), inputs resident in HBM",
-        "config": {"workload": primary, "frames_per_step": m["frames"], "graph_launches": m["launches"], "precision": "exact"},
+        "config": {"workload": primary, "frames_per_step": m["frames"], "graph_launches": m["launches"],
+                   "form": "exact" if only == "all" else only},
         "roofline": m["roofline"],
         "cpu_baseline": m.get("cpu_baseline"),
         "variants": {k: m[k] for k in ("approximate_gamma", "mfma_fp16", "learned") if k in m},

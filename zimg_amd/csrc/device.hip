@@ -1650,6 +1650,8 @@ std::string DeviceProgram::describe() const
 				for (const CspSegment &g : segs)
 					t += g.sur ? " surrogate(f" + std::to_string(g.sur->func()) + ")" : " colorspace[" + g.prog.describe() + "]";
 				r += t + "\n";
+			} else if (dev::colorspace_has_chain_kernel(s.prog)) {
+				r += "  chain kernel: operations fixed at compile time\n";
 			}
 		}
 		if (s.kind == Step::RESIZE && s.imma.ok) {

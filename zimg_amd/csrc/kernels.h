@@ -253,6 +253,9 @@ struct JobTile {
 int launch_resize(const ResizeArg &a, void *stream);
 int launch_point(const PointArg &a, void *stream);
 int launch_colorspace(const CspArg &a, void *stream);
+// Whether launch_colorspace runs the program on a compile-time chain kernel
+// (straight-line code for that operation list) rather than the general one.
+bool colorspace_has_chain_kernel(const CspProgram &prog);
 int launch_error_diffusion(const EdArg &a, void *stream);
 int launch_mfma_resize(const MfmaArg &a, void *stream);
 int launch_unresize(const UnresizeArg &a, void *stream);

@@ -1,5 +1,6 @@
 // gfx950 kernels of the per-plane path: fused two-pass resize, point depth
-// conversions, the fused colourspace stage and the error-diffusion wavefront.
+// conversions, unresize and the error-diffusion wavefront (the colourspace
+// stage is in csp.hip).
 //
 // Arithmetic is the reference's portable C path (resize_impl.cpp:31-114,
 // depth_convert.cpp:23-64, dither.cpp:37-89, operation_impl.cpp:21-285),
@@ -11,138 +12,13 @@
 #include <cstdint>
 #include <type_traits>
 
-#include "bluenoise.h"
+#include "devutil.h"
 #include "kernels.h"
-#include "zmath.h"
 
 namespace zamd {
 namespace dev {
 
 namespace {
-
-constexpr unsigned kThreads = 256;
-
-__device__ __forceinline__ const char *frame_ptr(const PlaneArg &p, unsigned f) { return p.ptr + static_cast<long long>(f) * p.fstride; }
-__device__ __forceinline__ char *frame_ptr_w(const PlaneArg &p, unsigned f) { return p.ptr + static_cast<long long>(f) * p.fstride; }
-
-__device__ __forceinline__ const char *row_ptr(const char *base, long long stride, unsigned r) { return base + static_cast<long long>(r) * stride; }
-__device__ __forceinline__ char *row_ptr(char *base, long long stride, unsigned r) { return base + static_cast<long long>(r) * stride; }
-
-// half_to_float (quantize.cpp:24-77): exact widening; a NaN keeps its payload
-// and is made quiet.
-__device__ __forceinline__ float half_to_float(uint16_t h)
-{
-	if ((h & 0x7C00u) == 0x7C00u && (h & 0x03FFu))
-		return __uint_as_float((static_cast<uint32_t>(h & 0x8000u) << 16) | 0x7FC00000u | (static_cast<uint32_t>(h & 0x03FFu) << 13));
-	return static_cast<float>(__builtin_bit_cast(_Float16, h));
-}
-
-// float_to_half (quantize.cpp:79-144): IEEE binary16 round-to-nearest-even,
-// overflow to infinity; a NaN keeps the top of its payload and is made quiet.
-__device__ __forceinline__ uint16_t float_to_half(float x)
-{
-	const uint32_t u = __float_as_uint(x);
-	if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x007FFFFFu))
-		return static_cast<uint16_t>(((u >> 16) & 0x8000u) | 0x7E00u | ((u & 0x007FFFFFu) >> 13));
-	return __builtin_bit_cast(uint16_t, static_cast<_Float16>(x));
-}
-
-// Storage types: 0 BYTE, 1 WORD, 2 HALF, 3 FLOAT.
-__device__ __forceinline__ float load_as_float(const char *row, unsigned j, int type)
-{
-	if (type == 0) return static_cast<float>(reinterpret_cast<const uint8_t *>(row)[j]);
-	if (type == 1) return static_cast<float>(reinterpret_cast<const uint16_t *>(row)[j]);
-	if (type == 2) return half_to_float(reinterpret_cast<const uint16_t *>(row)[j]);
-	return reinterpret_cast<const float *>(row)[j];
-}
-
-// Raw storage bits (for copies and integer shifts).
-__device__ __forceinline__ unsigned load_as_uint(const char *row, unsigned j, int type)
-{
-	if (type == 0) return reinterpret_cast<const uint8_t *>(row)[j];
-	if (type == 1 || type == 2) return reinterpret_cast<const uint16_t *>(row)[j];
-	return reinterpret_cast<const uint32_t *>(row)[j];
-}
-
-__device__ __forceinline__ void store_uint(char *row, unsigned j, int type, unsigned v)
-{
-	if (type == 0) reinterpret_cast<uint8_t *>(row)[j] = static_cast<uint8_t>(v);
-	else if (type == 1 || type == 2) reinterpret_cast<uint16_t *>(row)[j] = static_cast<uint16_t>(v);
-	else reinterpret_cast<uint32_t *>(row)[j] = v;
-}
-
-// An unquantised value into a floating-point plane (HALF or FLOAT).
-__device__ __forceinline__ void store_float(char *row, unsigned j, int type, float v)
-{
-	if (type == 2) reinterpret_cast<uint16_t *>(row)[j] = float_to_half(v);
-	else reinterpret_cast<float *>(row)[j] = v;
-}
-
-// Bayer 16x16 ordered-dither offset for output pixel (i, j) of plane variant p
-// (dither.cpp:137-217): index bits interleave (row xor col, col) from the top;
-// variants flip horizontally, vertically, or transpose; value (t+1)/257 - 1/2.
-__device__ __forceinline__ float bayer_offset(int plane, unsigned i, unsigned j)
-{
-	unsigned r = i & 15, c = j & 15;
-	switch (plane & 3) {
-	case 1: c = 15 - c; break;
-	case 2: r = 15 - r; break;
-	case 3: { unsigned t = r; r = c; c = t; } break;
-	default: break;
-	}
-	const unsigned x = r ^ c;
-	unsigned v = 0;
-#pragma unroll
-	for (unsigned b = 0; b < 4; ++b) {
-		v |= ((x >> b) & 1u) << (7 - 2 * b);
-		v |= ((c >> b) & 1u) << (6 - 2 * b);
-	}
-	return static_cast<float>(static_cast<int>(v) + 1) / static_cast<float>(255u + 2u) - 0.5f;
-}
-
-__constant__ uint8_t c_blue_noise[ZAMD_BLUE_NOISE_LEN * ZAMD_BLUE_NOISE_LEN] = { ZAMD_BLUE_NOISE_BYTES };
-
-// Blue-noise offset for output pixel (i, j) of plane variant p (dither.cpp:219-232):
-// each variant starts the 64x64 map at its own (column, row) offset.
-__device__ __forceinline__ float blue_noise_offset(int plane, unsigned i, unsigned j)
-{
-	const unsigned variant = static_cast<unsigned>(plane) & 3;
-	const unsigned hoff = variant == 0 ? 0 : variant == 1 ? 32 : variant == 2 ? 16 : 48;
-	const unsigned voff = variant == 0 ? 0 : variant == 1 ? 12 : variant == 2 ? 55 : 26;
-	const unsigned t = c_blue_noise[((i + voff) % ZAMD_BLUE_NOISE_LEN) * ZAMD_BLUE_NOISE_LEN + ((hoff + j) & (ZAMD_BLUE_NOISE_LEN - 1))];
-	return static_cast<float>(static_cast<int>(t) + 1) / static_cast<float>(255u + 2u) - 0.5f;
-}
-
-// dither_ordered (dither.cpp:37-55) for one value already converted to float.
-__device__ __forceinline__ unsigned quantize(float x, const OutXform &o, unsigned i, unsigned j)
-{
-	x = x * o.scale + o.offset;
-	const float d = o.dither == 1 ? bayer_offset(o.plane, i, j) : o.dither == 2 ? blue_noise_offset(o.plane, i, j) : 0.0f;
-	x += d;
-	const float maxval = static_cast<float>(1UL << o.bits) - 1;
-	x = x < 0.0f ? 0.0f : (maxval < x ? maxval : x);
-	return static_cast<unsigned>(static_cast<int>(__builtin_rintf(x)));
-}
-
-__device__ __forceinline__ int32_t med3_i32(int32_t x, int32_t lo, int32_t hi)
-{
-	int32_t r;
-	asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(lo), "v"(hi));
-	return r;
-}
-
-// pack_pixel_u16 (resize_impl.cpp:36-42): ((acc + 2^13) >> 14) + 32768 clamped
-// to [0, pixel_max]. Adding 32768 << 14 before the shift is the same integer.
-__device__ __forceinline__ uint16_t pack_word(int32_t acc, int32_t pixel_max)
-{
-	return static_cast<uint16_t>(med3_i32((acc + ((32768 << 14) + (1 << 13))) >> 14, 0, pixel_max));
-}
-
-// Two 16-bit values (low halves of lo, hi) in one dword.
-__device__ __forceinline__ uint32_t pack16x2(uint32_t lo, uint32_t hi)
-{
-	return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
-}
 
 template <bool WORD>
 using lds_t = std::conditional_t<WORD, uint16_t, float>;
@@ -1502,194 +1378,6 @@ __global__ void __launch_bounds__(kThreads) k_point(PointArg a)
 		store_float(drow, j, a.out.type, x);
 }
 
-// ColorspaceConversionImpl::process (colorspace.cpp:45-77): up to six in-place
-// operations on three float channels, fused with the depth conversions before
-// and after it. One operation runs over NP pixels of three channels; the
-// operation (and a transfer function's selection) is decided once for all.
-template <int NP>
-__device__ __forceinline__ void apply_op(const CspOp &op, float *a, float *b, float *c)
-{
-	switch (op.kind) {
-	case CSP_MATRIX_FMA:
-#pragma unroll
-		for (int p = 0; p < NP; ++p) {
-			const float x = __builtin_fmaf(op.m[2], c[p], __builtin_fmaf(op.m[1], b[p], op.m[0] * a[p]));
-			const float y = __builtin_fmaf(op.m[5], c[p], __builtin_fmaf(op.m[4], b[p], op.m[3] * a[p]));
-			const float z = __builtin_fmaf(op.m[8], c[p], __builtin_fmaf(op.m[7], b[p], op.m[6] * a[p]));
-			a[p] = x; b[p] = y; c[p] = z;
-		}
-		break;
-	case CSP_LUT_LINEAR: {
-		// index = cvtps2dq(fma(x, 2^15, 2^14)): round to nearest even, NaN and
-		// values outside int32 to INT32_MIN; clamped to the table
-		auto idx = [](float x) {
-			const float v = __builtin_fmaf(x, 32768.0f, 16384.0f);
-			const int i = (v >= -2147483648.0f && v < 2147483648.0f) ? static_cast<int>(__builtin_rintf(v)) : INT_MIN;
-			return static_cast<unsigned>(min(max(i, 0), static_cast<int>(kLutLinearSize) - 1));
-		};
-#pragma unroll
-		for (int p = 0; p < NP; ++p) {
-			a[p] = op.lut[idx(a[p])];
-			b[p] = op.lut[idx(b[p])];
-			c[p] = op.lut[idx(c[p])];
-		}
-		break;
-	}
-	case CSP_LUT_GAMMA:
-#pragma unroll
-		for (int p = 0; p < NP; ++p) {
-			a[p] = op.lut[float_to_half(a[p])];
-			b[p] = op.lut[float_to_half(b[p])];
-			c[p] = op.lut[float_to_half(c[p])];
-		}
-		break;
-	case CSP_MATRIX:
-#pragma unroll
-		for (int p = 0; p < NP; ++p) {
-			const float x = op.m[0] * a[p] + op.m[1] * b[p] + op.m[2] * c[p];
-			const float y = op.m[3] * a[p] + op.m[4] * b[p] + op.m[5] * c[p];
-			const float z = op.m[6] * a[p] + op.m[7] * b[p] + op.m[8] * c[p];
-			a[p] = x; b[p] = y; c[p] = z;
-		}
-		break;
-	case CSP_GAMMA: {
-		float x[3 * NP];
-#pragma unroll
-		for (int p = 0; p < NP; ++p) {
-			x[p] = a[p] * op.pre;
-			x[NP + p] = b[p] * op.pre;
-			x[2 * NP + p] = c[p] * op.pre;
-		}
-		zm::transfer_n<3 * NP>(op.func, x);
-#pragma unroll
-		for (int p = 0; p < NP; ++p) {
-			a[p] = op.post * x[p];
-			b[p] = op.post * x[NP + p];
-			c[p] = op.post * x[2 * NP + p];
-		}
-		break;
-	}
-	case CSP_B67:
-#pragma unroll
-		for (int p = 0; p < NP; ++p) {
-			const float gamma = 1.2f;
-			const float r = a[p] * op.pre, g = b[p] * op.pre, bl = c[p] * op.pre;
-			const float yd = zm::maxf(op.kr * r + op.kg * g + op.kb * bl, zm::FLT_MIN_);
-			const float ys_inv = zm::powf_(yd, (1.0f - gamma) / gamma);
-			a[p] = zm::arib_b67_oetf(r * ys_inv);
-			b[p] = zm::arib_b67_oetf(g * ys_inv);
-			c[p] = zm::arib_b67_oetf(bl * ys_inv);
-		}
-		break;
-	case CSP_B67_SCALE:
-#pragma unroll
-		for (int p = 0; p < NP; ++p) {
-			const float gamma = 1.2f;
-			const float r = a[p] * op.pre, g = b[p] * op.pre, bl = c[p] * op.pre;
-			const float yd = zm::maxf(op.kr * r + op.kg * g + op.kb * bl, zm::FLT_MIN_);
-			const float ys_inv = zm::powf_(yd, (1.0f - gamma) / gamma);
-			a[p] = r * ys_inv;
-			b[p] = g * ys_inv;
-			c[p] = bl * ys_inv;
-		}
-		break;
-	case CSP_B67_INV:
-#pragma unroll
-		for (int p = 0; p < NP; ++p) {
-			const float gamma = 1.2f;
-			float r = zm::arib_b67_inverse_oetf(a[p]), g = zm::arib_b67_inverse_oetf(b[p]), bl = zm::arib_b67_inverse_oetf(c[p]);
-			float ys = zm::maxf(op.kr * r + op.kg * g + op.kb * bl, zm::FLT_MIN_);
-			ys = zm::powf_(ys, gamma - 1.0f);
-			r = r * ys; g = g * ys; bl = bl * ys;
-			a[p] = r * op.post; b[p] = g * op.post; c[p] = bl * op.post;
-		}
-		break;
-	case CSP_CL_TO_RGB: {
-		float x[3 * NP];
-#pragma unroll
-		for (int p = 0; p < NP; ++p) {
-			const float y = a[p], u = b[p], v = c[p];
-			const float bmy = u < 0 ? u * 2.0f * op.nb : u * 2.0f * op.pb;
-			const float rmy = v < 0 ? v * 2.0f * op.nr : v * 2.0f * op.pr;
-			x[p] = bmy + y;
-			x[NP + p] = rmy + y;
-			x[2 * NP + p] = y;
-		}
-		zm::transfer_n<3 * NP>(op.func, x);
-#pragma unroll
-		for (int p = 0; p < NP; ++p) {
-			const float bl = x[p], r = x[NP + p], yl = x[2 * NP + p];
-			const float g = (yl - op.kr * r - op.kb * bl) / op.kg;
-			a[p] = r * op.post; b[p] = g * op.post; c[p] = bl * op.post;
-		}
-		break;
-	}
-	case CSP_CL_TO_YUV: {
-		float x[3 * NP];
-#pragma unroll
-		for (int p = 0; p < NP; ++p) {
-			const float r = a[p] * op.pre, g = b[p] * op.pre, bl = c[p] * op.pre;
-			x[p] = op.kr * r + op.kg * g + op.kb * bl;
-			x[NP + p] = bl;
-			x[2 * NP + p] = r;
-		}
-		zm::transfer_n<3 * NP>(op.func, x);
-#pragma unroll
-		for (int p = 0; p < NP; ++p) {
-			const float y = x[p], bl = x[NP + p], r = x[2 * NP + p];
-			const float u = (bl - y < 0.0f) ? (bl - y) / (2.0f * op.nb) : (bl - y) / (2.0f * op.pb);
-			const float v = (r - y < 0.0f) ? (r - y) / (2.0f * op.nr) : (r - y) / (2.0f * op.pr);
-			a[p] = y; b[p] = u; c[p] = v;
-		}
-		break;
-	}
-	default:
-		break;
-	}
-}
-
-// Pixels per thread of the colourspace kernel (consecutive blocks of kThreads columns).
-constexpr int kCspPix = 4;
-
-__global__ void __launch_bounds__(kThreads) k_colorspace(CspArg a)
-{
-	zm::lds_tables_init();
-	const unsigned f = blockIdx.z;
-	const unsigned i = blockIdx.y;
-	const unsigned jb = blockIdx.x * (kThreads * kCspPix) + threadIdx.x;
-	if (i >= a.height || jb >= a.width)
-		return;
-	float v[3][kCspPix];
-#pragma unroll
-	for (int p = 0; p < 3; ++p) {
-		const char *srow = row_ptr(frame_ptr(a.src[p], f), a.src[p].stride, i);
-#pragma unroll
-		for (int k = 0; k < kCspPix; ++k) {
-			const unsigned j = min(jb + k * kThreads, a.width - 1);
-			float x = load_as_float(srow, j, a.in[p].type);
-			if (a.in[p].op == 2)
-				x = x * a.in[p].scale + a.in[p].offset;
-			v[p][k] = x;
-		}
-	}
-	for (int k = 0; k < a.prog.n; ++k)
-		apply_op<kCspPix>(a.prog.op[k], v[0], v[1], v[2]);
-#pragma unroll
-	for (int p = 0; p < 3; ++p) {
-		char *drow = row_ptr(frame_ptr_w(a.dst[p], f), a.dst[p].stride, i);
-#pragma unroll
-		for (int k = 0; k < kCspPix; ++k) {
-			const unsigned j = jb + k * kThreads;
-			if (j >= a.width)
-				break;
-			if (a.out[p].op == 1)
-				store_uint(drow, j, a.out[p].type, quantize(v[p][k], a.out[p], i, j));
-			else
-				store_float(drow, j, a.out[p].type, v[p][k]);
-		}
-	}
-}
-
 // Unresize (unresize_impl.cpp:27-85): the reference's float order, one row
 // (horizontal) or one column (vertical) per thread; the band coefficients
 // and LU factors of an output sample are the same for every thread, so they
@@ -2062,13 +1750,6 @@ int launch_unresize(const UnresizeArg &a, void *stream)
 		hipLaunchKernelGGL(k_unresize_h, grid, dim3(kThreads), 0, static_cast<hipStream_t>(stream), a);
 	else
 		hipLaunchKernelGGL(k_unresize_v, grid, dim3(kThreads), 0, static_cast<hipStream_t>(stream), a);
-	return static_cast<int>(hipGetLastError());
-}
-
-int launch_colorspace(const CspArg &a, void *stream)
-{
-	dim3 grid((a.width + kThreads * kCspPix - 1) / (kThreads * kCspPix), a.height, a.frames);
-	hipLaunchKernelGGL(k_colorspace, grid, dim3(kThreads), 0, static_cast<hipStream_t>(stream), a);
 	return static_cast<int>(hipGetLastError());
 }
 

@@ -470,6 +470,16 @@ ZHD float transfer(int f, float x)
 	}
 }
 
+// transfer() with the function fixed at compile time.
+template <int F>
+ZHD float transfer_c(float x)
+{
+#define ZAMD_CASE(id, fn) if constexpr (F == id) return fn(x); else
+	ZAMD_TRANSFERS(ZAMD_CASE)
+#undef ZAMD_CASE
+	return bitsf(0x7fc00000u);
+}
+
 // transfer() over n values with the function selected once.
 template <int N>
 ZHD void transfer_n(int f, float *x)
