@@ -45,6 +45,10 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 METRIC = "frames/s + HBM GB/s, UHD Lanczos & 709→2020; MFMA/learned vs plain at ≤0.5 LSB"
 WORKLOADS = ["fhd_709_to_2020", "uhd_lanczos", "fp32_down", "fp32_up", "batch64", "uhd_pq_to_hlg"]
+# Streams the timed batches alternate on: the error-diffusion stage of
+# uhd_pq_to_hlg is latency-bound (one dependent wavefront per plane), so the
+# next batch's colourspace stage runs beside it.
+DEFAULT_STREAMS = {"uhd_pq_to_hlg": 2}
 
 
 def parse():
@@ -58,20 +62,27 @@ def parse():
     ap.add_argument("--variant", default="all", choices=["all", "exact", "fp16", "bf16", "approx", "learned"],
                     help="with --workload: measure only this form (exact path, FP16/BF16 matrix-core resize, "
                          "approximate gamma, learned transfers); default: the exact path and every form the workload has")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="streams the timed batches alternate on (0: the workload's default, 2 where a latency-bound stage "
+                         "can run beside the next batch's compute-bound stage)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--cpu-runs", type=int, default=10, help="runs per CPU baseline mode (median reported)")
     return ap.parse_args()
 
 
-def device_batch(torch, graph, planes, out_shapes, frames):
-    """Upload `frames` distinct frames per plane; return a launcher closure and its tensors."""
+def device_batch(torch, graph, planes, out_shapes, frames, stream=None, inputs=None):
+    """Upload `frames` distinct frames per plane (or reuse `inputs`, the
+    tensors of another launcher); return a launcher closure on `stream`
+    (default: torch's current stream) with its own outputs and tmp, and its tensors."""
     import zimg_amd as z
     keep = []
     src_bufs, src_fs = [None] * 4, [0] * 4
+    ins = []
     for p, stack in enumerate(planes):
         if stack is None:
             continue
-        t = torch.from_numpy(np.ascontiguousarray(stack)).to("cuda")
+        t = inputs[len(ins)] if inputs is not None else torch.from_numpy(np.ascontiguousarray(stack)).to("cuda")
+        ins.append(t)
         keep.append(t)
         F, h, w = stack.shape
         src_bufs[p] = (t.data_ptr(), w * stack.itemsize)
@@ -89,16 +100,43 @@ def device_batch(torch, graph, planes, out_shapes, frames):
     keep.append(tmp)
     src = z.image_buffer(src_bufs)
     dst = z.image_buffer(dst_bufs)
-    stream = torch.cuda.current_stream().cuda_stream
+    hs = (stream or torch.cuda.current_stream()).cuda_stream
 
     def launch():
-        graph.process_device_batch(frames, src, src_fs, dst, dst_fs, tmp.data_ptr(), stream)
+        graph.process_device_batch(frames, src, src_fs, dst, dst_fs, tmp.data_ptr(), hs)
 
     def profiled():
-        return graph.process_device_profiled(frames, src, src_fs, dst, dst_fs, tmp.data_ptr(), stream)
+        return graph.process_device_profiled(frames, src, src_fs, dst, dst_fs, tmp.data_ptr(), hs)
 
     launch.profiled = profiled
+    launch.inputs = ins
     return launch, keep
+
+
+def time_steps_pipelined(torch, launches, streams, steps, warmup):
+    """Seconds for `steps` launches issued alternately on len(streams)
+    streams (each launcher its own outputs and tmp): consecutive batches are
+    independent, so one batch's latency-bound stage runs beside the next
+    batch's compute-bound stage. Events on the first stream bracket all of it."""
+    n = len(streams)
+    for k in range(warmup):
+        with torch.cuda.stream(streams[k % n]):
+            launches[k % n]()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(streams[0])
+    for st in streams[1:]:
+        st.wait_event(start)
+    for k in range(steps):
+        launches[k % n]()
+    for st in streams[1:]:
+        ev = torch.cuda.Event()
+        ev.record(st)
+        streams[0].wait_event(ev)
+    end.record(streams[0])
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / 1000.0
 
 
 def time_steps(torch, launch, steps, warmup):
@@ -183,7 +221,7 @@ def train_chain_surrogates():
     return [pq, hlg], info
 
 
-def measure(torch, name, frames, steps, warmup, precision=0, approx=False, surrogates=None):
+def measure(torch, name, frames, steps, warmup, precision=0, approx=False, surrogates=None, streams=1):
     g, planes, outs, algo_bytes, dtype = build_workload(name, frames, approx)
     if approx:
         dtype += " (approximate gamma: lookup tables)"
@@ -195,6 +233,13 @@ def measure(torch, name, frames, steps, warmup, precision=0, approx=False, surro
         dtype += f" (resize on MFMA, {'fp16' if precision == 1 else 'bf16'} hi+lo, FP32 accumulate)"
     launch, keep = device_batch(torch, g, planes, outs, frames)
     elapsed = time_steps(torch, launch, steps, warmup)
+    single = elapsed
+    if streams > 1:
+        # The same input frames, each stream its own outputs and tmp.
+        sts = [torch.cuda.Stream() for _ in range(streams)]
+        launches = [device_batch(torch, g, planes, outs, frames, stream=st, inputs=launch.inputs) for st in sts]
+        elapsed = time_steps_pipelined(torch, [l for l, _ in launches], sts, steps, warmup)
+        del launches
     per_step = elapsed / steps
     stages = g.device_stages()
     acc = None
@@ -223,6 +268,9 @@ def measure(torch, name, frames, steps, warmup, precision=0, approx=False, surro
     torch.cuda.empty_cache()
     r = {"fps": round(frames * steps / elapsed, 1), "ms_per_step": round(per_step * 1e3, 4), "frames": frames, "launches": len(stages),
          "dtype": dtype, "roofline": roof}
+    if streams > 1:
+        r["streams"] = streams
+        r["one_stream_fps"] = round(frames * steps / single, 1)
     if surrogates:
         r["surrogates"] = surrogates[1]
     return r
@@ -372,6 +420,10 @@ def main():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
     frames = args.frames or 64
+
+    def nstreams(name):
+        return args.streams or DEFAULT_STREAMS.get(name, 1)
+
     todo = [args.workload] if args.workload else WORKLOADS
     results = {}
     only = args.variant if args.workload else "all"
@@ -383,18 +435,18 @@ def main():
             kw = {"fp16": {"precision": 1}, "bf16": {"precision": 2}, "approx": {"approx": True}}.get(only, {})
             if only == "learned":
                 kw = {"surrogates": train_chain_surrogates()}
-            results[name] = measure(torch, name, frames, args.steps, args.warmup, **kw)
+            results[name] = measure(torch, name, frames, args.steps, args.warmup, streams=nstreams(name), **kw)
         elif name == "batch64":
             results[name] = measure_batch(torch, args.steps, args.warmup)
         else:
-            results[name] = measure(torch, name, frames, args.steps, args.warmup)
+            results[name] = measure(torch, name, frames, args.steps, args.warmup, streams=nstreams(name))
         if only == "all" and name != "batch64":
             if name in ("fp32_down", "fp32_up"):
                 results[name]["mfma_fp16"] = measure(torch, name, frames, args.steps, args.warmup, precision=1)
             if name == "fhd_709_to_2020":
                 results[name]["approximate_gamma"] = measure(torch, name, frames, args.steps, args.warmup, approx=True)
             if name == "uhd_pq_to_hlg":
-                learned = measure(torch, name, frames, args.steps, args.warmup, surrogates=train_chain_surrogates())
+                learned = measure(torch, name, frames, args.steps, args.warmup, surrogates=train_chain_surrogates(), streams=nstreams(name))
                 learned["gain_vs_exact"] = round(learned["fps"] / results[name]["fps"], 3)
                 results[name]["learned"] = learned
         if not args.no_cpu:
@@ -416,9 +468,10 @@ def main():
         "dtype": m["dtype"],
         "data": "synthetic (seeded, zimg_amd/workloads.py), inputs resident in HBM",
         "config": {"workload": primary, "frames_per_step": m["frames"], "graph_launches": m["launches"],
-                   "form": "exact" if only == "all" else only},
+                   "form": "exact" if only == "all" else only, "streams": m.get("streams", 1)},
         "roofline": m["roofline"],
         "cpu_baseline": m.get("cpu_baseline"),
+        "one_stream_fps": m.get("one_stream_fps"),
         "variants": {k: m[k] for k in ("approximate_gamma", "mfma_fp16", "learned") if k in m},
         "configs": {k: v for k, v in results.items() if k != primary},
     }

@@ -1561,9 +1561,12 @@ __global__ void __launch_bounds__(kEdWaves * 64) k_error_diffusion(EdArg a)
 #pragma unroll
 			for (int t = 0; t < kEdChunk; ++t) {
 				const int j = g.jb + t;
-				// wave_shr:1 -- lane l takes lane l - 1's error; lane 0 keeps `old`, the ring value.
+				// Lane l takes lane l - 1's error, lane 0 keeps `old`, the ring value:
+				// row_bcast:15 into rows 1-3 (lanes 16, 32, 48 take lanes 15, 31, 47),
+				// then row_shr:1 over it (measured faster than one wave_shr:1).
 				const int old = __builtin_amdgcn_readlane(__float_as_int(rv), t);
-				const float recv = __int_as_float(__builtin_amdgcn_update_dpp(old, __float_as_int(prod), 0x138, 0xF, 0xF, false));
+				const int cross = __builtin_amdgcn_update_dpp(old, __float_as_int(prod), 0x142, 0xE, 0xF, false);
+				const float recv = __int_as_float(__builtin_amdgcn_update_dpp(cross, __float_as_int(prod), 0x111, 0xF, 0xF, false));
 				t_m1 = t_0;
 				t_0 = t_p1;
 				t_p1 = recv;
