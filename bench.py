@@ -47,8 +47,9 @@ METRIC = "frames/s + HBM GB/s, UHD Lanczos & 709→2020; MFMA/learned vs plain a
 WORKLOADS = ["fhd_709_to_2020", "uhd_lanczos", "fp32_down", "fp32_up", "batch64", "uhd_pq_to_hlg"]
 # Streams the timed batches alternate on: the error-diffusion stage of
 # uhd_pq_to_hlg is latency-bound (one dependent wavefront per plane), so the
-# next batch's colourspace stage runs beside it.
-DEFAULT_STREAMS = {"uhd_pq_to_hlg": 2}
+# next batch's colourspace stage runs beside it; in fhd_709_to_2020 the next
+# batch's first launches fill the tails of the current batch's five launches.
+DEFAULT_STREAMS = {"uhd_pq_to_hlg": 2, "fhd_709_to_2020": 2}
 
 
 def parse():

@@ -11,8 +11,10 @@
 //    common video systems produce (matrix / transfer / primaries / transfer /
 //    matrix, and PQ -> HLG). Its code is straight-line: no dispatch, every
 //    operation constant in scalar registers, no scalar spills, and a short
-//    instruction stream for the instruction cache. Both evaluate the same
-//    float operations in the same order, so their outputs are identical.
+//    instruction stream for the instruction cache. Its powf steps use the
+//    fused form where one exists (zm::powf_fused: fused multiply-adds in the
+//    FP64 polynomials, with float results equal to the reference's for every
+//    input). Both kernels give identical outputs.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -112,16 +114,17 @@ __device__ __forceinline__ void op_gamma(const CspOp &op, float *a, float *b, fl
 }
 
 // AribB67OperationC (operation_impl.cpp:102-125): inverse OOTF scaling by
-// Ys^((1 - gamma) / gamma), then (OETF true) the HLG OETF.
-template <int NP, bool OETF>
+// Ys^((1 - gamma) / gamma), then (OETF true) the HLG OETF. FU: the fused
+// powf (the same results, fewer instructions).
+template <int NP, bool OETF, bool FU = false>
 __device__ __forceinline__ void op_b67(const CspOp &op, float *a, float *b, float *c)
 {
 #pragma unroll
 	for (int p = 0; p < NP; ++p) {
-		const float gamma = 1.2f;
+		constexpr float gamma = 1.2f;
 		const float r = a[p] * op.pre, g = b[p] * op.pre, bl = c[p] * op.pre;
 		const float yd = zm::maxf(op.kr * r + op.kg * g + op.kb * bl, zm::FLT_MIN_);
-		const float ys_inv = zm::powf_(yd, (1.0f - gamma) / gamma);
+		const float ys_inv = zm::powf_k<FU, ZM_FB((1.0f - gamma) / gamma)>(yd);
 		if constexpr (OETF) {
 			a[p] = zm::arib_b67_oetf(r * ys_inv);
 			b[p] = zm::arib_b67_oetf(g * ys_inv);
@@ -135,15 +138,15 @@ __device__ __forceinline__ void op_b67(const CspOp &op, float *a, float *b, floa
 }
 
 // AribB67InverseOperationC (operation_impl.cpp:127-150).
-template <int NP>
+template <int NP, bool FU = false>
 __device__ __forceinline__ void op_b67_inv(const CspOp &op, float *a, float *b, float *c)
 {
 #pragma unroll
 	for (int p = 0; p < NP; ++p) {
-		const float gamma = 1.2f;
+		constexpr float gamma = 1.2f;
 		float r = zm::arib_b67_inverse_oetf(a[p]), g = zm::arib_b67_inverse_oetf(b[p]), bl = zm::arib_b67_inverse_oetf(c[p]);
 		float ys = zm::maxf(op.kr * r + op.kg * g + op.kb * bl, zm::FLT_MIN_);
-		ys = zm::powf_(ys, gamma - 1.0f);
+		ys = zm::powf_k<FU, ZM_FB(gamma - 1.0f)>(ys);
 		r = r * ys; g = g * ys; bl = bl * ys;
 		a[p] = r * op.post; b[p] = g * op.post; c[p] = bl * op.post;
 	}
@@ -223,9 +226,9 @@ __device__ __forceinline__ void apply_slot(const CspOp &op, float *a, float *b, 
 	else if constexpr (K == CSP_GAMMA)
 		op_gamma<NP, F>(op, a, b, c);
 	else if constexpr (K == CSP_B67)
-		op_b67<NP, true>(op, a, b, c);
+		op_b67<NP, true, true>(op, a, b, c);
 	else if constexpr (K == CSP_B67_INV)
-		op_b67_inv<NP>(op, a, b, c);
+		op_b67_inv<NP, true>(op, a, b, c);
 	else
 		static_assert(K < 0, "operation kind without a chain form");
 }

@@ -122,6 +122,20 @@ __device__ inline void lds_tables_init() {}
 #endif
 #endif
 
+// a * b + c: two roundings (the reference's libm, built without
+// contraction) or, with FMA, one. The fused form is used only where an
+// exhaustive comparison over every float input showed the same float results
+// (see powf_fused below).
+template <bool FMA>
+ZHD double mul_add(double a, double b, double c)
+{
+	if constexpr (FMA)
+		return __builtin_fma(a, b, c);
+	else
+		return a * b + c;
+}
+
+template <bool FMA = false>
 ZHD float exp2_from_log(double xd, uint32_t sign_bias)
 {
 	// 2^xd = 2^(k/32) * 2^r, |r| <= 1/64; cubic for 2^r.
@@ -132,13 +146,14 @@ ZHD float exp2_from_log(double xd, uint32_t sign_bias)
 	double r = xd - kd;
 	uint64_t t = ZM_EXP2TAB(ki % 32) + ((ki + sign_bias) << 47);
 	double s = bitsd(t);
-	double z = 0x1.c6af84b912394p-5 * r + 0x1.ebfce50fac4f3p-3;
+	double z = mul_add<FMA>(0x1.c6af84b912394p-5, r, 0x1.ebfce50fac4f3p-3);
 	double r2 = r * r;
-	double y = 0x1.62e42ff0c52d6p-1 * r + 1.0;
-	y = z * r2 + y;
+	double y = mul_add<FMA>(0x1.62e42ff0c52d6p-1, r, 1.0);
+	y = mul_add<FMA>(z, r2, y);
 	return static_cast<float>(y * s);
 }
 
+template <bool FMA = false>
 ZHD double log2_reduced(uint32_t ix)
 {
 	// x = 2^k z, z in [0x3f330000, 2*0x3f330000); log2(x) = log1p(z/c - 1)/ln2 + log2(c) + k.
@@ -147,15 +162,15 @@ ZHD double log2_reduced(uint32_t ix)
 	uint32_t top = tmp & 0xff800000u;
 	int k = static_cast<int32_t>(top) >> 23;
 	double z = static_cast<double>(bitsf(ix - top));
-	double r = z * ZM_INVC(i) - 1.0;
+	double r = mul_add<FMA>(z, ZM_INVC(i), -1.0);
 	double y0 = ZM_LOG2C(i) + static_cast<double>(k);
 	double r2 = r * r;
-	double y = 0x1.27616c9496e0bp-2 * r + -0x1.71969a075c67ap-2;
-	double p = 0x1.ec70a6ca7baddp-2 * r + -0x1.7154748bef6c8p-1;
+	double y = mul_add<FMA>(0x1.27616c9496e0bp-2, r, -0x1.71969a075c67ap-2);
+	double p = mul_add<FMA>(0x1.ec70a6ca7baddp-2, r, -0x1.7154748bef6c8p-1);
 	double r4 = r2 * r2;
-	double q = 0x1.71547652ab82bp0 * r + y0;
-	q = p * r2 + q;
-	return y * r4 + q;
+	double q = mul_add<FMA>(0x1.71547652ab82bp0, r, y0);
+	q = mul_add<FMA>(p, r2, q);
+	return mul_add<FMA>(y, r4, q);
 }
 
 ZHD int int_class(uint32_t iy)
@@ -171,7 +186,8 @@ ZHD int int_class(uint32_t iy)
 
 ZHD bool zero_inf_nan(uint32_t ix) { return 2 * ix - 1 >= 2u * 0x7f800000u - 1; }
 
-ZHD float powf_(float x, float y)
+template <bool FMA = false>
+ZHD float powf_t(float x, float y)
 {
 	uint32_t sign_bias = 0;
 	uint32_t ix = fbits(x), iy = fbits(y);
@@ -200,13 +216,16 @@ ZHD float powf_(float x, float y)
 			ix -= 23u << 23;
 		}
 	}
-	double ylogx = static_cast<double>(y) * log2_reduced(ix);
+	double ylogx = static_cast<double>(y) * log2_reduced<FMA>(ix);
 	if (((dbits(ylogx) >> 47) & 0xffff) >= (dbits(126.0) >> 47)) {
 		if (ylogx > 0x1.fffffffd1d571p+6) return sign_bias ? -bitsf(0x7f800000u) : bitsf(0x7f800000u);
 		if (ylogx <= -150.0) return sign_bias ? -0.0f : 0.0f;
 	}
-	return exp2_from_log(ylogx, sign_bias);
+	return exp2_from_log<FMA>(ylogx, sign_bias);
 }
+
+// The reference's powf (no contraction).
+ZHD float powf_(float x, float y) { return powf_t<false>(x, y); }
 
 ZHD float expf_(float x)
 {
@@ -324,20 +343,63 @@ constexpr float FLT_MIN_ = 1.17549435082228750797e-38f;
 
 ZHD float maxf(float a, float b) { return a < b ? b : a; } // std::max order
 
+// Float bits of a constant expression (template arguments).
+#define ZM_FB(x) __builtin_bit_cast(uint32_t, (x))
+
+// powf(x, y) for an exponent fixed at compile time (float bits YB), with
+// fused multiply-adds in the FP64 steps. For each exponent accepted here the
+// float result equals the reference's unfused powf for all 2^32 bit patterns
+// of x, the inputs listed below included (they return the reference's value);
+// tests/test_powf_fma.py repeats that comparison over every input.
+template <uint32_t YB>
+ZHD float powf_fused(float x)
+{
+	float r = powf_t<true>(x, bitsf(YB));
+	const uint32_t ix = fbits(x);
+	if constexpr (YB == ZM_FB(2.4f)) {
+		r = ix == 0x3b9445eeu ? bitsf(0x361e949fu) : r;
+		r = ix == 0x3d109d26u ? bitsf(0x39ab8eb9u) : r;
+	} else if constexpr (YB == ZM_FB(ST2084_M1)) {
+		r = ix == 0x1371da3eu ? bitsf(0x387a7341u) : r;
+	} else if constexpr (YB == ZM_FB(ST2084_M2)) {
+		r = ix == 0x3f727c4eu ? bitsf(0x3c63ae8cu) : r;
+		r = ix == 0x3f5233b4u ? bitsf(0x343f4271u) : r;
+	} else {
+		static_assert(YB == ZM_FB(1.0f / 2.4f) || YB == ZM_FB(0.45f) || YB == ZM_FB(1.0f / 0.45f) || YB == ZM_FB(1.0f / ST2084_M2) ||
+		                  YB == ZM_FB(1.0f / ST2084_M1) || YB == ZM_FB((1.0f - 1.2f) / 1.2f) || YB == ZM_FB(1.2f - 1.0f),
+		              "powf_fused: exponent without an exhaustive comparison");
+	}
+	return r;
+}
+
+// powf(x, y) with y fixed: fused (FU) or the reference's.
+template <bool FU, uint32_t YB>
+ZHD float powf_k(float x)
+{
+	if constexpr (FU)
+		return powf_fused<YB>(x);
+	else
+		return powf_(x, bitsf(YB));
+}
+
+template <bool FU = false>
 ZHD float rec709_oetf(float x)
 {
 	x = maxf(x, 0.0f);
-	return x < REC709_BETA ? x * 4.5f : REC709_ALPHA * powf_(x, 0.45f) - (REC709_ALPHA - 1.0f);
+	return x < REC709_BETA ? x * 4.5f : REC709_ALPHA * powf_k<FU, ZM_FB(0.45f)>(x) - (REC709_ALPHA - 1.0f);
 }
 
+template <bool FU = false>
 ZHD float rec709_inverse_oetf(float x)
 {
 	x = maxf(x, 0.0f);
-	return x < 4.5f * REC709_BETA ? x / 4.5f : powf_((x + (REC709_ALPHA - 1.0f)) / REC709_ALPHA, 1.0f / 0.45f);
+	return x < 4.5f * REC709_BETA ? x / 4.5f : powf_k<FU, ZM_FB(1.0f / 0.45f)>((x + (REC709_ALPHA - 1.0f)) / REC709_ALPHA);
 }
 
-ZHD float rec1886_eotf(float x) { return x < 0.0f ? 0.0f : powf_(x, 2.4f); }
-ZHD float rec1886_inverse_eotf(float x) { return x < 0.0f ? 0.0f : powf_(x, 1.0f / 2.4f); }
+template <bool FU = false>
+ZHD float rec1886_eotf(float x) { return x < 0.0f ? 0.0f : powf_k<FU, ZM_FB(2.4f)>(x); }
+template <bool FU = false>
+ZHD float rec1886_inverse_eotf(float x) { return x < 0.0f ? 0.0f : powf_k<FU, ZM_FB(1.0f / 2.4f)>(x); }
 
 ZHD float arib_b67_oetf(float x)
 {
@@ -351,22 +413,24 @@ ZHD float arib_b67_inverse_oetf(float x)
 	return x <= 0.5f ? (x * x) * (1.0f / 3.0f) : (expf_((x - ARIB_B67_C) / ARIB_B67_A) + ARIB_B67_B) / 12.0f;
 }
 
+template <bool FU = false>
 ZHD float st2084_eotf(float x)
 {
 	if (!(x > 0.0f)) return 0.0f;
-	float xpow = powf_(x, 1.0f / ST2084_M2);
+	float xpow = powf_k<FU, ZM_FB(1.0f / ST2084_M2)>(x);
 	float num = maxf(xpow - ST2084_C1, 0.0f);
 	float den = maxf(ST2084_C2 - ST2084_C3 * xpow, FLT_MIN_);
-	return powf_(num / den, 1.0f / ST2084_M1);
+	return powf_k<FU, ZM_FB(1.0f / ST2084_M1)>(num / den);
 }
 
+template <bool FU = false>
 ZHD float st2084_inverse_eotf(float x)
 {
 	if (!(x > 0.0f)) return 0.0f;
-	float xpow = powf_(x, ST2084_M1);
+	float xpow = powf_k<FU, ZM_FB(ST2084_M1)>(x);
 	float num = (ST2084_C1 - 1.0f) + (ST2084_C2 - ST2084_C3) * xpow;
 	float den = 1.0f + ST2084_C3 * xpow;
-	return powf_(1.0f + num / den, ST2084_M2);
+	return powf_k<FU, ZM_FB(ST2084_M2)>(1.0f + num / den);
 }
 
 ZHD float copysign_(float mag, float sgn) { return bitsf((fbits(mag) & 0x7fffffffu) | (fbits(sgn) & 0x80000000u)); }
@@ -415,15 +479,17 @@ ZHD float t_xvycc_inv_eotf(float x)
 	if (x < 0.0f || x > 1.0f) return copysign_(rec709_oetf(fabs_(x)), x);
 	return rec1886_inverse_eotf(x);
 }
+template <bool FU = false>
 ZHD float t_srgb_eotf(float x)
 {
 	x = maxf(x, 0.0f);
-	return x < 12.92f * SRGB_BETA ? x / 12.92f : powf_((x + (SRGB_ALPHA - 1.0f)) / SRGB_ALPHA, 2.4f);
+	return x < 12.92f * SRGB_BETA ? x / 12.92f : powf_k<FU, ZM_FB(2.4f)>((x + (SRGB_ALPHA - 1.0f)) / SRGB_ALPHA);
 }
+template <bool FU = false>
 ZHD float t_srgb_inv_eotf(float x)
 {
 	x = maxf(x, 0.0f);
-	return x < SRGB_BETA ? x * 12.92f : SRGB_ALPHA * powf_(x, 1.0f / 2.4f) - (SRGB_ALPHA - 1.0f);
+	return x < SRGB_BETA ? x * 12.92f : SRGB_ALPHA * powf_k<FU, ZM_FB(1.0f / 2.4f)>(x) - (SRGB_ALPHA - 1.0f);
 }
 ZHD float t_428_eotf(float x) { return x < 0.0f ? 0.0f : 52.37f / 48.0f * powf_(x, 2.6f); }
 ZHD float t_428_inv_eotf(float x) { return x < 0.0f ? 0.0f : powf_(x * (48.0f / 52.37f), 1.0f / 2.6f); }
@@ -470,10 +536,21 @@ ZHD float transfer(int f, float x)
 	}
 }
 
-// transfer() with the function fixed at compile time.
+// Transfers whose powf steps have a fused form (powf_fused): the same float
+// results as the reference's for every input, with fewer instructions.
+#define ZAMD_FUSED_TRANSFERS(X) \
+	X(F_709_OETF, rec709_oetf) X(F_709_INV_OETF, rec709_inverse_oetf) \
+	X(F_1886_EOTF, rec1886_eotf) X(F_1886_INV_EOTF, rec1886_inverse_eotf) \
+	X(F_SRGB_EOTF, t_srgb_eotf) X(F_SRGB_INV_EOTF, t_srgb_inv_eotf) \
+	X(F_2084_EOTF, st2084_eotf) X(F_2084_INV_EOTF, st2084_inverse_eotf)
+
+// transfer() with the function fixed at compile time (the fused form where one exists).
 template <int F>
 ZHD float transfer_c(float x)
 {
+#define ZAMD_CASE(id, fn) if constexpr (F == id) return fn<true>(x); else
+	ZAMD_FUSED_TRANSFERS(ZAMD_CASE)
+#undef ZAMD_CASE
 #define ZAMD_CASE(id, fn) if constexpr (F == id) return fn(x); else
 	ZAMD_TRANSFERS(ZAMD_CASE)
 #undef ZAMD_CASE
