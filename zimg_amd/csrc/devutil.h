@@ -73,26 +73,60 @@ __device__ __forceinline__ void store_float(char *row, unsigned j, int type, flo
 	else reinterpret_cast<float *>(row)[j] = v;
 }
 
-// Bayer 16x16 ordered-dither offset for output pixel (i, j) of plane variant p
-// (dither.cpp:137-217): index bits interleave (row xor col, col) from the top;
-// variants flip horizontally, vertically, or transpose; value (t+1)/257 - 1/2.
-__device__ __forceinline__ float bayer_offset(int plane, unsigned i, unsigned j)
+// Ordered-dither offset of an 8-bit threshold t: (t + 1) / 257 - 1/2 in float
+// (dither.cpp:137-232). Constant evaluation rounds each operation to nearest
+// as the device's IEEE division and subtraction do, so the tables below hold
+// the reference's values bit for bit.
+__host__ __device__ constexpr float dither_unit(unsigned t)
 {
-	unsigned r = i & 15, c = j & 15;
+	return static_cast<float>(static_cast<int>(t) + 1) / static_cast<float>(255u + 2u) - 0.5f;
+}
+
+// Bayer 16x16 threshold of position (r, c) = (row & 15, column & 15) of plane
+// variant p (dither.cpp:137-217): index bits interleave (row xor col, col)
+// from the top; variants flip horizontally, vertically, or transpose.
+__host__ __device__ constexpr unsigned bayer_threshold(int plane, unsigned r, unsigned c)
+{
 	switch (plane & 3) {
 	case 1: c = 15 - c; break;
 	case 2: r = 15 - r; break;
-	case 3: { unsigned t = r; r = c; c = t; } break;
+	case 3: { const unsigned t = r; r = c; c = t; } break;
 	default: break;
 	}
 	const unsigned x = r ^ c;
 	unsigned v = 0;
-#pragma unroll
 	for (unsigned b = 0; b < 4; ++b) {
 		v |= ((x >> b) & 1u) << (7 - 2 * b);
 		v |= ((c >> b) & 1u) << (6 - 2 * b);
 	}
-	return static_cast<float>(static_cast<int>(v) + 1) / static_cast<float>(255u + 2u) - 0.5f;
+	return v;
+}
+
+// Offsets per Bayer variant and position (variant * 256 + r * 16 + c), and per
+// 8-bit threshold, built at compile time: one load per output instead of the
+// index interleave and a division.
+struct alignas(16) DitherTables {
+	float bayer[4 * 256];
+	float unit[256];
+};
+
+constexpr DitherTables make_dither_tables()
+{
+	DitherTables t{};
+	for (unsigned p = 0; p < 4; ++p)
+		for (unsigned r = 0; r < 16; ++r)
+			for (unsigned c = 0; c < 16; ++c)
+				t.bayer[p * 256 + r * 16 + c] = dither_unit(bayer_threshold(static_cast<int>(p), r, c));
+	for (unsigned v = 0; v < 256; ++v)
+		t.unit[v] = dither_unit(v);
+	return t;
+}
+
+__constant__ DitherTables c_dither = make_dither_tables();
+
+__device__ __forceinline__ float bayer_offset(int plane, unsigned i, unsigned j)
+{
+	return c_dither.bayer[(static_cast<unsigned>(plane) & 3) * 256 + (i & 15) * 16 + (j & 15)];
 }
 
 __constant__ uint8_t c_blue_noise[ZAMD_BLUE_NOISE_LEN * ZAMD_BLUE_NOISE_LEN] = { ZAMD_BLUE_NOISE_BYTES };
@@ -104,19 +138,40 @@ __device__ __forceinline__ float blue_noise_offset(int plane, unsigned i, unsign
 	const unsigned variant = static_cast<unsigned>(plane) & 3;
 	const unsigned hoff = variant == 0 ? 0 : variant == 1 ? 32 : variant == 2 ? 16 : 48;
 	const unsigned voff = variant == 0 ? 0 : variant == 1 ? 12 : variant == 2 ? 55 : 26;
-	const unsigned t = c_blue_noise[((i + voff) % ZAMD_BLUE_NOISE_LEN) * ZAMD_BLUE_NOISE_LEN + ((hoff + j) & (ZAMD_BLUE_NOISE_LEN - 1))];
-	return static_cast<float>(static_cast<int>(t) + 1) / static_cast<float>(255u + 2u) - 0.5f;
+	return c_dither.unit[c_blue_noise[((i + voff) % ZAMD_BLUE_NOISE_LEN) * ZAMD_BLUE_NOISE_LEN + ((hoff + j) & (ZAMD_BLUE_NOISE_LEN - 1))]];
 }
 
-// dither_ordered (dither.cpp:37-55) for one value already converted to float.
-__device__ __forceinline__ unsigned quantize(float x, const OutXform &o, unsigned i, unsigned j)
+__device__ __forceinline__ float dither_offset(const OutXform &o, unsigned i, unsigned j)
+{
+	return o.dither == 1 ? bayer_offset(o.plane, i, j) : o.dither == 2 ? blue_noise_offset(o.plane, i, j) : 0.0f;
+}
+
+// Offsets of outputs (i, j .. j + 3) for j a multiple of 4 (one 16-byte load
+// for Bayer).
+__device__ __forceinline__ float4 dither_offset4(const OutXform &o, unsigned i, unsigned j)
+{
+	if (o.dither == 1)
+		return *reinterpret_cast<const float4 *>(&c_dither.bayer[(static_cast<unsigned>(o.plane) & 3) * 256 + (i & 15) * 16 + (j & 15)]);
+	if (o.dither == 2)
+		return make_float4(blue_noise_offset(o.plane, i, j), blue_noise_offset(o.plane, i, j + 1), blue_noise_offset(o.plane, i, j + 2),
+		                   blue_noise_offset(o.plane, i, j + 3));
+	return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// dither_ordered (dither.cpp:37-55) for one value already converted to float,
+// with its dither offset d.
+__device__ __forceinline__ unsigned quantize_d(float x, const OutXform &o, float d)
 {
 	x = x * o.scale + o.offset;
-	const float d = o.dither == 1 ? bayer_offset(o.plane, i, j) : o.dither == 2 ? blue_noise_offset(o.plane, i, j) : 0.0f;
 	x += d;
 	const float maxval = static_cast<float>(1UL << o.bits) - 1;
 	x = x < 0.0f ? 0.0f : (maxval < x ? maxval : x);
 	return static_cast<unsigned>(static_cast<int>(__builtin_rintf(x)));
+}
+
+__device__ __forceinline__ unsigned quantize(float x, const OutXform &o, unsigned i, unsigned j)
+{
+	return quantize_d(x, o, dither_offset(o, i, j));
 }
 
 __device__ __forceinline__ int32_t med3_i32(int32_t x, int32_t lo, int32_t hi)

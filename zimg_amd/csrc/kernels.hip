@@ -687,6 +687,7 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 				const unsigned g = t / (kThreads / 2);
 				const unsigned n4 = j4 < ncols ? min(4u, ncols - j4) : 0;
 				const bool wide = n4 == 4 && a.out.op == 0 && a.out.type == 1;
+				const bool qbyte = n4 == 4 && a.out.op == 1 && a.out.type == 0;
 				for (unsigned i = g; i < nrows; i += 2) {
 					const unsigned row = i0 + i;
 					const uint32_t *m = mid + ((vtop[i] - r0) / 2) * pm + j4;
@@ -731,6 +732,13 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 					const uint16_t v2 = pack_word_pre(a2, a.pixel_max), v3 = pack_word_pre(a3, a.pixel_max);
 					if (wide) {
 						*reinterpret_cast<uint2 *>(drow + 2 * static_cast<size_t>(j0 + j4)) = make_uint2(pack16x2(v0, v1), pack16x2(v2, v3));
+					} else if (qbyte) {
+						// Four quantised BYTE outputs (j0 + j4 is a multiple of 4): one
+						// offset load, one 4-byte store.
+						const float4 d = dither_offset4(a.out, row, j0 + j4);
+						const uint32_t q = quantize_d(static_cast<float>(v0), a.out, d.x) | (quantize_d(static_cast<float>(v1), a.out, d.y) << 8) |
+						                   (quantize_d(static_cast<float>(v2), a.out, d.z) << 16) | (quantize_d(static_cast<float>(v3), a.out, d.w) << 24);
+						*reinterpret_cast<uint32_t *>(drow + j0 + j4) = q;
 					} else if (n4) {
 						store_pair<true>(drow, j0 + j4, v0, v1, n4 > 1, a.out, row);
 						if (n4 > 2)
