@@ -445,7 +445,7 @@ void DeviceProgram::check_tiled(const Step &s, const TiledConfig &c, const Filte
 		const unsigned nr = std::min(th, s.height - i0), il = i0 + nr - 1;
 		unsigned r0 = i0, span = nr;
 		if (tv_tab) {
-			const bool pairs = w && s.h_first;
+			const bool pairs = w;
 			r0 = pairs ? pair_left(*tv_tab, i0) : tv_tab->left[i0];
 			span = (pairs ? pair_left(*tv_tab, il) + 2 * pair_count(*tv_tab) : tv_tab->left[il] + tv_tab->filter_width) - r0;
 			for (unsigned i = i0; i <= il; ++i) {
@@ -476,13 +476,19 @@ void DeviceProgram::configure_resize(Step &s)
 	const bool sorted_all = sorted1 && (!t2 || t2->sorted());
 	int fwh = 0;
 	if (sorted_all) {
-		const unsigned need = th_tab ? (s.word ? pair_count(*th_tab) : th_tab->filter_width) : 1;
+		// The horizontal taps set the register bucket; a vertical-only step takes
+		// it from its vertical taps (its tap loop is unrolled over the bucket,
+		// wider filters take the kernel's tap loop in the smallest bucket).
+		const unsigned need = th_tab ? (s.word ? pair_count(*th_tab) : th_tab->filter_width)
+		                             : tv_tab ? (s.word ? pair_count(*tv_tab) : tv_tab->filter_width) : 1;
 		for (int b : dev::kFastTapBuckets) {
 			if (need <= static_cast<unsigned>(b)) {
 				fwh = b;
 				break;
 			}
 		}
+		if (!fwh && !th_tab)
+			fwh = dev::kFastTapBuckets[0];
 	}
 	if (fwh) {
 		// Horizontal window [cb, cb + nc) of output columns [j0, j0 + nc_out) exactly as the kernel computes it.
@@ -493,7 +499,7 @@ void DeviceProgram::configure_resize(Step &s)
 			return (ce - cb + 15) & ~15u;
 		};
 		// Per-row vertical taps held in LDS for a tile row (start row + taps).
-		const unsigned vtaps = (s.h_first && !t2) ? 0 : (s.word && s.h_first ? pair_count(*tv_tab) : tv_tab->filter_width);
+		const unsigned vtaps = (s.h_first && !t2) ? 0 : (s.word ? pair_count(*tv_tab) : tv_tab->filter_width);
 		auto ceil_div = [](size_t a, size_t b) { return (a + b - 1) / b; };
 		std::vector<unsigned> widths = s.h_first ? std::vector<unsigned>{ dev::kFastTileW } : std::vector<unsigned>{ 512, 256, 128, 64, 32, 16 };
 		for (unsigned tw : widths) {
@@ -509,7 +515,7 @@ void DeviceProgram::configure_resize(Step &s)
 					unsigned span;
 					if (!tv_tab)
 						span = nr;
-					else if (s.h_first && s.word)
+					else if (s.word)
 						span = pair_left(*tv_tab, il) + 2 * pair_count(*tv_tab) - pair_left(*tv_tab, i0);
 					else
 						span = tv_tab->left[il] + tv_tab->filter_width - tv_tab->left[i0];
@@ -527,7 +533,7 @@ void DeviceProgram::configure_resize(Step &s)
 				// second pass, weighted by taps), scaled by how many blocks share a CU.
 				const size_t blocks = ceil_div(s.width, tw) * ceil_div(s.height, th);
 				const double stage = static_cast<double>(ceil_div(static_cast<size_t>(in_rows) * in_cols * elem / 16, 256)) * 6.0 + 40.0;
-				const double fwv = tv_tab ? (s.word && s.h_first ? pair_count(*tv_tab) : tv_tab->filter_width) : 0;
+				const double fwv = tv_tab ? (s.word ? pair_count(*tv_tab) : tv_tab->filter_width) : 0;
 				const double fwhh = th_tab ? fwh : 0;
 				double work;
 				if (s.h_first) {

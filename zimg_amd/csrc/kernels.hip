@@ -443,6 +443,104 @@ __device__ __forceinline__ void stage_tile(lds_t<WORD> *xin, unsigned pin, const
 	}
 }
 
+// WORD staging with row pairs interleaved: dword (p, c) of the LDS window
+// (pitch `pin` dwords per pair) holds biased samples (r0 + 2p, c0 + c) in its
+// low half and (r0 + 2p + 1, c0 + c) in its high half, so a vertical pass
+// applies two taps per v_dot2. nr is even; BYTE or WORD input. The loads
+// follow stage_tile: each unit is the same 16-byte chunk of both rows of a
+// pair, every load of a batch issued before any is used, none conditional,
+// chunks outside the plane read at a clamped address and rebuilt sample by
+// sample (rows past rows_valid as zero samples).
+template <int U>
+__device__ __forceinline__ void stage_tile_pairs(uint32_t *xw, unsigned pin, const char *src, long long stride, unsigned r0,
+                                                 unsigned nr, unsigned rows_valid, unsigned c0, unsigned nc, unsigned src_w,
+                                                 const InXform &in)
+{
+	const unsigned esz = in.type == 0 ? 1 : 2;
+	const unsigned ce = 16 / esz;
+	const unsigned cpr = nc / ce;
+	const unsigned np = nr / 2;
+	const unsigned total = np * cpr;
+	const unsigned last_full = src_w >= ce ? (src_w - ce) / ce * ce : 0;
+	const unsigned s = in.op == 1 ? in.shift : 0;
+	const unsigned step_rows = kThreads / cpr, step_cols = kThreads % cpr;
+	unsigned pr = threadIdx.x / cpr, col = threadIdx.x - (threadIdx.x / cpr) * cpr;
+	for (unsigned base = 0; base < total; base += U * kThreads) {
+		uint4 raw[U][2];
+		unsigned pp[U], cc[U];
+		bool full[U][2];
+#pragma unroll
+		for (int u = 0; u < U; ++u) {
+			const bool past = pr >= np;
+			pp[u] = past ? np - 1 : pr;
+			cc[u] = c0 + (past ? cpr - 1 : col) * ce;
+			col += step_cols;
+			pr += step_rows;
+			if (col >= cpr) {
+				col -= cpr;
+				++pr;
+			}
+			const unsigned cl = min(cc[u], last_full);
+#pragma unroll
+			for (int h = 0; h < 2; ++h) {
+				const unsigned row = r0 + 2 * pp[u] + h;
+				full[u][h] = row < rows_valid && cc[u] + ce <= src_w;
+				raw[u][h] = *reinterpret_cast<const uint4 *>(row_ptr(src, stride, min(row, rows_valid - 1)) + static_cast<size_t>(cl) * esz);
+			}
+		}
+#pragma unroll
+		for (int u = 0; u < U; ++u) {
+#pragma unroll
+			for (int h = 0; h < 2; ++h) {
+				const unsigned row = r0 + 2 * pp[u] + h;
+				if (!full[u][h]) {
+					uint32_t w[4] = { 0, 0, 0, 0 };
+					if (row < rows_valid) {
+						const char *srow = row_ptr(src, stride, row);
+#pragma unroll
+						for (unsigned k = 0; k < 16; ++k) {
+							const unsigned c = cc[u] + k;
+							if (k < ce && c < src_w) {
+								const uint32_t v = load_as_uint(srow, c, in.type);
+								if (esz == 2)
+									w[(k >> 1) & 3] |= v << (16 * (k & 1));
+								else
+									w[k >> 2] |= v << (8 * (k & 3));
+							}
+						}
+					}
+					raw[u][h] = make_uint4(w[0], w[1], w[2], w[3]);
+				}
+			}
+			const uint32_t ev[4] = { raw[u][0].x, raw[u][0].y, raw[u][0].z, raw[u][0].w };
+			const uint32_t od[4] = { raw[u][1].x, raw[u][1].y, raw[u][1].z, raw[u][1].w };
+			uint4 *d = reinterpret_cast<uint4 *>(xw + static_cast<size_t>(pp[u]) * pin + (cc[u] - c0));
+			if (esz == 2) {
+				// dword 2k: low halves of the two rows' dword k; 2k + 1: high halves.
+				uint32_t o[8];
+#pragma unroll
+				for (int k = 0; k < 4; ++k) {
+					o[2 * k] = shift_bias2(__builtin_amdgcn_perm(od[k], ev[k], 0x05040100u), s);
+					o[2 * k + 1] = shift_bias2(__builtin_amdgcn_perm(od[k], ev[k], 0x07060302u), s);
+				}
+				d[0] = make_uint4(o[0], o[1], o[2], o[3]);
+				d[1] = make_uint4(o[4], o[5], o[6], o[7]);
+			} else {
+				// dword 4k + m: byte m of the two rows' dword k, zero-extended.
+				uint32_t o[16];
+#pragma unroll
+				for (int k = 0; k < 4; ++k)
+#pragma unroll
+					for (int m = 0; m < 4; ++m)
+						o[4 * k + m] = shift_bias2(__builtin_amdgcn_perm(od[k], ev[k], 0x0C000C00u | (static_cast<uint32_t>(4 + m) << 16) | static_cast<uint32_t>(m)), s);
+#pragma unroll
+				for (int k = 0; k < 4; ++k)
+					d[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+			}
+		}
+	}
+}
+
 // Horizontal taps of a thread's two output columns, in registers. WORD:
 // N packed int16 pairs per column over the even-aligned window (dot2), dword
 // offsets into an LDS row; pairs past the table's fwp are zero, so all N are
@@ -543,7 +641,7 @@ __host__ __device__ inline unsigned tiled_vtaps(const ResizeArg &a)
 	if (a.h_first && a.passes == 1)
 		return 0;
 	const PassArg &pv = a.h_first ? a.p2 : a.p1;
-	return a.word && a.h_first ? pv.fwp : pv.fw;
+	return a.word ? pv.fwp : pv.fw;
 }
 
 // Start rows occupy the first round4(tile_h) words, then one row of taps per
@@ -592,8 +690,8 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 	const unsigned pin = a.lds_in_cols;
 
 	// Vertical table rows of this tile: vtab[i] = first source row,
-	// vtab[nrows + i * taps + k] = tap k (int16 pairs for WORD after a
-	// horizontal pass, else int16 or float bits).
+	// vtab[nrows + i * taps + k] = tap k (WORD: int16 pairs of the pair
+	// table, whose windows start on an even row; FLOAT: float bits).
 	const unsigned vtaps = tiled_vtaps(a);
 	const unsigned vstride = (vtaps + 3) & ~3u;
 	const uint32_t *vtop = vtab;
@@ -608,14 +706,13 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 	unsigned vpos[kVtRegs];
 	auto vt_entry = [&](unsigned e, uint32_t &val, unsigned &pos) {
 		const PassArg &pv = a.h_first ? a.p2 : a.p1;
-		const bool pairs = WORD && a.h_first;
 		if (e < nrows) {
-			val = pairs ? pv.pl[i0 + e] : pv.left[i0 + e];
+			val = WORD ? pv.pl[i0 + e] : pv.left[i0 + e];
 			pos = e;
 		} else {
 			const unsigned c = e - nrows, i = c / vtaps, k = c - i * vtaps;
 			const size_t g = static_cast<size_t>(i0) * vtaps + c;
-			val = pairs ? static_cast<uint32_t>(pv.pc[g]) : WORD ? static_cast<uint32_t>(static_cast<int32_t>(pv.ci[g])) : __float_as_uint(pv.cf[g]);
+			val = WORD ? static_cast<uint32_t>(pv.pc[g]) : __float_as_uint(pv.cf[g]);
 			pos = ((a.tile_h + 3) & ~3u) + i * vstride + k;
 		}
 	};
@@ -843,10 +940,11 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 
 	// Vertical pass first (or alone): stage the input rows of the tile's
 	// output rows over the columns the horizontal pass (if any) reads.
+	// WORD: row pairs from the even-aligned window starts (pair tables).
 	const PassArg &pv = a.p1;
 	const unsigned il = i0 + nrows - 1;
-	const unsigned r0 = pv.left[i0];
-	const unsigned span = pv.left[il] + pv.fw - r0;
+	const unsigned r0 = WORD ? pv.pl[i0] : pv.left[i0];
+	const unsigned span = WORD ? pv.pl[il] + 2 * pv.fwp - r0 : pv.left[il] + pv.fw - r0;
 	unsigned cb, nc;
 	if (a.passes == 2) {
 		const PassArg &ph = a.p2;
@@ -871,57 +969,99 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 	HRegs<WORD, N> h;
 	if (a.passes == 2)
 		h.load(a.p2, oj, otwo ? oj + 1 : oj, cb);
-	stage_tile<WORD, 8>(xin, pin, src, a.src.stride, r0, span, a.src_h, cb, nc, a.src_w, a.in);
+	if constexpr (WORD)
+		stage_tile_pairs<4>(reinterpret_cast<uint32_t *>(xin), pin, src, a.src.stride, r0, span, a.src_h, cb, nc, a.src_w, a.in);
+	else
+		stage_tile<WORD, 8>(xin, pin, src, a.src.stride, r0, span, a.src_h, cb, nc, a.src_w, a.in);
 	vt_store();
 	__syncthreads();
 
-	// Vertical pass over column pairs: groups of threads split the rows.
-	const unsigned pairs = nc / 2;
-	const unsigned pstride = pairs >= kThreads ? kThreads : pairs;
-	const unsigned groups = kThreads / pstride;
-	const unsigned grp = t / pstride;
 	const unsigned pm = a.lds_mid_cols;
-	if (grp < groups) {
-		for (unsigned q = t - grp * pstride; q < pairs; q += kThreads) {
-			for (unsigned i = grp; i < nrows; i += groups) {
-				const unsigned row = i0 + i;
-				const unsigned top = vtop[i] - r0;
-				const uint32_t *cv = vcoef + i * vstride;
-				T v0, v1;
-				if constexpr (WORD) {
-					const uint32_t *xw = reinterpret_cast<const uint32_t *>(xin) + top * (pin / 2) + q;
-					const int32_t init = a.passes == 2 ? kBiasedInit : kWordInit;
-					int32_t a0 = init, a1 = init;
-					if (vtaps <= static_cast<unsigned>(N)) {
-						// Unrolled over N taps (zero taps past vtaps add exactly 0).
-						uint32_t wv[N];
-						int32_t cc[N];
+	if constexpr (WORD) {
+		// Vertical pass over the interleaved row pairs, two taps per v_dot2:
+		// items of four columns x one output row spread over all threads.
+		const uint32_t *xw = reinterpret_cast<const uint32_t *>(xin);
+		const unsigned quads = nc / 4, items = quads * nrows;
+		const unsigned sq = kThreads % quads, si = kThreads / quads;
+		unsigned q = t % quads, i = t / quads;
+		const int32_t init = a.passes == 2 ? kBiasedInit : kWordInit;
+		const bool wide = a.out.op == 0 && a.out.type == 1;
+		const bool qbyte = a.out.op == 1 && a.out.type == 0;
+		for (unsigned it = t; it < items; it += kThreads) {
+			const uint32_t *m = xw + ((vtop[i] - r0) / 2) * pin + 4 * q;
+			const uint32_t *cv = vcoef + i * vstride;
+			int32_t a0 = init, a1 = init, a2 = init, a3 = init;
+			if (vtaps <= static_cast<unsigned>(N)) {
+				// N tap pairs unrolled, all LDS reads issued first; pairs past
+				// vtaps get a zero tap over the last pair (adds exactly 0).
+				uint4 mm[N];
+				int32_t cc[N];
 #pragma unroll
-						for (int k = 0; k < N; ++k) {
-							const unsigned kk = min(static_cast<unsigned>(k), vtaps - 1);
-							wv[k] = xw[kk * (pin / 2)];
-							cc[k] = static_cast<unsigned>(k) < vtaps ? static_cast<int32_t>(cv[kk]) : 0;
-						}
+				for (int k = 0; k < N; ++k) {
+					const unsigned kk = min(static_cast<unsigned>(k), vtaps - 1);
+					mm[k] = *reinterpret_cast<const uint4 *>(m + kk * pin);
+					cc[k] = static_cast<unsigned>(k) < vtaps ? static_cast<int32_t>(cv[kk]) : 0;
+				}
 #pragma unroll
-						for (int k = 0; k < N; ++k) {
-							a0 += cc[k] * static_cast<int32_t>(static_cast<int16_t>(wv[k] & 0xFFFFu));
-							a1 += cc[k] * (static_cast<int32_t>(wv[k]) >> 16);
-						}
-					} else {
-						for (unsigned k = 0; k < vtaps; ++k) {
-							const uint32_t w = xw[k * (pin / 2)];
-							const int32_t c = static_cast<int32_t>(cv[k]);
-							a0 += c * static_cast<int32_t>(static_cast<int16_t>(w & 0xFFFFu));
-							a1 += c * (static_cast<int32_t>(w) >> 16);
-						}
-					}
-					if (a.passes == 2) {
-						reinterpret_cast<uint32_t *>(mid_base)[i * (pm / 2) + q] = pack16x2(pack_biased_pre(a0, a.pixel_max), pack_biased_pre(a1, a.pixel_max));
-						continue;
-					}
-					v0 = pack_word_pre(a0, a.pixel_max);
-					v1 = pack_word_pre(a1, a.pixel_max);
-				} else {
+				for (int k = 0; k < N; ++k) {
+					a0 = dot2(mm[k].x, cc[k], a0);
+					a1 = dot2(mm[k].y, cc[k], a1);
+					a2 = dot2(mm[k].z, cc[k], a2);
+					a3 = dot2(mm[k].w, cc[k], a3);
+				}
+			} else {
+				for (unsigned k = 0; k < vtaps; ++k) {
+					const uint4 mm = *reinterpret_cast<const uint4 *>(m + k * pin);
+					const int32_t c = static_cast<int32_t>(cv[k]);
+					a0 = dot2(mm.x, c, a0);
+					a1 = dot2(mm.y, c, a1);
+					a2 = dot2(mm.z, c, a2);
+					a3 = dot2(mm.w, c, a3);
+				}
+			}
+			if (a.passes == 2) {
+				*reinterpret_cast<uint2 *>(reinterpret_cast<uint32_t *>(mid_base) + i * (pm / 2) + 2 * q) =
+					make_uint2(pack16x2(pack_biased_pre(a0, a.pixel_max), pack_biased_pre(a1, a.pixel_max)),
+					           pack16x2(pack_biased_pre(a2, a.pixel_max), pack_biased_pre(a3, a.pixel_max)));
+			} else {
+				// Vertical only: cb = j0, four output columns j0 + 4q ...
+				const unsigned row = i0 + i, jc = 4 * q;
+				const unsigned n4 = jc < ncols ? min(4u, ncols - jc) : 0;
+				char *drow = row_ptr(dst, a.dst.stride, row);
+				const uint16_t v0 = pack_word_pre(a0, a.pixel_max), v1 = pack_word_pre(a1, a.pixel_max);
+				const uint16_t v2 = pack_word_pre(a2, a.pixel_max), v3 = pack_word_pre(a3, a.pixel_max);
+				if (n4 == 4 && wide) {
+					*reinterpret_cast<uint2 *>(drow + 2 * static_cast<size_t>(j0 + jc)) = make_uint2(pack16x2(v0, v1), pack16x2(v2, v3));
+				} else if (n4 == 4 && qbyte) {
+					const float4 d = dither_offset4(a.out, row, j0 + jc);
+					const uint32_t qv = quantize_d(static_cast<float>(v0), a.out, d.x) | (quantize_d(static_cast<float>(v1), a.out, d.y) << 8) |
+					                    (quantize_d(static_cast<float>(v2), a.out, d.z) << 16) | (quantize_d(static_cast<float>(v3), a.out, d.w) << 24);
+					*reinterpret_cast<uint32_t *>(drow + j0 + jc) = qv;
+				} else if (n4) {
+					store_pair<true>(drow, j0 + jc, v0, v1, n4 > 1, a.out, row);
+					if (n4 > 2)
+						store_pair<true>(drow, j0 + jc + 2, v2, v3, n4 > 3, a.out, row);
+				}
+			}
+			q += sq;
+			i += si;
+			if (q >= quads) {
+				q -= quads;
+				++i;
+			}
+		}
+	} else {
+		// Vertical pass over column pairs: groups of threads split the rows.
+		const unsigned pairs = nc / 2;
+		const unsigned pstride = pairs >= kThreads ? kThreads : pairs;
+		const unsigned groups = kThreads / pstride;
+		const unsigned grp = t / pstride;
+		if (grp < groups) {
+			for (unsigned q = t - grp * pstride; q < pairs; q += kThreads) {
+				for (unsigned i = grp; i < nrows; i += groups) {
+					const unsigned row = i0 + i;
+					const unsigned top = vtop[i] - r0;
+					const uint32_t *cv = vcoef + i * vstride;
 					const float *xr = xin + top * pin + 2 * q;
 					float a0 = 0.0f, a1 = 0.0f;
 					if (vtaps == static_cast<unsigned>(N)) {
@@ -967,12 +1107,10 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 						*reinterpret_cast<float2 *>(reinterpret_cast<float *>(mid_base) + i * pm + 2 * q) = make_float2(a0, a1);
 						continue;
 					}
-					v0 = a0;
-					v1 = a1;
+					const unsigned jc = 2 * q;
+					if (jc < ncols)
+						store_pair<false>(row_ptr(dst, a.dst.stride, row), j0 + jc, a0, a1, jc + 1 < ncols, a.out, row);
 				}
-				const unsigned jc = 2 * q;
-				if (jc < ncols)
-					store_pair<WORD>(row_ptr(dst, a.dst.stride, row), j0 + jc, v0, v1, jc + 1 < ncols, a.out, row);
 			}
 		}
 	}
