@@ -525,7 +525,7 @@ void DeviceProgram::configure_resize(Step &s)
 					mid_rows = s.h_first ? in_rows : th;
 					mid_cols = s.h_first ? tw : in_cols;
 				}
-				const size_t vtab = vtaps ? align_up((align_up(th, 4) + static_cast<size_t>(th) * align_up(vtaps, 4)) * 4, 16) : 0;
+				const size_t vtab = vtaps ? align_up((align_up(th, 4) + static_cast<size_t>(th) * vtaps) * 4, 16) : 0;
 				const size_t bytes = vtab + align_up(elem * in_rows * in_cols, 16) + elem * mid_rows * mid_cols;
 				if (bytes > kLdsBudget)
 					continue;

@@ -645,13 +645,14 @@ __host__ __device__ inline unsigned tiled_vtaps(const ResizeArg &a)
 }
 
 // Start rows occupy the first round4(tile_h) words, then one row of taps per
-// output row at a stride of round4(taps) words (16-byte aligned rows).
+// output row, rows back to back (the same order as the taps in HBM, so a
+// tile's taps are one contiguous copy).
 __host__ __device__ inline unsigned tiled_vtab_words(const ResizeArg &a)
 {
 	const unsigned taps = tiled_vtaps(a);
 	if (!taps)
 		return 0;
-	return ((a.tile_h + 3) & ~3u) + a.tile_h * ((taps + 3) & ~3u);
+	return ((a.tile_h + 3) & ~3u) + a.tile_h * taps;
 }
 
 __host__ __device__ inline size_t tiled_lds_bytes(const ResizeArg &a, size_t *in_off, size_t *mid_off)
@@ -693,13 +694,13 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 	// vtab[nrows + i * taps + k] = tap k (WORD: int16 pairs of the pair
 	// table, whose windows start on an even row; FLOAT: float bits).
 	const unsigned vtaps = tiled_vtaps(a);
-	const unsigned vstride = (vtaps + 3) & ~3u;
+	const unsigned vstride = vtaps;
 	const uint32_t *vtop = vtab;
 	const uint32_t *vcoef = vtab + ((a.tile_h + 3) & ~3u);
 	// The tile's rows of the vertical table are contiguous in HBM (nrows
-	// start rows, then nrows x vtaps taps): the first kVtRegs x kThreads
-	// entries are loaded into registers now, beside the other loads, and
-	// stored to LDS (taps at a padded row stride) after the staging.
+	// start rows, then nrows x vtaps taps) and keep their order in LDS: up to
+	// kVtRegs x kThreads entries are loaded into registers now, beside the
+	// other loads, and stored to LDS after the staging.
 	constexpr int kVtRegs = 4;
 	const unsigned vtot = vtaps ? nrows * (vtaps + 1) : 0;
 	uint32_t vval[kVtRegs];
@@ -710,23 +711,26 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 			val = WORD ? pv.pl[i0 + e] : pv.left[i0 + e];
 			pos = e;
 		} else {
-			const unsigned c = e - nrows, i = c / vtaps, k = c - i * vtaps;
+			const unsigned c = e - nrows;
 			const size_t g = static_cast<size_t>(i0) * vtaps + c;
 			val = WORD ? static_cast<uint32_t>(pv.pc[g]) : __float_as_uint(pv.cf[g]);
-			pos = ((a.tile_h + 3) & ~3u) + i * vstride + k;
+			pos = ((a.tile_h + 3) & ~3u) + c;
 		}
 	};
+	// Register slots past the table (a uniform test) are not used.
 	if (vtot) {
 #pragma unroll
 		for (int u = 0; u < kVtRegs; ++u)
-			vt_entry(min(t + u * kThreads, vtot - 1), vval[u], vpos[u]);
+			if (u == 0 || u * kThreads < vtot)
+				vt_entry(min(t + u * kThreads, vtot - 1), vval[u], vpos[u]);
 	}
 	auto vt_store = [&]() {
 		if (!vtot)
 			return;
 #pragma unroll
 		for (int u = 0; u < kVtRegs; ++u)
-			vtab[vpos[u]] = vval[u];
+			if (u == 0 || u * kThreads < vtot)
+				vtab[vpos[u]] = vval[u];
 		for (unsigned e = t + kVtRegs * kThreads; e < vtot; e += kThreads) {
 			uint32_t v;
 			unsigned p;
@@ -809,19 +813,13 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 							a3 = dot2(mm[k].w, cc[k], a3);
 						}
 					} else {
-						for (unsigned k = 0; k < vtaps; k += 4) {
-							const uint4 c4 = *reinterpret_cast<const uint4 *>(cv + k);
-							const uint32_t cc[4] = { c4.x, c4.y, c4.z, c4.w };
-#pragma unroll
-							for (unsigned u = 0; u < 4; ++u) {
-								if (k + u < vtaps) {
-									const uint4 mm = *reinterpret_cast<const uint4 *>(m + (k + u) * pm);
-									a0 = dot2(mm.x, static_cast<int32_t>(cc[u]), a0);
-									a1 = dot2(mm.y, static_cast<int32_t>(cc[u]), a1);
-									a2 = dot2(mm.z, static_cast<int32_t>(cc[u]), a2);
-									a3 = dot2(mm.w, static_cast<int32_t>(cc[u]), a3);
-								}
-							}
+						for (unsigned k = 0; k < vtaps; ++k) {
+							const uint4 mm = *reinterpret_cast<const uint4 *>(m + k * pm);
+							const int32_t c = static_cast<int32_t>(cv[k]);
+							a0 = dot2(mm.x, c, a0);
+							a1 = dot2(mm.y, c, a1);
+							a2 = dot2(mm.z, c, a2);
+							a3 = dot2(mm.w, c, a3);
 						}
 					}
 					char *drow = row_ptr(dst, a.dst.stride, row);
