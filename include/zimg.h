@@ -496,9 +496,11 @@ zimg_error_code_e zimg_filter_graph_process_device_profiled(const zimg_filter_gr
  * graphs must outlive the jobs object (the captured launches read their
  * coefficient tables). tmp holds
  * zimg_device_jobs_get_tmp_size bytes (may be NULL when that is 0).
- * ZIMG_DEVICE_JOBS_AUTOTUNE times the cost model's best tile shapes for each
- * distinct stage shape on the given planes at creation (results are cached
- * per process); ZIMG_DEVICE_JOBS_NO_GRAPH skips the capture (each run
+ * ZIMG_DEVICE_JOBS_AUTOTUNE times the cost model's candidate tile shapes for
+ * each distinct stage shape on the given planes at creation, in the
+ * persistent kernel with its LDS limit (results are cached per process);
+ * without it the cost model's first choice is taken.
+ * ZIMG_DEVICE_JOBS_NO_GRAPH skips the capture (each run
  * enqueues the launches). `stream` at creation is used for autotuning. */
 typedef struct zimg_device_jobs zimg_device_jobs;
 

@@ -21,6 +21,10 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 dev::PlaneArg plane_of(const DeviceBuffers &b, int idx) { return dev::PlaneArg{ b.ptr[idx], b.stride[idx], b.fstride[idx] }; }
 
+// LDS per workgroup of a persistent launch whose shapes were autotuned: four
+// workgroups per CU (the walker's registers allow four as well).
+constexpr size_t kJobsLdsCap = 40 * 1024;
+
 // Tile shapes measured once per step shape and process.
 std::mutex g_tune_mutex;
 std::map<std::string, TiledConfig> g_tuned;
@@ -79,8 +83,14 @@ size_t DeviceJobs::tmp_size(const std::vector<const DeviceProgram *> &progs)
 	return total;
 }
 
-// Times the cheapest candidate shapes of the cost model on the real planes
-// (a few launches each, HIP events on `stream`) and keeps the fastest.
+// Times the cost model's candidate shapes in the persistent walker itself,
+// as the batch runs them: the shape's tiles over enough frames of the same
+// plane (frame stride 0, so every frame writes the same values to the same
+// place) that each workgroup of the walker's grid takes about 4 tiles, with
+// the walker's LDS. A one-frame launch of the plane on its own would favour
+// small tiles (more workgroups to fill the GPU, more resident per CU), while
+// in the walker every tile pays a serial chain of memory round trips and
+// barriers. Candidates needing more than kJobsLdsCap are skipped.
 TiledConfig DeviceJobs::tune(const DeviceProgram &prog, const DeviceProgram::Step &s, const dev::PlaneArg &src, const dev::PlaneArg &dst,
                              const DeviceProgram::Tables &tabs, hipStream_t stream)
 {
@@ -91,30 +101,76 @@ TiledConfig DeviceJobs::tune(const DeviceProgram &prog, const DeviceProgram::Ste
 		if (it != g_tuned.end())
 			return it->second;
 	}
+	int device = 0, cus = 0;
+	check_hip(hipGetDevice(&device), "hipGetDevice");
+	check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
+	const unsigned grid = static_cast<unsigned>(cus) * 4;
 	hipEvent_t e0, e1;
 	check_hip(hipEventCreate(&e0), "hipEventCreate");
 	check_hip(hipEventCreate(&e1), "hipEventCreate");
+	void *buf = nullptr;
+	size_t buf_bytes = 0;
 	TiledConfig best = s.fast;
 	float best_ms = 0;
-	const size_t ncand = std::min<size_t>(s.fast_options.size(), 4);
-	for (size_t c = 0; c < ncand; ++c) {
-		// Tiles below 8 rows or 64 columns cost more per tile in a persistent launch than they save here.
-		if (c > 0 && (s.fast_options[c].tile_h < 8 || s.fast_options[c].tile_w < 64))
-			continue;
-		const dev::ResizeArg a = prog.resize_arg(s, src, dst, 1, tabs, &s.fast_options[c]);
-		check_hip(static_cast<hipError_t>(dev::launch_resize(a, stream)), "autotune launch");
-		check_hip(hipEventRecord(e0, stream), "hipEventRecord");
-		for (int r = 0; r < 4; ++r)
-			check_hip(static_cast<hipError_t>(dev::launch_resize(a, stream)), "autotune launch");
-		check_hip(hipEventRecord(e1, stream), "hipEventRecord");
-		check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
-		float ms = 0;
-		check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-		if (c == 0 || ms < best_ms) {
-			best_ms = ms;
-			best = s.fast_options[c];
+	bool have = false;
+	try {
+		for (const TiledConfig &c : s.fast_options) {
+			dev::ResizeArg a = prog.resize_arg(s, src, dst, 1, tabs, &c);
+			if (dev::resize_lds_bytes(a) > kJobsLdsCap)
+				continue;
+			const unsigned gx = (a.dst_w + a.tile_w - 1) / a.tile_w, gy = (a.dst_h + a.tile_h - 1) / a.tile_h;
+			const unsigned frames = std::max(1u, (4 * grid + gx * gy - 1) / (gx * gy));
+			a.frames = frames;
+			a.src.fstride = 0;
+			a.dst.fstride = 0;
+			std::vector<dev::JobTile> tl;
+			tl.reserve(static_cast<size_t>(frames) * gx * gy);
+			for (unsigned f = 0; f < frames; ++f)
+				for (unsigned by = 0; by < gy; ++by)
+					for (unsigned bx = 0; bx < gx; ++bx)
+						tl.push_back(dev::JobTile{ 0, bx, by, f });
+			const size_t dbytes = align_up(sizeof(dev::ResizeArg), 256);
+			const size_t need = dbytes + tl.size() * sizeof(dev::JobTile);
+			if (need > buf_bytes) {
+				if (buf)
+					check_hip(hipFree(buf), "hipFree(autotune)");
+				buf = nullptr;
+				check_hip(hipMalloc(&buf, need), "hipMalloc(autotune)");
+				buf_bytes = need;
+			}
+			// The previous candidate's launches have completed (event synchronised).
+			check_hip(hipMemcpy(buf, &a, sizeof(a), hipMemcpyHostToDevice), "hipMemcpy(autotune)");
+			check_hip(hipMemcpy(static_cast<char *>(buf) + dbytes, tl.data(), tl.size() * sizeof(dev::JobTile), hipMemcpyHostToDevice),
+			          "hipMemcpy(autotune)");
+			const auto *descs = static_cast<const dev::ResizeArg *>(buf);
+			const auto *tiles = reinterpret_cast<const dev::JobTile *>(static_cast<char *>(buf) + dbytes);
+			const unsigned n = static_cast<unsigned>(tl.size());
+			const unsigned g = std::min(n, grid);
+			check_hip(static_cast<hipError_t>(dev::launch_resize_jobs(descs, tiles, n, g, kJobsLdsCap, stream)), "autotune launch");
+			check_hip(hipEventRecord(e0, stream), "hipEventRecord");
+			for (int r = 0; r < 3; ++r)
+				check_hip(static_cast<hipError_t>(dev::launch_resize_jobs(descs, tiles, n, g, kJobsLdsCap, stream)), "autotune launch");
+			check_hip(hipEventRecord(e1, stream), "hipEventRecord");
+			check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+			float ms = 0;
+			check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+			// Time per output sample: candidates differ in how many frames fill the grid.
+			ms /= static_cast<float>(frames);
+			if (!have || ms < best_ms) {
+				best_ms = ms;
+				best = c;
+				have = true;
+			}
 		}
+	} catch (...) {
+		if (buf)
+			(void)hipFree(buf);
+		(void)hipEventDestroy(e0);
+		(void)hipEventDestroy(e1);
+		throw;
 	}
+	if (buf)
+		check_hip(hipFree(buf), "hipFree(autotune)");
 	(void)hipEventDestroy(e0);
 	(void)hipEventDestroy(e1);
 	++m_tuned;
