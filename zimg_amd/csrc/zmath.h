@@ -88,16 +88,24 @@ ZCONST static const double kLnC[16] = {
 	0x1.526e57720db08p-3, 0x1.bc2860d22477p-3, 0x1.1058bc8a07ee1p-2, 0x1.4043057b6ee09p-2,
 };
 
+// Addends of the three multiply-adds of the powf polynomials whose other
+// factor is also a constant (log2: the r and r^2 terms' pairs, exp2: the r^2
+// pair). In the fused form a VALU instruction reads at most one scalar
+// operand, so the device reads these from LDS into registers instead of
+// rebuilding a 64-bit literal with two moves at every use.
+constexpr double kPolyAdd[3] = { -0x1.71969a075c67ap-2, -0x1.7154748bef6c8p-1, 0x1.ebfce50fac4f3p-3 };
+
 #if defined(__HIP_DEVICE_COMPILE__)
 // Device code reads per-workgroup LDS copies of the tables (the lookups are
 // per lane, so the copies turn vector memory loads into LDS reads). Every
 // kernel that evaluates these functions calls lds_tables_init() first.
 static __shared__ uint64_t s_exp2_tab[32];
-static __shared__ double s_inv_c[16], s_log2_c[16], s_ln_c[16];
+static __shared__ double s_inv_c[16], s_log2_c[16], s_ln_c[16], s_poly_add[4];
 #define ZM_EXP2TAB(i) s_exp2_tab[i]
 #define ZM_INVC(i) s_inv_c[i]
 #define ZM_LOG2C(i) s_log2_c[i]
 #define ZM_LNC(i) s_ln_c[i]
+#define ZM_POLYADD(i) s_poly_add[i]
 
 __device__ __forceinline__ void lds_tables_init()
 {
@@ -108,6 +116,8 @@ __device__ __forceinline__ void lds_tables_init()
 			s_log2_c[k] = kLog2C[k];
 			s_ln_c[k] = kLnC[k];
 		}
+		if (k < 3)
+			s_poly_add[k] = kPolyAdd[k];
 	}
 	__syncthreads();
 }
@@ -116,6 +126,7 @@ __device__ __forceinline__ void lds_tables_init()
 #define ZM_INVC(i) kInvC[i]
 #define ZM_LOG2C(i) kLog2C[i]
 #define ZM_LNC(i) kLnC[i]
+#define ZM_POLYADD(i) kPolyAdd[i]
 #if defined(__HIPCC__)
 // Host pass of a HIP translation unit: kernels are parsed, never run here.
 __device__ inline void lds_tables_init() {}
@@ -135,6 +146,31 @@ ZHD double mul_add(double a, double b, double c)
 		return a * b + c;
 }
 
+// bitsd(tab + (k << 47)), modulo 2^64: the shifted term's low word is zero,
+// so only the high word changes (no carry). The device adds on the high word
+// of a two-word vector (a plain 64-bit form is compiled into a 64-bit add).
+ZHD double exp2_scale(uint64_t tab, uint32_t k)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+	typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+	u32x2 w = __builtin_bit_cast(u32x2, tab);
+	w.y += k << 15;
+	return __builtin_bit_cast(double, w);
+#else
+	return bitsd(tab + (static_cast<uint64_t>(k) << 47));
+#endif
+}
+
+// kPolyAdd[i]: from the device's LDS copy in the fused form, else the constant.
+template <bool FMA>
+ZHD double poly_add(int i)
+{
+	if constexpr (FMA)
+		return ZM_POLYADD(i);
+	else
+		return kPolyAdd[i];
+}
+
 template <bool FMA = false>
 ZHD float exp2_from_log(double xd, uint32_t sign_bias)
 {
@@ -144,9 +180,8 @@ ZHD float exp2_from_log(double xd, uint32_t sign_bias)
 	uint64_t ki = dbits(kd);
 	kd -= shift;
 	double r = xd - kd;
-	uint64_t t = ZM_EXP2TAB(ki % 32) + ((ki + sign_bias) << 47);
-	double s = bitsd(t);
-	double z = mul_add<FMA>(0x1.c6af84b912394p-5, r, 0x1.ebfce50fac4f3p-3);
+	double s = exp2_scale(ZM_EXP2TAB(ki % 32), static_cast<uint32_t>(ki + sign_bias));
+	double z = mul_add<FMA>(0x1.c6af84b912394p-5, r, poly_add<FMA>(2));
 	double r2 = r * r;
 	double y = mul_add<FMA>(0x1.62e42ff0c52d6p-1, r, 1.0);
 	y = mul_add<FMA>(z, r2, y);
@@ -165,8 +200,8 @@ ZHD double log2_reduced(uint32_t ix)
 	double r = mul_add<FMA>(z, ZM_INVC(i), -1.0);
 	double y0 = ZM_LOG2C(i) + static_cast<double>(k);
 	double r2 = r * r;
-	double y = mul_add<FMA>(0x1.27616c9496e0bp-2, r, -0x1.71969a075c67ap-2);
-	double p = mul_add<FMA>(0x1.ec70a6ca7baddp-2, r, -0x1.7154748bef6c8p-1);
+	double y = mul_add<FMA>(0x1.27616c9496e0bp-2, r, poly_add<FMA>(0));
+	double p = mul_add<FMA>(0x1.ec70a6ca7baddp-2, r, poly_add<FMA>(1));
 	double r4 = r2 * r2;
 	double q = mul_add<FMA>(0x1.71547652ab82bp0, r, y0);
 	q = mul_add<FMA>(p, r2, q);
@@ -185,6 +220,17 @@ ZHD int int_class(uint32_t iy)
 }
 
 ZHD bool zero_inf_nan(uint32_t ix) { return 2 * ix - 1 >= 2u * 0x7f800000u - 1; }
+
+// (w >> 15) & 0xffff; one bit-field extract on the device (the generic form
+// is widened back into a 64-bit mask and compare).
+ZHD uint32_t hi_bits_15_16(uint32_t w)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+	return __builtin_amdgcn_ubfe(w, 15, 16);
+#else
+	return (w >> 15) & 0xffffu;
+#endif
+}
 
 template <bool FMA = false>
 ZHD float powf_t(float x, float y)
@@ -217,7 +263,8 @@ ZHD float powf_t(float x, float y)
 		}
 	}
 	double ylogx = static_cast<double>(y) * log2_reduced<FMA>(ix);
-	if (((dbits(ylogx) >> 47) & 0xffff) >= (dbits(126.0) >> 47)) {
+	// Exponent bits of |ylogx| >= 126 (the test on bits 47-62, on the high word).
+	if (hi_bits_15_16(static_cast<uint32_t>(dbits(ylogx) >> 32)) >= static_cast<uint32_t>(dbits(126.0) >> 47)) {
 		if (ylogx > 0x1.fffffffd1d571p+6) return sign_bias ? -bitsf(0x7f800000u) : bitsf(0x7f800000u);
 		if (ylogx <= -150.0) return sign_bias ? -0.0f : 0.0f;
 	}
