@@ -114,6 +114,20 @@ def device_batch(torch, graph, planes, out_shapes, frames, stream=None, inputs=N
     return launch, keep
 
 
+_PIPE_STREAMS = []
+
+
+def pipe_streams(torch, n):
+    """The streams batches alternate on, created once per process and reused
+    by every workload: each new stream is bound to one of the device's few
+    hardware queues when it is created, and two streams that land on the
+    same queue run one after the other, so later workloads must not get a
+    fresh pair."""
+    while len(_PIPE_STREAMS) < n:
+        _PIPE_STREAMS.append(torch.cuda.Stream())
+    return _PIPE_STREAMS[:n]
+
+
 def time_steps_pipelined(torch, launches, streams, steps, warmup):
     """Seconds for `steps` launches issued alternately on len(streams)
     streams (each launcher its own outputs and tmp): consecutive batches are
@@ -237,7 +251,7 @@ def measure(torch, name, frames, steps, warmup, precision=0, approx=False, surro
     single = elapsed
     if streams > 1:
         # The same input frames, each stream its own outputs and tmp.
-        sts = [torch.cuda.Stream() for _ in range(streams)]
+        sts = pipe_streams(torch, streams)
         launches = [device_batch(torch, g, planes, outs, frames, stream=st, inputs=launch.inputs) for st in sts]
         elapsed = time_steps_pipelined(torch, [l for l, _ in launches], sts, steps, warmup)
         del launches
