@@ -26,7 +26,12 @@ kernel when it is not HBM (VALU issue, from the same counters).
 cpu_baseline: the oracle (the project's C restatement of the reference's C
 path) on this host: single-thread median of >= 10 runs, and frame-parallel
 over the usable cores (one frame per thread, the reference harness pattern of
-graphapp.cpp:355-426), with the CPU model.
+graphapp.cpp:355-426), with the CPU model. For batch64: jobs/s of
+single-thread passes over the 64-job list and of the jobs spread over the
+cores.
+
+Two-stream workloads alternate batches over one pair of streams created once
+per process.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME [--variant FORM]] [--no-cpu]
 """
@@ -418,6 +423,50 @@ def cpu_baseline(name, runs, full):
     return out
 
 
+def cpu_batch_baseline(runs):
+    """The 64-job list through the oracle chain (tests/pipelines.py batch_job)
+    on this host: jobs/s from the median of single-thread passes over the
+    whole list, and job-parallel over the usable cores (median of `runs`)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import pipelines as P
+    from zimg_amd import workloads as W
+    jobs = W.batch_jobs(64)
+    cache, planes = {}, []
+    for j in jobs:
+        key = (tuple(j["src"]), j["bits"])
+        if key not in cache:
+            cache[key] = W.batch_job_planes(j, seed=900 + len(cache))
+        planes.append(cache[key])
+
+    def one_pass():
+        for j, pl in zip(jobs, planes):
+            P.batch_job(j, pl)
+    one_pass()  # warm-up
+    n1 = max(3, runs // 3)
+    single = []
+    for _ in range(n1):
+        t0 = time.perf_counter()
+        one_pass()
+        single.append(time.perf_counter() - t0)
+    med1 = statistics.median(single)
+    out = {"value": round(len(jobs) / med1, 2), "unit": "jobs/s", "cores": 1, "kind": "port",
+           "sample": f"batch64: median of {n1} single-thread passes over the 64-job list through the oracle (tests/pipelines.py)",
+           "cpu": cpu_model()}
+    from concurrent.futures import ThreadPoolExecutor
+    n = usable_cores()
+    par = []
+    with ThreadPoolExecutor(max_workers=n) as ex:
+        list(ex.map(lambda a: P.batch_job(*a), zip(jobs, planes)))
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            list(ex.map(lambda a: P.batch_job(*a), zip(jobs, planes)))
+            par.append(time.perf_counter() - t0)
+    medn = statistics.median(par)
+    out["job_parallel"] = {"value": round(len(jobs) / medn, 2), "cores": n,
+                           "sample": f"median of {runs} runs of the 64 jobs, one job per thread at a time"}
+    return out
+
+
 def variant_applies(name, variant):
     """Whether a workload has the form `variant` (the exact path always)."""
     return variant in ("all", "exact") or (variant in ("fp16", "bf16") and name in ("fp32_down", "fp32_up")) or \
@@ -465,7 +514,8 @@ def main():
                 learned["gain_vs_exact"] = round(learned["fps"] / results[name]["fps"], 3)
                 results[name]["learned"] = learned
         if not args.no_cpu:
-            results[name]["cpu_baseline"] = cpu_baseline(name, args.cpu_runs, full=name in ("fhd_709_to_2020", "uhd_lanczos"))
+            results[name]["cpu_baseline"] = cpu_batch_baseline(args.cpu_runs) if name == "batch64" else \
+                cpu_baseline(name, args.cpu_runs, full=name in ("fhd_709_to_2020", "uhd_lanczos"))
         print(f"# {name}: {results[name]['fps']} {results[name].get('unit', 'frames/s')}", file=sys.stderr, flush=True)
     primary = todo[0]
     m = results[primary]

@@ -5,8 +5,8 @@ restatement for every stage. They are written from the reference's rules, not
 from this library's planner, so the two derivations check each other."""
 import numpy as np
 
-from oracle_lib import (BICUBIC, FLOAT, LANCZOS, WORD, colorspace_apply, colorspace_ops, dither, resize_plane,
-                        scale_offset, to_float)
+from oracle_lib import (BICUBIC, BYTE, FLOAT, LANCZOS, WORD, colorspace_apply, colorspace_ops, dither, left_shift,
+                        resize_plane, scale_offset, to_float)
 
 # internal colourspace enums (colorspace.h)
 M_709, M_2020 = 3, 7
@@ -22,6 +22,31 @@ def word_resize(src, dst_w, dst_h, ftype=LANCZOS, a=3.0, b=0.0, depth=16):
 
 def float_resize(src, dst_w, dst_h, ftype, a, b):
     return resize_plane(src, ftype, a, b, dst_w, dst_h)
+
+
+def batch_job(job, planes):
+    """One job of the batched form (zimg_amd.workloads.batch_job_graph): each
+    4:2:0 plane (8-bit promoted to WORD by a left shift, or 10-bit WORD)
+    resized as WORD with the job's filter (chroma sited LEFT, so shifted
+    horizontally by 1/4 - 1/4 x scale), then ordered (Bayer) dither to 8 bits,
+    as the reference's graph for that format pair."""
+    (sw, sh), (dw, dh), bits = job["src"], job["dst"], job["bits"]
+    filt = {"bilinear": (1, 0.0, 0.0), "bicubic": (2, 0.0, 0.5), "spline36": (4, 0.0, 0.0), "lanczos3": (5, 3.0, 0.0)}[job["filter"]]
+    res = []
+    for q, plane in enumerate(planes):
+        chroma = q > 0
+        if bits == 8:
+            x, wd = left_shift(plane, BYTE, WORD, 8), 16
+        else:
+            x, wd = plane, 10
+        cw_s, ch_s = plane.shape[1], plane.shape[0]
+        cw_d, ch_d = (dw // 2, dh // 2) if chroma else (dw, dh)
+        shift_w = (0.25 - 0.25 / (cw_d / cw_s)) if chroma else 0.0
+        r = resize_plane(x, filt[0], filt[1], filt[2], cw_d, ch_d, shift_w=shift_w, shift_h=0.0, subwidth=float(cw_s),
+                         subheight=float(ch_s), depth=wd)
+        sc, of = scale_offset((WORD, wd, 0, int(chroma), 0), (BYTE, 8, 0, int(chroma), 0))
+        res.append(dither(r, WORD, BYTE, sc, of, 8, dither_type=1, plane=q))
+    return res
 
 
 def yuv420_10_709_to_2020(Y, U, V, approx=0):

@@ -4,8 +4,9 @@ own launch sequence produces, and that equals the oracle chain."""
 import numpy as np
 import pytest
 
+import pipelines as P
 import zimg_amd as z
-from oracle_lib import BYTE, WORD, dither, left_shift, resize_plane, scale_offset
+from oracle_lib import BYTE, WORD, dither, resize_plane, scale_offset
 from zimg_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -48,23 +49,7 @@ def _read(ob):
 
 
 def _oracle(job, pl):
-    (sw, sh), (dw, dh), bits = job["src"], job["dst"], job["bits"]
-    filt = {"bilinear": (1, 0.0, 0.0), "bicubic": (2, 0.0, 0.5), "spline36": (4, 0.0, 0.0), "lanczos3": (5, 3.0, 0.0)}[job["filter"]]
-    res = []
-    for q, plane in enumerate(pl):
-        chroma = q > 0
-        if bits == 8:
-            x, wd = left_shift(plane, BYTE, WORD, 8), 16
-        else:
-            x, wd = plane, 10
-        cw_s, ch_s = plane.shape[1], plane.shape[0]
-        cw_d, ch_d = (dw // 2, dh // 2) if chroma else (dw, dh)
-        shift_w = (0.25 - 0.25 / (cw_d / cw_s)) if chroma else 0.0
-        r = resize_plane(x, filt[0], filt[1], filt[2], cw_d, ch_d, shift_w=shift_w, shift_h=0.0, subwidth=float(cw_s),
-                         subheight=float(ch_s), depth=wd)
-        sc, of = scale_offset((WORD, wd, 0, int(chroma), 0), (BYTE, 8, 0, int(chroma), 0))
-        res.append(dither(r, WORD, BYTE, sc, of, 8, dither_type=1, plane=q))
-    return res
+    return P.batch_job(job, pl)
 
 
 @pytest.mark.parametrize("flags", [0, z.DEVICE_JOBS_AUTOTUNE, z.DEVICE_JOBS_NO_GRAPH])
@@ -164,8 +149,6 @@ def test_jobs_from_fresh_graphs_with_lazy_tables():
     the HIP graph with every table they use already resident."""
     import torch
     print("case: fresh graphs, precision-form FLOAT job + WORD upscale with error diffusion", flush=True)
-    from oracle_lib import resize_plane, scale_offset as so
-    import pipelines as P
     sw, sh, dw, dh = 320, 180, 640, 360
     fsrc = W.fp32_plane(sw, sh, seed=4)
     s, d, p = W.fp32_formats(sw, sh, dw, dh, z.RESIZE_BICUBIC, 0.0, 0.5)
@@ -192,7 +175,6 @@ def test_jobs_from_fresh_graphs_with_lazy_tables():
     exact = P.float_resize(fsrc, dw, dh, z.RESIZE_BICUBIC, 0.0, 0.5)
     assert np.max(np.abs(got_f - exact)) < 2e-6
     r = resize_plane(wsrc, 5, 3.0, 0.0, dw, dh, depth=16)
-    sc, of = so((1, 16, 0, 0, 0), (0, 8, 1, 0, 0))
-    from oracle_lib import WORD, BYTE
+    sc, of = scale_offset((1, 16, 0, 0, 0), (0, 8, 1, 0, 0))
     want_w = dither(r, WORD, BYTE, sc, of, 8, dither_type=3)
     np.testing.assert_array_equal(wo.cpu().numpy().reshape(dh, dw), want_w)
