@@ -145,7 +145,7 @@ TiledConfig DeviceJobs::tune(const DeviceProgram &prog, const DeviceProgram::Ste
 			const auto *descs = static_cast<const dev::ResizeArg *>(buf);
 			const auto *tiles = reinterpret_cast<const dev::JobTile *>(static_cast<char *>(buf) + dbytes);
 			const unsigned n = static_cast<unsigned>(tl.size());
-			const unsigned g = std::min(n, grid);
+			const unsigned g = std::min((n + dev::kJobRun - 1) / dev::kJobRun, grid);
 			check_hip(static_cast<hipError_t>(dev::launch_resize_jobs(descs, tiles, n, g, kJobsLdsCap, stream)), "autotune launch");
 			check_hip(hipEventRecord(e0, stream), "hipEventRecord");
 			for (int r = 0; r < 3; ++r)
@@ -222,7 +222,7 @@ DeviceJobs::DeviceJobs(std::vector<Job> jobs, void *tmp, unsigned flags, void *s
 	if (off && !m_tmp)
 		fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "device tmp buffer required");
 
-	// Most expensive tiles first; workgroups take tiles round-robin.
+	// Most expensive tiles first; workgroups take runs of kJobRun tiles round-robin.
 	std::vector<size_t> order(tiles.size());
 	for (size_t k = 0; k < order.size(); ++k)
 		order[k] = k;
@@ -247,7 +247,7 @@ DeviceJobs::DeviceJobs(std::vector<Job> jobs, void *tmp, unsigned flags, void *s
 		check_hip(hipGetDevice(&device), "hipGetDevice");
 		check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
 		const size_t per_cu = std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / std::max<size_t>(m_lds, 1)));
-		m_grid = static_cast<unsigned>(std::min<size_t>(m_ntiles, static_cast<size_t>(cus) * per_cu));
+		m_grid = static_cast<unsigned>(std::min<size_t>((m_ntiles + dev::kJobRun - 1) / dev::kJobRun, static_cast<size_t>(cus) * per_cu));
 	}
 
 	if (!(m_flags & NO_GRAPH)) {

@@ -263,6 +263,8 @@ int launch_mfma_fused(const MfmaFusedArg &a, void *stream);
 int launch_resize_imma(const ImmaArg &a, void *stream);
 // Largest register capacity (tap pairs) the persistent batched kernel handles.
 constexpr int kJobsMaxTaps = 12;
+// Consecutive tiles of the persistent walker's list a workgroup takes at a time.
+constexpr unsigned kJobRun = 8;
 // Persistent launch over WORD tiled resize descriptors (device arrays):
 // `grid` workgroups walk the tile list with a stride of `grid`.
 int launch_resize_jobs(const ResizeArg *descs, const JobTile *tiles, unsigned ntiles, unsigned grid, size_t lds, void *stream);

@@ -1461,16 +1461,23 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
                                                           unsigned ntiles)
 {
 	extern __shared__ __align__(16) unsigned char smem[];
-	for (unsigned it = blockIdx.x; it < ntiles; it += gridDim.x) {
-		const JobTile &jt = tiles[it];
-		const ResizeArg &a = descs[jt.desc];
-		switch (a.fwh) {
+	// Workgroup b takes runs of kJobRun consecutive tiles: runs b, b + G,
+	// b + 2G, ... (G = gridDim.x). A run's tiles are neighbours of one
+	// descriptor (the list is sorted by cost, stably), so its fields stay in
+	// the scalar cache and the shared halo rows in L2.
+	for (unsigned run = blockIdx.x; run * kJobRun < ntiles; run += gridDim.x) {
+		const unsigned end = min(ntiles, (run + 1) * kJobRun);
+		for (unsigned it = run * kJobRun; it < end; ++it) {
+			const JobTile &jt = tiles[it];
+			const ResizeArg &a = descs[jt.desc];
+			switch (a.fwh) {
 #define ZAMD_JOB(n) case n: if (a.h_first) resize_tile<true, n, true>(a, jt.bx, jt.by, jt.f, smem); else resize_tile<true, n, false>(a, jt.bx, jt.by, jt.f, smem); break;
-		ZAMD_JOB(2) ZAMD_JOB(3) ZAMD_JOB(4) ZAMD_JOB(5) ZAMD_JOB(6) ZAMD_JOB(8) ZAMD_JOB(12)
+			ZAMD_JOB(2) ZAMD_JOB(3) ZAMD_JOB(4) ZAMD_JOB(5) ZAMD_JOB(6) ZAMD_JOB(8) ZAMD_JOB(12)
 #undef ZAMD_JOB
-		default: break;
+			default: break;
+			}
+			__syncthreads();
 		}
-		__syncthreads();
 	}
 }
 
