@@ -371,3 +371,17 @@ def test_packed_layout_checks():
     planar = g.device_packed_tmp_size(2, z.PACKING_PLANAR, z.PACKING_PLANAR)
     nv12 = g.device_packed_tmp_size(2, z.PACKING_NV12, z.PACKING_NV12)
     assert planar == g.device_tmp_size(2) and nv12 >= planar + 2 * 2 * (256 * 48 + 2 * 256 * 24)
+
+
+def test_error_diffusion_width_limit():
+    """Error diffusion takes planes up to 16 x 2048 - 129 = 32639 samples wide
+    (the wavefront's LDS ring of 2048 entries per wave); one sample wider is
+    refused when the graph is built, not at launch."""
+    def graph(w):
+        return z.FilterGraph(z.image_format(width=w, height=4, pixel_type=z.PIXEL_FLOAT, depth=32),
+                             z.image_format(width=w, height=4, pixel_type=z.PIXEL_BYTE, depth=8, pixel_range=z.RANGE_FULL),
+                             z.builder_params(dither_type=z.DITHER_ERROR_DIFFUSION))
+    assert "error_diffusion" in graph(32639).describe()
+    with pytest.raises(z.ZimgError) as e:
+        graph(32640)
+    assert e.value.code == z.ERROR_UNSUPPORTED_OPERATION

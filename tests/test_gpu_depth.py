@@ -148,3 +148,21 @@ def test_half_yuv_to_half_rgb():
     r, gg, b = colorspace_apply(ops, *(half_to_float(p) for p in planes))
     for q, want in enumerate((r, gg, b)):
         _assert_bits(outs[q], to_half(want, FLOAT), f"plane {q}")
+
+
+@pytest.mark.parametrize("w", [20000, 32639])
+def test_error_diffusion_wide_planes(w):
+    """Planes wider than a 1024-entry LDS ring per wave (up to the 32639-sample
+    limit: 2048 entries, 128 KiB of LDS) through the error-diffusion
+    wavefront, three strips so the rows cross between waves through the
+    ring, against the sequential oracle."""
+    h = 150
+    print(f"case: GREY FLOAT {w}x{h} -> BYTE error diffusion", flush=True)
+    rng = np.random.default_rng(23)
+    src = rng.random((h, w), dtype=np.float32)
+    g = z.FilterGraph(_grey(w, h, z.PIXEL_FLOAT, 32), _grey(w, h, z.PIXEL_BYTE, 8, pixel_range=z.RANGE_FULL),
+                      z.builder_params(dither_type=z.DITHER_ERROR_DIFFUSION))
+    (out, *_) = gpu_util.run_graph(g, [src, None, None, None], [(h, w, np.uint8), None, None, None])
+    sc, of = scale_offset((FLOAT, 32, 1, 0, 0), (BYTE, 8, 1, 0, 0))
+    want = dither(src, FLOAT, BYTE, sc, of, 8, dither_type=3, plane=0)
+    _assert_bits(out, want, f"error diffusion {w} wide")

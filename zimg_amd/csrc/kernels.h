@@ -170,9 +170,12 @@ struct PackArg {
 };
 int launch_pack(const PackArg &a, void *stream);
 
-// Widest plane the error-diffusion wavefront takes (its LDS ring of rows
-// handed between strips holds at most 1024 floats per wave).
-constexpr unsigned kEdMaxWidth = 16257;
+// Widest plane the error-diffusion wavefront takes. The LDS ring of rows
+// handed between strips holds up to kEdMaxRing floats per wave (16 waves:
+// 128 KiB), and a plane W wide needs a ring of at least (W + 144) / 16
+// entries (rounded down), so W <= 16 kEdMaxRing - 129.
+constexpr unsigned kEdMaxRing = 2048;
+constexpr unsigned kEdMaxWidth = 16 * kEdMaxRing - 129;
 
 // Error diffusion over up to 4 planes of equal size in one launch (grid.y = plane).
 struct EdArg {

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <type_traits>
 
@@ -1601,6 +1602,7 @@ __global__ void __launch_bounds__(kThreads) k_unresize_v(UnresizeArg a)
 // with paired accesses, and the ring is read and written once.
 constexpr int kEdWaves = 16;
 constexpr int kEdChunk = 16;
+static_assert(kEdMaxWidth == kEdWaves * kEdMaxRing - 129, "error-diffusion width limit follows the ring limit");
 constexpr int kEdMinDelay = 2 * 63 + 2 + kEdChunk;
 
 struct EdGeom {
@@ -1916,12 +1918,24 @@ int launch_error_diffusion(const EdArg &a, void *stream)
 	unsigned ring = 256;
 	while (ring < delay)
 		ring *= 2;
-	if (ring > 1024)
-		return static_cast<int>(hipErrorInvalidValue); // widths above 16257 samples
+	if (ring > kEdMaxRing)
+		return static_cast<int>(hipErrorInvalidValue); // widths above kEdMaxWidth
 	EdArg b = a;
 	b.delay = delay;
 	b.ring = ring;
 	const size_t lds = sizeof(float) * static_cast<size_t>(kEdWaves) * ring;
+	if (lds > 64 * 1024) {
+		// Rings above 1024 entries take more than the default 64 KiB of
+		// dynamic LDS (gfx950 has 160 KiB per CU).
+		static std::atomic<bool> raised{ false };
+		if (!raised.load(std::memory_order_acquire)) {
+			const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_error_diffusion),
+			                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(sizeof(float) * kEdWaves * kEdMaxRing));
+			if (e != hipSuccess)
+				return static_cast<int>(e);
+			raised.store(true, std::memory_order_release);
+		}
+	}
 	hipLaunchKernelGGL(k_error_diffusion, dim3(b.frames, b.planes), dim3(kEdWaves * 64), lds, static_cast<hipStream_t>(stream), b);
 	return static_cast<int>(hipGetLastError());
 }
