@@ -1049,8 +1049,90 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 				++i;
 			}
 		}
+	} else if constexpr (N <= 12) {
+		// FLOAT vertical pass: items of four columns x one output row spread
+		// over all threads (float4 reads of the staged rows; the taps read
+		// once per four columns). Each sum is the reference's: from 0, tap
+		// k added in order, no contraction.
+		const float *xf = reinterpret_cast<const float *>(xin);
+		const unsigned quads = nc / 4, items = quads * nrows;
+		const unsigned sq = kThreads % quads, si = kThreads / quads;
+		unsigned q = t % quads, i = t / quads;
+		const bool wide = a.out.op == 0 && a.out.type == 3;
+		for (unsigned it = t; it < items; it += kThreads) {
+			const float *m = xf + (vtop[i] - r0) * pin + 4 * q;
+			const uint32_t *cv = vcoef + i * vstride;
+			float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+			if (vtaps == static_cast<unsigned>(N)) {
+				float4 mm[N];
+				float cc[N];
+#pragma unroll
+				for (int k = 0; k < N; ++k) {
+					mm[k] = *reinterpret_cast<const float4 *>(m + k * pin);
+					cc[k] = __uint_as_float(cv[k]);
+				}
+#pragma unroll
+				for (int k = 0; k < N; ++k) {
+					a0 = a0 + cc[k] * mm[k].x;
+					a1 = a1 + cc[k] * mm[k].y;
+					a2 = a2 + cc[k] * mm[k].z;
+					a3 = a3 + cc[k] * mm[k].w;
+				}
+			} else if (vtaps < static_cast<unsigned>(N)) {
+				// Taps past vtaps leave the sums as they are.
+				float4 mm[N];
+				float cc[N];
+#pragma unroll
+				for (int k = 0; k < N; ++k) {
+					const unsigned kk = min(static_cast<unsigned>(k), vtaps - 1);
+					mm[k] = *reinterpret_cast<const float4 *>(m + kk * pin);
+					cc[k] = __uint_as_float(cv[kk]);
+				}
+#pragma unroll
+				for (int k = 0; k < N; ++k) {
+					const bool on = static_cast<unsigned>(k) < vtaps;
+					const float s0 = a0 + cc[k] * mm[k].x, s1 = a1 + cc[k] * mm[k].y;
+					const float s2 = a2 + cc[k] * mm[k].z, s3 = a3 + cc[k] * mm[k].w;
+					a0 = on ? s0 : a0;
+					a1 = on ? s1 : a1;
+					a2 = on ? s2 : a2;
+					a3 = on ? s3 : a3;
+				}
+			} else {
+				for (unsigned k = 0; k < vtaps; ++k) {
+					const float4 mm = *reinterpret_cast<const float4 *>(m + k * pin);
+					const float c = __uint_as_float(cv[k]);
+					a0 += c * mm.x;
+					a1 += c * mm.y;
+					a2 += c * mm.z;
+					a3 += c * mm.w;
+				}
+			}
+			if (a.passes == 2) {
+				*reinterpret_cast<float4 *>(reinterpret_cast<float *>(mid_base) + i * pm + 4 * q) = make_float4(a0, a1, a2, a3);
+			} else {
+				// Vertical only: cb = j0, four output columns j0 + 4q ...
+				const unsigned row = i0 + i, jc = 4 * q;
+				const unsigned n4 = jc < ncols ? min(4u, ncols - jc) : 0;
+				char *drow = row_ptr(dst, a.dst.stride, row);
+				if (n4 == 4 && wide) {
+					*reinterpret_cast<float4 *>(drow + 4 * static_cast<size_t>(j0 + jc)) = make_float4(a0, a1, a2, a3);
+				} else if (n4) {
+					store_pair<false>(drow, j0 + jc, a0, a1, n4 > 1, a.out, row);
+					if (n4 > 2)
+						store_pair<false>(drow, j0 + jc + 2, a2, a3, n4 > 3, a.out, row);
+				}
+			}
+			q += sq;
+			i += si;
+			if (q >= quads) {
+				q -= quads;
+				++i;
+			}
+		}
 	} else {
-		// Vertical pass over column pairs: groups of threads split the rows.
+		// Vertical pass over column pairs (wide filters: fewer registers per
+		// item): groups of threads split the rows.
 		const unsigned pairs = nc / 2;
 		const unsigned pstride = pairs >= kThreads ? kThreads : pairs;
 		const unsigned groups = kThreads / pstride;
