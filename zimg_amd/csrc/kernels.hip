@@ -1768,6 +1768,11 @@ __global__ void __launch_bounds__(kEdWaves * 64) k_error_diffusion(EdArg a)
 	float xin[kEdChunk], xnext[kEdChunk];
 	EdGeom gn = ed_geom(0, w, lane, D, nstrips, W, H);
 	ed_load16(row_ptr(src, sp.stride, gn.row_ok ? gn.row : 0), gn.jb, W, a.in.type, gn.strip >= 0 && gn.row_ok, xnext);
+	// The first chunk's inputs arrive before the loop. Without this wait the
+	// loop's entry state holds them in flight, and the compiler waits for all
+	// memory operations (the next chunk's prefetch and the stores included)
+	// before the first steps of every chunk.
+	__builtin_amdgcn_s_waitcnt(0);
 
 	for (int ch = 0; ch < nchunks; ++ch) {
 		const int T0 = ch * kEdChunk;
