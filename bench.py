@@ -31,7 +31,9 @@ single-thread passes over the 64-job list and of the jobs spread over the
 cores.
 
 Two-stream workloads alternate batches over one pair of streams created once
-per process.
+per process. Before the timed region each graph's tiled resize steps are
+autotuned on the workload's own buffers (zimg_filter_graph_autotune_device:
+candidate tile shapes timed per step, the fastest kept; outputs unchanged).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME [--variant FORM]] [--no-cpu]
 """
@@ -114,7 +116,11 @@ def device_batch(torch, graph, planes, out_shapes, frames, stream=None, inputs=N
     def profiled():
         return graph.process_device_profiled(frames, src, src_fs, dst, dst_fs, tmp.data_ptr(), hs)
 
+    def autotune():
+        return graph.autotune_device(frames, src, src_fs, dst, dst_fs, tmp.data_ptr(), hs)
+
     launch.profiled = profiled
+    launch.autotune = autotune
     launch.inputs = ins
     return launch, keep
 
@@ -252,6 +258,9 @@ def measure(torch, name, frames, steps, warmup, precision=0, approx=False, surro
         g.set_resize_precision(precision)
         dtype += f" (resize on MFMA, {'fp16' if precision == 1 else 'bf16'} hi+lo, FP32 accumulate)"
     launch, keep = device_batch(torch, g, planes, outs, frames)
+    # Tile shapes of the tiled resize steps timed on these buffers before the
+    # timed region (zimg_filter_graph_autotune_device); outputs are the same.
+    tuned = launch.autotune()
     elapsed = time_steps(torch, launch, steps, warmup)
     single = elapsed
     if streams > 1:
@@ -287,7 +296,7 @@ def measure(torch, name, frames, steps, warmup, precision=0, approx=False, surro
     del keep
     torch.cuda.empty_cache()
     r = {"fps": round(frames * steps / elapsed, 1), "ms_per_step": round(per_step * 1e3, 4), "frames": frames, "launches": len(stages),
-         "dtype": dtype, "roofline": roof}
+         "dtype": dtype, "roofline": roof, "autotuned_steps": tuned}
     if streams > 1:
         r["streams"] = streams
         r["one_stream_fps"] = round(frames * steps / single, 1)

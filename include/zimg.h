@@ -422,6 +422,19 @@ typedef enum zimg_resize_precision_e {
 ZIMG_VISIBILITY
 zimg_error_code_e zimg_filter_graph_set_resize_precision(zimg_filter_graph *ptr, int mode);
 
+/* device extension, no counterpart in the reference (as a tile-shape
+ * autotuner for callers that process many batches of one format pair): time
+ * the candidate tile shapes of the graph's tiled resize steps on these
+ * buffers with `count` frames (arguments as zimg_filter_graph_process_device_batch;
+ * dst is overwritten) and keep the fastest for later calls. Outputs are the
+ * same with any shape. *changed (may be NULL) receives the number of steps
+ * whose shape changed. Not safe against concurrent processing of the same
+ * graph. */
+ZIMG_VISIBILITY
+zimg_error_code_e zimg_filter_graph_autotune_device(zimg_filter_graph *ptr, unsigned count, const zimg_image_buffer_const *src,
+                                                   const ptrdiff_t src_frame_stride[4], const zimg_image_buffer *dst,
+                                                   const ptrdiff_t dst_frame_stride[4], void *tmp, void *stream, unsigned *changed);
+
 /* device extension, no counterpart in the reference: the allocation-free form
  * of zimg_filter_graph_process_device_batch, for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops, harnesses

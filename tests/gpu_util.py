@@ -24,10 +24,11 @@ def _upload(torch, arr, pad_bytes, pitch_align=4):
     return t, pitch, h * pitch
 
 
-def run_graph(graph, src_planes, dst_specs, frames=None, pad_bytes=0, stream=0, pitch_align=4):
+def run_graph(graph, src_planes, dst_specs, frames=None, pad_bytes=0, stream=0, pitch_align=4, autotune=False):
     """src_planes: list of 4 (numpy (h,w) or (F,h,w), or None).
     dst_specs: list of 4 ((h, w, dtype) or None). Row pitches are rounded up to
-    pitch_align bytes. Returns list of numpy outputs."""
+    pitch_align bytes. autotune: time the graph's tile shapes on these buffers
+    first (zimg_filter_graph_autotune_device). Returns list of numpy outputs."""
     torch = _torch()
     keep = []
     src_bufs, src_fs = [None] * 4, [0] * 4
@@ -58,6 +59,8 @@ def run_graph(graph, src_planes, dst_specs, frames=None, pad_bytes=0, stream=0, 
     tmp = torch.empty(max(tmp_size, 1), dtype=torch.uint8, device="cuda")
     src = z.image_buffer(src_bufs)
     dst = z.image_buffer(dst_bufs)
+    if autotune:
+        graph.autotune_device(frames, src, src_fs, dst, dst_fs, tmp.data_ptr(), stream)
     if frames == 1:
         graph.process_device(src, dst, tmp.data_ptr(), stream)
     else:

@@ -385,3 +385,13 @@ def test_error_diffusion_width_limit():
     with pytest.raises(z.ZimgError) as e:
         graph(32640)
     assert e.value.code == z.ERROR_UNSUPPORTED_OPERATION
+
+
+def test_autotune_device_rejects_null_graph():
+    """zimg_filter_graph_autotune_device checks its graph before touching the device."""
+    import ctypes
+    n = ctypes.c_uint(7)
+    ss = (ctypes.c_ssize_t * 4)()
+    rc = z.lib().zimg_filter_graph_autotune_device(None, 1, None, ss, None, ss, None, None, ctypes.byref(n))
+    assert rc == z.ERROR_ILLEGAL_ARGUMENT
+    assert n.value == 7

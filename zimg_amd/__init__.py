@@ -66,6 +66,7 @@ EXPORTS = [
     "zimg_image_format_default", "zimg_graph_builder_params_default", "zimg_filter_graph_build",
     "zimg_filter_graph_get_device_tmp_size", "zimg_filter_graph_process_device",
     "zimg_filter_graph_process_device_batch", "zimg_filter_graph_process_device_batch_noalloc", "zimg_filter_graph_describe",
+    "zimg_filter_graph_autotune_device",
     "zimg_filter_graph_get_device_packed_tmp_size", "zimg_filter_graph_process_device_packed",
     "zimg_filter_graph_set_resize_precision", "zimg_filter_graph_set_transfer_surrogates",
     "zimg_filter_graph_process_device_profiled",
@@ -190,6 +191,10 @@ def lib():
         L.zimg_filter_graph_process_device_batch.argtypes = [vp, ctypes.c_uint, ctypes.POINTER(ImageBufferStruct),
                                                              ctypes.POINTER(ctypes.c_ssize_t), ctypes.POINTER(ImageBufferStruct),
                                                              ctypes.POINTER(ctypes.c_ssize_t), vp, vp]
+        L.zimg_filter_graph_autotune_device.restype = ctypes.c_int
+        L.zimg_filter_graph_autotune_device.argtypes = [vp, ctypes.c_uint, ctypes.POINTER(ImageBufferStruct),
+                                                        ctypes.POINTER(ctypes.c_ssize_t), ctypes.POINTER(ImageBufferStruct),
+                                                        ctypes.POINTER(ctypes.c_ssize_t), vp, vp, ctypes.POINTER(ctypes.c_uint)]
         L.zimg_filter_graph_process_device_batch_noalloc.restype = ctypes.c_int
         L.zimg_filter_graph_process_device_batch_noalloc.argtypes = [vp, ctypes.c_uint, ctypes.POINTER(ImageBufferStruct),
                                                                      ctypes.POINTER(ctypes.c_ssize_t), ctypes.POINTER(ImageBufferStruct),
@@ -407,6 +412,17 @@ class FilterGraph:
         ss = (ctypes.c_ssize_t * 4)(*src_frame_stride)
         ds = (ctypes.c_ssize_t * 4)(*dst_frame_stride)
         check(lib().zimg_filter_graph_process_device_batch(self._h, count, ctypes.byref(src), ss, ctypes.byref(dst), ds, tmp, stream))
+
+    def autotune_device(self, count, src, src_frame_stride, dst, dst_frame_stride, tmp=0, stream=0):
+        """zimg_filter_graph_autotune_device: time the tiled resize steps' candidate
+        tile shapes on these buffers (dst is overwritten) and keep the fastest;
+        returns how many steps changed shape."""
+        ss = (ctypes.c_ssize_t * 4)(*src_frame_stride)
+        ds = (ctypes.c_ssize_t * 4)(*dst_frame_stride)
+        n = ctypes.c_uint()
+        check(lib().zimg_filter_graph_autotune_device(self._h, count, ctypes.byref(src), ss, ctypes.byref(dst), ds, tmp, stream,
+                                                      ctypes.byref(n)))
+        return n.value
 
 
 NOALLOC_SCRATCH_BYTES = 128 << 20

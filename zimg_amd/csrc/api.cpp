@@ -811,6 +811,25 @@ zimg_error_code_e zimg_filter_graph_process_device_batch(const zimg_filter_graph
 	}
 }
 
+zimg_error_code_e zimg_filter_graph_autotune_device(zimg_filter_graph *ptr, unsigned count, const zimg_image_buffer_const *src,
+                                                   const ptrdiff_t src_frame_stride[4], const zimg_image_buffer *dst,
+                                                   const ptrdiff_t dst_frame_stride[4], void *tmp, void *stream, unsigned *changed)
+{
+	try {
+		if (!ptr || !ptr->prog)
+			fail(ZIMG_ERROR_ILLEGAL_ARGUMENT, "null graph");
+		const Plan &plan = ptr->prog->plan();
+		DeviceBuffers ds = import_device_buffer(src, src_frame_stride, plan.src_w, true);
+		DeviceBuffers dd = import_device_buffer(dst, dst_frame_stride, plan.dst_w, false);
+		const unsigned n = ptr->prog->autotune(ds, dd, tmp, count, stream);
+		if (changed)
+			*changed = n;
+		return ZIMG_ERROR_SUCCESS;
+	} catch (...) {
+		return handle_current_exception();
+	}
+}
+
 zimg_error_code_e zimg_filter_graph_process_device_batch_noalloc(const zimg_filter_graph *ptr, unsigned count,
                                                                  const zimg_image_buffer_const *src, const ptrdiff_t src_frame_stride[4],
                                                                  const zimg_image_buffer *dst, const ptrdiff_t dst_frame_stride[4],

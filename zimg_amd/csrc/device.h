@@ -107,6 +107,15 @@ public:
 	// events (optional, steps().size() + 1 hipEvent_t): recorded before each step and after the last.
 	void run(const DeviceBuffers &src, const DeviceBuffers &dst, void *tmp, unsigned frames, void *stream,
 	         void *const *events = nullptr) const;
+	// Times the cost model's candidate tile shapes of every tiled resize
+	// step on these buffers (the program runs whole, the step's own time is
+	// taken from HIP events around it on `stream`, minimum of 5 runs) and
+	// keeps the fastest for each step; a candidate replaces the current shape
+	// only when at least 2 % faster.
+	// Outputs do not change (every candidate computes the same values).
+	// Returns how many steps changed shape. Not thread-safe against
+	// concurrent processing of the program.
+	unsigned autotune(const DeviceBuffers &src, const DeviceBuffers &dst, void *tmp, unsigned frames, void *stream);
 	// Algorithmic bytes of one frame through each step: its input planes read
 	// once and its output planes written once.
 	std::vector<double> step_bytes() const;

@@ -1426,6 +1426,70 @@ dev::ResizeArg DeviceProgram::resize_arg(const Step &s, const dev::PlaneArg &src
 	return a;
 }
 
+unsigned DeviceProgram::autotune(const DeviceBuffers &src, const DeviceBuffers &dst, void *tmp, unsigned frames, void *stream)
+{
+	if (!frames)
+		return 0;
+	hipStream_t hs = static_cast<hipStream_t>(stream);
+	// One event before each step and one after the last (run()'s profiling
+	// form): a candidate is timed on its own step, the minimum of 5 runs.
+	std::vector<hipEvent_t> ev(m_steps.size() + 1, nullptr);
+	auto destroy = [&]() {
+		for (hipEvent_t e : ev)
+			if (e)
+				(void)hipEventDestroy(e);
+	};
+	unsigned changed = 0;
+	try {
+		for (hipEvent_t &e : ev)
+			check_hip(hipEventCreate(&e), "hipEventCreate");
+		auto time_step = [&](size_t si) -> float {
+			run(src, dst, tmp, frames, stream); // warm-up
+			float best = 0;
+			for (int r = 0; r < 5; ++r) {
+				run(src, dst, tmp, frames, stream, reinterpret_cast<void *const *>(ev.data()));
+				check_hip(hipEventSynchronize(ev[si + 1]), "hipEventSynchronize");
+				float ms = 0;
+				check_hip(hipEventElapsedTime(&ms, ev[si], ev[si + 1]), "hipEventElapsedTime");
+				best = r == 0 ? ms : std::min(best, ms);
+			}
+			check_hip(hipStreamSynchronize(hs), "hipStreamSynchronize");
+			return best;
+		};
+		auto same = [](const TiledConfig &x, const TiledConfig &y) {
+			return x.fwh == y.fwh && x.tile_w == y.tile_w && x.tile_h == y.tile_h && x.in_rows == y.in_rows && x.in_cols == y.in_cols &&
+			       x.mid_rows == y.mid_rows && x.mid_cols == y.mid_cols;
+		};
+		for (size_t si = 0; si < m_steps.size(); ++si) {
+			Step &s = m_steps[si];
+			// Steps on the matrix cores (precision form, integer WORD) do not use the tiled shape.
+			if (s.dead || s.kind != Step::RESIZE || !s.fast.fwh || s.fast_options.size() < 2 || s.imma.ok || mfma_step(s))
+				continue;
+			const TiledConfig orig = s.fast;
+			TiledConfig best = orig;
+			float best_ms = time_step(si);
+			for (const TiledConfig &c : s.fast_options) {
+				if (same(c, orig))
+					continue;
+				s.fast = c;
+				const float ms = time_step(si);
+				if (ms < best_ms * 0.98f) {
+					best = c;
+					best_ms = ms;
+				}
+			}
+			s.fast = best;
+			if (!same(best, orig))
+				++changed;
+		}
+	} catch (...) {
+		destroy();
+		throw;
+	}
+	destroy();
+	return changed;
+}
+
 std::vector<double> DeviceProgram::step_bytes() const
 {
 	auto elem = [](int type) { return type == 0 ? 1.0 : type == 3 ? 4.0 : 2.0; };
