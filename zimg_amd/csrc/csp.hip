@@ -29,8 +29,12 @@ namespace dev {
 
 namespace {
 
-// Pixels per thread of the colourspace kernels (consecutive blocks of kThreads columns).
+// Pixels per thread of the colourspace kernels (consecutive blocks of kThreads
+// columns): the run-time operation list, and the compile-time chains (their
+// straight-line code keeps more pixels' FP64 chains in flight in fewer
+// registers: 8 pixels measured 5 % faster than 4 on C2, 2 slower).
 constexpr int kCspPix = 4;
+constexpr int kCspChainPix = 8;
 
 template <int NP>
 __device__ __forceinline__ void op_matrix(const CspOp &op, float *a, float *b, float *c)
@@ -233,14 +237,15 @@ __device__ __forceinline__ void apply_slot(const CspOp &op, float *a, float *b, 
 		static_assert(K < 0, "operation kind without a chain form");
 }
 
-// The loads (with their depth conversion) of a thread's kCspPix pixels.
-__device__ __forceinline__ void csp_load(const CspArg &a, unsigned f, unsigned i, unsigned jb, float (*v)[kCspPix])
+// The loads (with their depth conversion) of a thread's NPX pixels.
+template <int NPX>
+__device__ __forceinline__ void csp_load(const CspArg &a, unsigned f, unsigned i, unsigned jb, float (*v)[NPX])
 {
 #pragma unroll
 	for (int p = 0; p < 3; ++p) {
 		const char *srow = row_ptr(frame_ptr(a.src[p], f), a.src[p].stride, i);
 #pragma unroll
-		for (int k = 0; k < kCspPix; ++k) {
+		for (int k = 0; k < NPX; ++k) {
 			const unsigned j = min(jb + k * kThreads, a.width - 1);
 			float x = load_as_float(srow, j, a.in[p].type);
 			if (a.in[p].op == 2)
@@ -250,13 +255,14 @@ __device__ __forceinline__ void csp_load(const CspArg &a, unsigned f, unsigned i
 	}
 }
 
-__device__ __forceinline__ void csp_store(const CspArg &a, unsigned f, unsigned i, unsigned jb, float (*v)[kCspPix])
+template <int NPX>
+__device__ __forceinline__ void csp_store(const CspArg &a, unsigned f, unsigned i, unsigned jb, float (*v)[NPX])
 {
 #pragma unroll
 	for (int p = 0; p < 3; ++p) {
 		char *drow = row_ptr(frame_ptr_w(a.dst[p], f), a.dst[p].stride, i);
 #pragma unroll
-		for (int k = 0; k < kCspPix; ++k) {
+		for (int k = 0; k < NPX; ++k) {
 			const unsigned j = jb + k * kThreads;
 			if (j >= a.width)
 				break;
@@ -277,10 +283,10 @@ __global__ void __launch_bounds__(kThreads) k_colorspace(CspArg a)
 	if (i >= a.height || jb >= a.width)
 		return;
 	float v[3][kCspPix];
-	csp_load(a, f, i, jb, v);
+	csp_load<kCspPix>(a, f, i, jb, v);
 	for (int k = 0; k < a.prog.n; ++k)
 		apply_op<kCspPix>(a.prog.op[k], v[0], v[1], v[2]);
-	csp_store(a, f, i, jb, v);
+	csp_store<kCspPix>(a, f, i, jb, v);
 }
 
 template <int K0, int F0, int K1, int F1, int K2, int F2, int K3, int F3, int K4, int F4>
@@ -289,17 +295,17 @@ __global__ void __launch_bounds__(kThreads) k_colorspace_chain(CspArg a)
 	zm::lds_tables_init();
 	const unsigned f = blockIdx.z;
 	const unsigned i = blockIdx.y;
-	const unsigned jb = blockIdx.x * (kThreads * kCspPix) + threadIdx.x;
+	const unsigned jb = blockIdx.x * (kThreads * kCspChainPix) + threadIdx.x;
 	if (i >= a.height || jb >= a.width)
 		return;
-	float v[3][kCspPix];
-	csp_load(a, f, i, jb, v);
-	if constexpr (K0 >= 0) apply_slot<kCspPix, K0, F0>(a.prog.op[0], v[0], v[1], v[2]);
-	if constexpr (K1 >= 0) apply_slot<kCspPix, K1, F1>(a.prog.op[1], v[0], v[1], v[2]);
-	if constexpr (K2 >= 0) apply_slot<kCspPix, K2, F2>(a.prog.op[2], v[0], v[1], v[2]);
-	if constexpr (K3 >= 0) apply_slot<kCspPix, K3, F3>(a.prog.op[3], v[0], v[1], v[2]);
-	if constexpr (K4 >= 0) apply_slot<kCspPix, K4, F4>(a.prog.op[4], v[0], v[1], v[2]);
-	csp_store(a, f, i, jb, v);
+	float v[3][kCspChainPix];
+	csp_load<kCspChainPix>(a, f, i, jb, v);
+	if constexpr (K0 >= 0) apply_slot<kCspChainPix, K0, F0>(a.prog.op[0], v[0], v[1], v[2]);
+	if constexpr (K1 >= 0) apply_slot<kCspChainPix, K1, F1>(a.prog.op[1], v[0], v[1], v[2]);
+	if constexpr (K2 >= 0) apply_slot<kCspChainPix, K2, F2>(a.prog.op[2], v[0], v[1], v[2]);
+	if constexpr (K3 >= 0) apply_slot<kCspChainPix, K3, F3>(a.prog.op[3], v[0], v[1], v[2]);
+	if constexpr (K4 >= 0) apply_slot<kCspChainPix, K4, F4>(a.prog.op[4], v[0], v[1], v[2]);
+	csp_store<kCspChainPix>(a, f, i, jb, v);
 }
 
 // Chains with a compile-time kernel: (kind, transfer) per slot, -1 past the end.
@@ -373,10 +379,11 @@ CspKernel chain_kernel(const CspProgram &prog)
 
 int launch_colorspace(const CspArg &a, void *stream)
 {
-	dim3 grid((a.width + kThreads * kCspPix - 1) / (kThreads * kCspPix), a.height, a.frames);
 	CspKernel k = chain_kernel(a.prog);
+	const unsigned px = k ? kCspChainPix : kCspPix;
 	if (!k)
 		k = k_colorspace;
+	dim3 grid((a.width + kThreads * px - 1) / (kThreads * px), a.height, a.frames);
 	CspArg arg = a;
 	void *args[] = { &arg };
 	return static_cast<int>(hipLaunchKernel(reinterpret_cast<const void *>(k), grid, dim3(kThreads), args, 0, static_cast<hipStream_t>(stream)));
