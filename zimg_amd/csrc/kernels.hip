@@ -608,18 +608,30 @@ struct HRegs {
 		v1 = a1;
 	}
 
-	__device__ __forceinline__ void apply_float(const float *row, int fw, float &v0, float &v1) const
+	// Calls body(apply) once, apply(row, v0, v1) being the sum form for fw.
+	// The branch on fw is taken here, outside the caller's row loop: inside
+	// it, the taps of the common forms sit at fixed offsets from one address
+	// per column, instead of one address per tap kept for the clamped form
+	// and advanced every row.
+	template <class Body>
+	__device__ __forceinline__ void with_float_form(int fw, Body body) const
 	{
 		if (fw == N) {
-			apply_float_n<N>(row, v0, v1);
+			body([this](const float *row, float &v0, float &v1) { apply_float_n<N>(row, v0, v1); });
 			return;
 		}
 		if constexpr (N > 2) {
 			if (fw == N - 1) {
-				apply_float_n<N - 1>(row, v0, v1);
+				body([this](const float *row, float &v0, float &v1) { apply_float_n<N - 1>(row, v0, v1); });
 				return;
 			}
 		}
+		body([this, fw](const float *row, float &v0, float &v1) { apply_float_sel(row, fw, v0, v1); });
+	}
+
+	// Any fw <= N: taps past fw leave the sums as they are.
+	__device__ __forceinline__ void apply_float_sel(const float *row, int fw, float &v0, float &v1) const
+	{
 		float a0 = 0.0f, a1 = 0.0f;
 #pragma unroll
 		for (int k = 0; k < N; ++k) {
@@ -853,11 +865,13 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 			if (a.passes == 2) {
 				float *mid = reinterpret_cast<float *>(mid_base);
 				const unsigned pm = a.lds_mid_cols;
-				for (unsigned r = 0; r < span; ++r) {
-					float v0, v1;
-					h.apply_float(xin + r * pin, ph.fw, v0, v1);
-					*reinterpret_cast<float2 *>(mid + r * pm + jc) = make_float2(v0, v1);
-				}
+				h.with_float_form(ph.fw, [&](auto apply) {
+					for (unsigned r = 0; r < span; ++r) {
+						float v0, v1;
+						apply(xin + r * pin, v0, v1);
+						*reinterpret_cast<float2 *>(mid + r * pm + jc) = make_float2(v0, v1);
+					}
+				});
 				__syncthreads();
 				const unsigned j4 = 4 * (t % (kThreads / 2));
 				const unsigned g = t / (kThreads / 2);
@@ -926,12 +940,14 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 					}
 				}
 			} else {
-				for (unsigned r = 0; r < span; ++r) {
-					float v0, v1;
-					h.apply_float(xin + r * pin, ph.fw, v0, v1);
-					if (active)
-						store_pair<false>(row_ptr(dst, a.dst.stride, r0 + r), j, v0, v1, two, a.out, r0 + r);
-				}
+				h.with_float_form(ph.fw, [&](auto apply) {
+					for (unsigned r = 0; r < span; ++r) {
+						float v0, v1;
+						apply(xin + r * pin, v0, v1);
+						if (active)
+							store_pair<false>(row_ptr(dst, a.dst.stride, r0 + r), j, v0, v1, two, a.out, r0 + r);
+					}
+				});
 			}
 		}
 		return;
@@ -1202,17 +1218,22 @@ __device__ __forceinline__ void resize_tile(const ResizeArg &a, unsigned bx, uns
 	// Horizontal pass from the intermediate rows.
 	if (og >= ogroups)
 		return;
-	for (unsigned i = og; i < nrows; i += ogroups) {
-		const unsigned row = i0 + i;
-		if constexpr (WORD) {
+	if constexpr (WORD) {
+		for (unsigned i = og; i < nrows; i += ogroups) {
+			const unsigned row = i0 + i;
 			int32_t a0, a1;
 			h.apply_word(reinterpret_cast<const uint32_t *>(mid_base) + i * (pm / 2), a0, a1, kWordInit);
 			store_pair<true>(row_ptr(dst, a.dst.stride, row), oj, pack_word_pre(a0, a.pixel_max), pack_word_pre(a1, a.pixel_max), otwo, a.out, row);
-		} else {
-			float v0, v1;
-			h.apply_float(reinterpret_cast<const float *>(mid_base) + i * pm, a.p2.fw, v0, v1);
-			store_pair<false>(row_ptr(dst, a.dst.stride, row), oj, v0, v1, otwo, a.out, row);
 		}
+	} else {
+		h.with_float_form(a.p2.fw, [&](auto apply) {
+			for (unsigned i = og; i < nrows; i += ogroups) {
+				const unsigned row = i0 + i;
+				float v0, v1;
+				apply(reinterpret_cast<const float *>(mid_base) + i * pm, v0, v1);
+				store_pair<false>(row_ptr(dst, a.dst.stride, row), oj, v0, v1, otwo, a.out, row);
+			}
+		});
 	}
 }
 
