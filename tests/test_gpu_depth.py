@@ -150,6 +150,26 @@ def test_half_yuv_to_half_rgb():
         _assert_bits(outs[q], to_half(want, FLOAT), f"plane {q}")
 
 
+@pytest.mark.parametrize("w,out", [(640, "byte"), (641, "byte"), (640, "word"), (643, "word")])
+def test_error_diffusion_frames_widths(w, out):
+    """FLOAT -> BYTE 8-bit / WORD 12-bit error diffusion at even and odd
+    widths (odd: the last column pair of a row is half outside it), values
+    past both ends of the range; 200 rows end in a strip of 8 rows, and two
+    frames share one launch; every frame against the sequential oracle."""
+    h, F = 200, 2
+    print(f"case: GREY FLOAT {w}x{h} x{F} frames -> {out} error diffusion", flush=True)
+    rng = np.random.default_rng(29)
+    src = rng.random((F, h, w), dtype=np.float32) * 1.2 - 0.1
+    ptype, depth, dt, otype = (z.PIXEL_BYTE, 8, np.uint8, BYTE) if out == "byte" else (z.PIXEL_WORD, 12, np.uint16, WORD)
+    g = z.FilterGraph(_grey(w, h, z.PIXEL_FLOAT, 32), _grey(w, h, ptype, depth, pixel_range=z.RANGE_FULL),
+                      z.builder_params(dither_type=z.DITHER_ERROR_DIFFUSION))
+    (o, *_) = gpu_util.run_graph(g, [src, None, None, None], [(h, w, dt), None, None, None])
+    sc, of = scale_offset((FLOAT, 32, 1, 0, 0), (otype, depth, 1, 0, 0))
+    for f in range(F):
+        want = dither(src[f], FLOAT, otype, sc, of, depth, dither_type=3, plane=0)
+        _assert_bits(o[f], want, f"error diffusion {w} wide, frame {f}")
+
+
 @pytest.mark.parametrize("w", [20000, 32639])
 def test_error_diffusion_wide_planes(w):
     """Planes wider than a 1024-entry LDS ring per wave (up to the 32639-sample

@@ -1274,7 +1274,7 @@ typedef int32_t imma_v4i __attribute__((ext_vector_type(4)));
 // The kernel asks for at least 4 waves per SIMD (at most 128 VGPRs):
 // otherwise the compiler keeps the matrix results in AGPRs and copies each
 // one out.
-__device__ __forceinline__ int32_t imma_pack(int32_t hh, int32_t mid, int32_t ll, int32_t pixel_max)
+__device__ __forceinline__ int32_t imma_sum(int32_t hh, int32_t mid, int32_t ll)
 {
 	// ((hh << 8) + mid) << 8 + ll = 65536 hh + 256 mid + ll in wrapping 32-bit
 	// arithmetic (the true sum fits), as two shift-adds: the empty asm keeps
@@ -1283,8 +1283,18 @@ __device__ __forceinline__ int32_t imma_pack(int32_t hh, int32_t mid, int32_t ll
 	// the compiler inserts the wait states those reads need.)
 	uint32_t t = (static_cast<uint32_t>(hh) << 8) + static_cast<uint32_t>(mid);
 	asm("" : "+v"(t));
-	const uint32_t s = (t << 8) + static_cast<uint32_t>(ll);
-	return med3_i32(static_cast<int32_t>(s) >> 14, 0, pixel_max);
+	return static_cast<int32_t>((t << 8) + static_cast<uint32_t>(ll));
+}
+__device__ __forceinline__ int32_t imma_pack(int32_t hh, int32_t mid, int32_t ll, int32_t pixel_max)
+{
+	return med3_i32(imma_sum(hh, mid, ll) >> 14, 0, pixel_max);
+}
+// Full 16-bit range (pixel_max 65535), accumulators started 2^29 lower (the
+// 32768 output bias taken out): two results as biased int16, y - 32768 =
+// sat16(s >> 14) (v_cvt_pk_i16_i32), low half first.
+__device__ __forceinline__ uint32_t imma_pack2_biased(const imma_v4i &hh, const imma_v4i &md, const imma_v4i &ll, int i)
+{
+	return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pk_i16(imma_sum(hh[i], md[i], ll[i]) >> 14, imma_sum(hh[i + 1], md[i + 1], ll[i + 1]) >> 14));
 }
 
 // The kernel is persistent: gridDim.x workgroups (a multiple of 8) walk the
@@ -1303,9 +1313,12 @@ struct ImmaPrefetch {
 	unsigned bx, by, f;
 };
 
-template <int MB>
+// FULL: pixel_max is 65535, so the clamp of both passes is the saturation of
+// a biased int16 pack (imma_pack2_biased).
+template <int MB, bool FULL>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) k_resize_imma(ImmaArg a)
 {
+	constexpr int32_t kInitShift = FULL ? (1 << 29) : 0;
 	extern __shared__ __align__(16) unsigned char smem[];
 	const unsigned lane = threadIdx.x & 63;
 	const unsigned w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1473,16 +1486,26 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 					long Bh, Bm, Bl;
 					operands(cur.hc[q], Bh, Bm, Bl);
 					const imma_v4i zero = { 0, 0, 0, 0 };
-					const imma_v4i init = { cur.hinit[q], cur.hinit[q], cur.hinit[q], cur.hinit[q] };
+					const int32_t hi0 = cur.hinit[q] - kInitShift;
+					const imma_v4i init = { hi0, hi0, hi0, hi0 };
 					const imma_v4i hh = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bh, zero, 0, 0, 0);
 					const imma_v4i md = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bm, zero, 0, 0, 0);
 					const imma_v4i ll = __builtin_amdgcn_mfma_i32_16x16x32_i8(A, Bl, init, 0, 0, 0);
-					const uint32_t y0 = imma_pack(hh[0], md[0], ll[0], a.pixel_max), y1 = imma_pack(hh[1], md[1], ll[1], a.pixel_max);
-					const uint32_t y2 = imma_pack(hh[2], md[2], ll[2], a.pixel_max), y3 = imma_pack(hh[3], md[3], ll[3], a.pixel_max);
-					const uint32_t p01 = __builtin_amdgcn_perm(y1, y0, 0x05010400u); // y0.b0 y1.b0 y0.b1 y1.b1
-					const uint32_t p23 = __builtin_amdgcn_perm(y3, y2, 0x05010400u);
-					const uint32_t lo = __builtin_amdgcn_perm(p23, p01, 0x05040100u) ^ 0x80808080u;
-					const uint32_t hi = __builtin_amdgcn_perm(p23, p01, 0x07060302u) ^ 0x80808080u;
+					uint32_t lo, hi;
+					if constexpr (FULL) {
+						// bytes of y - 32768: the high byte is already the signed
+						// high byte, the low byte takes ^ 0x80
+						const uint32_t p01 = imma_pack2_biased(hh, md, ll, 0), p23 = imma_pack2_biased(hh, md, ll, 2);
+						lo = __builtin_amdgcn_perm(p23, p01, 0x06040200u) ^ 0x80808080u;
+						hi = __builtin_amdgcn_perm(p23, p01, 0x07050301u);
+					} else {
+						const uint32_t y0 = imma_pack(hh[0], md[0], ll[0], a.pixel_max), y1 = imma_pack(hh[1], md[1], ll[1], a.pixel_max);
+						const uint32_t y2 = imma_pack(hh[2], md[2], ll[2], a.pixel_max), y3 = imma_pack(hh[3], md[3], ll[3], a.pixel_max);
+						const uint32_t p01 = __builtin_amdgcn_perm(y1, y0, 0x05010400u); // y0.b0 y1.b0 y0.b1 y1.b1
+						const uint32_t p23 = __builtin_amdgcn_perm(y3, y2, 0x05010400u);
+						lo = __builtin_amdgcn_perm(p23, p01, 0x05040100u) ^ 0x80808080u;
+						hi = __builtin_amdgcn_perm(p23, p01, 0x07060302u) ^ 0x80808080u;
+					}
 					const unsigned off = (16 * (2 * w + q) + (lane & 15)) * a.mid_pitch + 16 * m + 4 * (lane >> 4);
 					*reinterpret_cast<uint32_t *>(midh + off) = hi;
 					*reinterpret_cast<uint32_t *>(midl + off) = lo;
@@ -1508,7 +1531,8 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 			long Bh, Bm, Bl;
 			operands(vc[k], Bh, Bm, Bl);
 			const imma_v4i zero = { 0, 0, 0, 0 };
-			const imma_v4i init = { vinit[k], vinit[k], vinit[k], vinit[k] };
+			const int32_t vi0 = vinit[k] - kInitShift;
+			const imma_v4i init = { vi0, vi0, vi0, vi0 };
 			// Column block c + 1 is multiplied before block c is packed into the
 			// wave's LDS copy of its 16 output rows.
 			imma_v4i acc[2][3];
@@ -1527,9 +1551,15 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 				if (c + 1 < kImmaTileW / 16)
 					mul(c + 1, acc[(c + 1) & 1]);
 				const imma_v4i hh = acc[c & 1][0], md = acc[c & 1][1], ll = acc[c & 1][2];
-				const uint32_t y0 = imma_pack(hh[0], md[0], ll[0], a.pixel_max), y1 = imma_pack(hh[1], md[1], ll[1], a.pixel_max);
-				const uint32_t y2 = imma_pack(hh[2], md[2], ll[2], a.pixel_max), y3 = imma_pack(hh[3], md[3], ll[3], a.pixel_max);
-				*reinterpret_cast<uint2 *>(ot + (lane & 15) * kImmaOutPitch + 16 * c + 4 * (lane >> 4)) = make_uint2(pack16x2(y0, y1), pack16x2(y2, y3));
+				uint2 out;
+				if constexpr (FULL) {
+					out = make_uint2(imma_pack2_biased(hh, md, ll, 0) ^ 0x80008000u, imma_pack2_biased(hh, md, ll, 2) ^ 0x80008000u);
+				} else {
+					const uint32_t y0 = imma_pack(hh[0], md[0], ll[0], a.pixel_max), y1 = imma_pack(hh[1], md[1], ll[1], a.pixel_max);
+					const uint32_t y2 = imma_pack(hh[2], md[2], ll[2], a.pixel_max), y3 = imma_pack(hh[3], md[3], ll[3], a.pixel_max);
+					out = make_uint2(pack16x2(y0, y1), pack16x2(y2, y3));
+				}
+				*reinterpret_cast<uint2 *>(ot + (lane & 15) * kImmaOutPitch + 16 * c + 4 * (lane >> 4)) = out;
 			}
 			__builtin_amdgcn_wave_barrier();
 			// Whole row segments out: 16 lanes per row, 16 bytes each.
@@ -1949,7 +1979,10 @@ int launch_resize_imma(const ImmaArg &a, void *stream)
 	switch (a.mb) {
 #define ZAMD_IMMA(n) \
 	case n: \
-		hipLaunchKernelGGL((k_resize_imma<n>), dim3(static_cast<unsigned>(grid)), dim3(kThreads), a.lds, s, a); \
+		if (a.pixel_max == 65535) \
+			hipLaunchKernelGGL((k_resize_imma<n, true>), dim3(static_cast<unsigned>(grid)), dim3(kThreads), a.lds, s, a); \
+		else \
+			hipLaunchKernelGGL((k_resize_imma<n, false>), dim3(static_cast<unsigned>(grid)), dim3(kThreads), a.lds, s, a); \
 		break;
 	ZAMD_IMMA(1) ZAMD_IMMA(2) ZAMD_IMMA(3) ZAMD_IMMA(4) ZAMD_IMMA(5) ZAMD_IMMA(6)
 #undef ZAMD_IMMA
