@@ -293,11 +293,13 @@ __device__ __forceinline__ uint16_t pack_word_pre(int32_t acc, int32_t pixel_max
 	return static_cast<uint16_t>(med3_i32(acc >> 14, 0, pixel_max));
 }
 
-// Shift every 16-bit half of w left by s (s < 16), then bias to int16.
-__device__ __forceinline__ uint32_t shift_bias2(uint32_t w, unsigned s)
+// Shift every 16-bit half of w left by s (s < 16), then bias to int16 (x ^
+// 0x8000 = x - 32768; the matrix-core kernel passes 0x80808080, which also
+// turns each low byte into the signed byte low - 128).
+__device__ __forceinline__ uint32_t shift_bias2(uint32_t w, unsigned s, uint32_t bias = 0x80008000u)
 {
 	const uint32_t m = (0xFFFFu << s) & 0xFFFFu;
-	return ((w << s) & (m | (m << 16))) ^ 0x80008000u;
+	return ((w << s) & (m | (m << 16))) ^ bias;
 }
 
 // Stage source rows [r0, r0 + nr) x columns [c0, c0 + nc) into LDS with row
@@ -1252,11 +1254,12 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 //
 // A workgroup owns kImmaTileW output columns and 16 rb output rows. It
 // stages its source window (16 MB rows from the first vertical window, the
-// columns of its 8 horizontal windows) in LDS as biased samples, then:
+// columns of its 8 horizontal windows) in LDS as samples x ^ 0x8080 (both
+// bytes signed, kImmaBias), then:
 // horizontal pass: wave w takes column blocks 2w, 2w + 1 for each of the MB
 // 16-row blocks; A = 16 rows x [xh | xl'] (lane l: row l % 16, window
 // samples 4 (l / 16 % 2) + 0..3 and + 8..11, high bytes for l < 32, low
-// bytes ^ 0x80 else; the coefficient table uses the same order). The
+// bytes else; the coefficient table uses the same order). The
 // result (lane: rows 4 (l / 16) + r of column l % 16) is stored as bytes
 // (x ^ 0x80) into two LDS planes, high and low, one column after another
 // (mid_pitch bytes each), so a column's rows are contiguous. Each wave then
@@ -1270,6 +1273,10 @@ __global__ void __launch_bounds__(kThreads) k_resize_tiled(ResizeArg a)
 // second group of 16 lanes on odd banks. All coefficients are loaded before
 // the staging, so no memory round trip sits between the passes.
 typedef int32_t imma_v4i __attribute__((ext_vector_type(4)));
+// Staged samples x are stored as x ^ 0x8080: the high byte is the signed high
+// byte of x - 32768 and the low byte the signed low - 128, both ready as
+// matrix operands.
+constexpr uint32_t kImmaBias = 0x80808080u;
 
 // The kernel asks for at least 4 waves per SIMD (at most 128 VGPRs):
 // otherwise the compiler keeps the matrix results in AGPRs and copies each
@@ -1406,13 +1413,14 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 		uint16_t *d = xin + static_cast<size_t>(sr) * a.in_pitch + sc;
 		const uint32_t wv[4] = { v.x, v.y, v.z, v.w };
 		if (esz == 2) {
-			*reinterpret_cast<uint4 *>(d) = make_uint4(shift_bias2(wv[0], sh), shift_bias2(wv[1], sh), shift_bias2(wv[2], sh), shift_bias2(wv[3], sh));
+			*reinterpret_cast<uint4 *>(d) = make_uint4(shift_bias2(wv[0], sh, kImmaBias), shift_bias2(wv[1], sh, kImmaBias), shift_bias2(wv[2], sh, kImmaBias),
+			                                           shift_bias2(wv[3], sh, kImmaBias));
 		} else {
 			uint32_t o[8];
 #pragma unroll
 			for (int k = 0; k < 4; ++k) {
-				o[2 * k] = shift_bias2((wv[k] & 0xFFu) | ((wv[k] & 0xFF00u) << 8), sh);
-				o[2 * k + 1] = shift_bias2(((wv[k] >> 16) & 0xFFu) | ((wv[k] >> 8) & 0xFF0000u), sh);
+				o[2 * k] = shift_bias2((wv[k] & 0xFFu) | ((wv[k] & 0xFF00u) << 8), sh, kImmaBias);
+				o[2 * k + 1] = shift_bias2(((wv[k] >> 16) & 0xFFu) | ((wv[k] >> 8) & 0xFF0000u), sh, kImmaBias);
 			}
 			reinterpret_cast<uint4 *>(d)[0] = make_uint4(o[0], o[1], o[2], o[3]);
 			reinterpret_cast<uint4 *>(d)[1] = make_uint4(o[4], o[5], o[6], o[7]);
@@ -1469,7 +1477,6 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 		// Horizontal pass.
 		{
 			const uint32_t sel = lane < 32 ? 0x07050301u : 0x06040200u; // high or low bytes of 4 samples
-			const uint32_t lx = lane < 32 ? 0u : 0x80808080u;           // low byte - 128 = low byte ^ 0x80
 			const uint32_t *xw = reinterpret_cast<const uint32_t *>(xin);
 			const unsigned pw = a.in_pitch / 2;
 #pragma unroll
@@ -1481,7 +1488,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 					// the two groups of a half-wave read different banks of a row
 					const uint32_t *x = row + (cur.hkb[q] - cur.cb) / 2 + 2 * half;
 					const uint32_t d0 = x[0], d1 = x[1], d4 = x[4], d5 = x[5];
-					const uint32_t ax = __builtin_amdgcn_perm(d1, d0, sel) ^ lx, ay = __builtin_amdgcn_perm(d5, d4, sel) ^ lx;
+					const uint32_t ax = __builtin_amdgcn_perm(d1, d0, sel), ay = __builtin_amdgcn_perm(d5, d4, sel);
 					const long A = static_cast<long>((static_cast<uint64_t>(ay) << 32) | ax);
 					long Bh, Bm, Bl;
 					operands(cur.hc[q], Bh, Bm, Bl);
