@@ -91,6 +91,28 @@ def test_word_resize_integer_matrix_cores(sw, sh, dw, dh, ft, a, b, depth, frame
         _assert_equal(out[k], P.word_resize(src[k], dw, dh, ft, a, b, depth), f"frame {k}")
 
 
+@pytest.mark.parametrize("depth", [16, 12])
+def test_integer_matrix_cores_clamp_at_both_ends(depth):
+    """Full-range step edges (blocks of 0 and the largest code, single-sample
+    spikes) make Lanczos3 ring past both ends of the range in the horizontal
+    intermediate and in the output: 16-bit takes the saturating int16 pack,
+    12-bit the clamp to [0, pixel_max]; bit-exact against the oracle."""
+    sw, sh, dw, dh = 300, 170, 600, 340
+    print(f"case: WORD int8 matrix cores clamp {sw}x{sh}->{dw}x{dh} depth {depth}", flush=True)
+    top = (1 << depth) - 1
+    yy, xx = np.mgrid[0:sh, 0:sw]
+    src = np.where(((yy // 7) + (xx // 5)) % 2 == 0, top, 0).astype(np.uint16)
+    rng = np.random.default_rng(depth)
+    spikes = rng.random((sh, sw)) < 0.02
+    src[spikes] = top - src[spikes]
+    g = _word_graph(sw, sh, dw, dh, LANCZOS, 3.0, float("nan"), depth)
+    assert "integer matrix cores" in g.describe()
+    out, *_ = gpu_util.run_graph(g, [src, None, None, None], [(dh, dw, np.uint16), None, None, None], pitch_align=16)
+    want = P.word_resize(src, dw, dh, LANCZOS, 3.0, 0.0, depth)
+    assert (want == 0).any() and (want == top).any(), "the pattern reaches both clamps"
+    _assert_equal(out, want, f"clamp depth {depth}")
+
+
 @pytest.mark.parametrize("sw,sh,dw,dh,frames", [(333, 200, 700, 419, 2), (16, 20, 61, 47, 1)])
 def test_byte_to_word_upscale_integer_matrix_cores(sw, sh, dw, dh, frames):
     """BYTE 8-bit promoted to WORD 16-bit by the left shift folded into the
