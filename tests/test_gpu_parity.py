@@ -113,6 +113,27 @@ def test_integer_matrix_cores_clamp_at_both_ends(depth):
     _assert_equal(out, want, f"clamp depth {depth}")
 
 
+@pytest.mark.parametrize("depth", [16, 10])
+def test_tiled_word_downscale_clamps_at_both_ends(depth):
+    """The same full-range step edges through a 2x Lanczos3 downscale on the
+    tiled dot2 kernel (vertical pass first, row pairs): the ringing reaches 0
+    and the largest code in the intermediate and the output."""
+    sw, sh, dw, dh = 602, 342, 301, 171
+    print(f"case: WORD tiled clamp {sw}x{sh}->{dw}x{dh} depth {depth}", flush=True)
+    top = (1 << depth) - 1
+    yy, xx = np.mgrid[0:sh, 0:sw]
+    src = np.where(((yy // 9) + (xx // 7)) % 2 == 0, top, 0).astype(np.uint16)
+    rng = np.random.default_rng(depth + 1)
+    spikes = rng.random((sh, sw)) < 0.02
+    src[spikes] = top - src[spikes]
+    g = _word_graph(sw, sh, dw, dh, LANCZOS, 3.0, float("nan"), depth)
+    assert "integer matrix cores" not in g.describe()
+    out, *_ = gpu_util.run_graph(g, [src, None, None, None], [(dh, dw, np.uint16), None, None, None], pitch_align=16)
+    want = P.word_resize(src, dw, dh, LANCZOS, 3.0, 0.0, depth)
+    assert (want == 0).any() and (want == top).any(), "the pattern reaches both clamps"
+    _assert_equal(out, want, f"clamp depth {depth}")
+
+
 @pytest.mark.parametrize("sw,sh,dw,dh,frames", [(333, 200, 700, 419, 2), (16, 20, 61, 47, 1)])
 def test_byte_to_word_upscale_integer_matrix_cores(sw, sh, dw, dh, frames):
     """BYTE 8-bit promoted to WORD 16-bit by the left shift folded into the
